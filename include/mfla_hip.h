@@ -1,0 +1,216 @@
+/*
+ * mfla_hip.h -- C ABI of the MI355X (gfx950) Float64xN dense linear-algebra
+ * library, a drop-in device backend for MultiFloatLinearAlgebra.jl (MFLA).
+ *
+ * Each entry point replaces one MFLA generic function for device-resident
+ * Float64xN (N = 1..4) matrices; the reference function is cited beside it.
+ * A Julia wrapper binds these with ccall; the Python
+ * package multifloatlinearalgebra.jl_amd/mfla binds them with ctypes.
+ *
+ * Data model. A Float64xN matrix is LIMB-PLANAR: N planes of binary64, each a
+ * column-major rows x cols matrix with leading dimension `ld`; plane l starts
+ * `limb_stride` elements after plane 0. Element (i, j) (zero-based) limb l is
+ * data[l*limb_stride + j*ld + i]. All pointers in mfla_matrix are device
+ * pointers. Scalars (alpha, beta) are host arrays of N binary64 limbs.
+ *
+ * Conventions (MFLA's):
+ *  - Status codes follow MFLA: 0 success, -1 non-finite input, k > 0 the
+ *    one-based column/pivot where the factorization stopped. They are returned
+ *    through `int64_t *info` (host pointer). Pivot vectors are one-based int64.
+ *  - Return value: MFLA_OK or a negative error mirroring the Julia exception
+ *    the reference throws before touching memory: MFLA_ERR_DIMENSION
+ *    (DimensionMismatch), MFLA_ERR_ARGUMENT (ArgumentError: bad symbol,
+ *    unsupported N), MFLA_ERR_ALIAS (ArgumentError: output aliases an input),
+ *    MFLA_ERR_FAILED_FACTOR (PosDefException / SingularException on solve).
+ *  - beta == 0 means the output is not read.
+ *  - Calls are stream-ordered on the handle's stream. Factorizations and
+ *    backward_error synchronise the stream before returning (they report
+ *    host-visible status); the BLAS-like calls are asynchronous.
+ *  - Every reduction runs in ascending index order with a fixed
+ *    multi-double network: results are deterministic and independent of
+ *    the launch configuration.
+ */
+#ifndef MFLA_HIP_H
+#define MFLA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFLA_OK 0
+#define MFLA_ERR_ARGUMENT (-1)
+#define MFLA_ERR_DIMENSION (-2)
+#define MFLA_ERR_ALIAS (-3)
+#define MFLA_ERR_HIP (-4)
+#define MFLA_ERR_FAILED_FACTOR (-5)
+#define MFLA_ERR_WORKSPACE (-6)
+
+/* side / uplo / trans / diag symbols (MFLA's :left/:right, :lower/:upper,
+ * :N/:T, :nonunit/:unit). MFLA_GENERAL is residual!'s uplo=:general. */
+#define MFLA_LEFT 0
+#define MFLA_RIGHT 1
+#define MFLA_LOWER 0
+#define MFLA_UPPER 1
+#define MFLA_GENERAL 2
+#define MFLA_NOTRANS 0
+#define MFLA_TRANS 1
+#define MFLA_NONUNIT 0
+#define MFLA_UNIT 1
+
+/* gemm_strategy (src/kernels/gemm.jl:27-34) */
+#define MFLA_GEMM_AUTO 0
+#define MFLA_GEMM_DIRECT 1
+#define MFLA_GEMM_PACKED 2
+#define MFLA_GEMM_FUSED 3
+/* ldlt_strategy (src/factorizations/ldlt.jl:329-350) */
+#define MFLA_LDLT_AUTO 0
+#define MFLA_LDLT_UNBLOCKED 1
+#define MFLA_LDLT_BLOCKED 2
+/* backward-substitution order */
+#define MFLA_TRSM_ORDER_AUTO 0
+#define MFLA_TRSM_ORDER_EXACT 1
+#define MFLA_TRSM_ORDER_BLOCKED 2
+
+typedef struct mfla_matrix {
+    double *data;        /* device pointer: limb 0 of element (0, 0) */
+    int64_t rows;
+    int64_t cols;
+    int64_t ld;          /* column stride within a limb plane (elements) */
+    int64_t limb_stride; /* distance between limb planes (elements) */
+} mfla_matrix;
+
+/* KernelConfig (src/types.jl:13-27). CPU-only fields (reduction_tile,
+ * thread_count, gemm_panel_columns, gemm_micro_columns) are accepted and
+ * ignored. gemm_tile selects the GEMM tile shape (-1 = auto). */
+typedef struct mfla_config {
+    int64_t reduction_tile;
+    int64_t column_tile;
+    int64_t cholesky_block;
+    int64_t lu_block;
+    int64_t ldlt_block;            /* 0: MFLA default 16/12/8 by limb count */
+    int32_t ldlt_strategy;         /* MFLA_LDLT_* */
+    int32_t gemm_strategy;         /* MFLA_GEMM_* */
+    int64_t ldlt_blocked_crossover;
+    int64_t thread_count;
+    int64_t gemm_panel_columns;
+    int64_t gemm_micro_columns;
+    int32_t trsm_order;            /* MFLA_TRSM_ORDER_*; auto = exact for n <= 256 */
+    int32_t gemm_tile;             /* -1 auto, 0: 16x16, 1: 32x32 output tiles */
+} mfla_config;
+
+typedef struct mfla_handle_s *mfla_handle;
+typedef struct mfla_workspace_s *mfla_workspace;
+
+/* ----------------------------------------------------------- library */
+int mfla_version(void);
+const char *mfla_last_error(void);                 /* thread-local message */
+void mfla_config_default(mfla_config *config);     /* KernelConfig() */
+
+int mfla_handle_create(mfla_handle *out);           /* current device, default stream */
+int mfla_handle_destroy(mfla_handle h);
+int mfla_handle_set_stream(mfla_handle h, void *hip_stream);
+int mfla_synchronize(mfla_handle h);
+
+/* MFWorkspace (src/types.jl:71-123): caller-owned device scratch for LU/LDLT
+ * metadata and the blocked-LDLT weighted panel. */
+int mfla_workspace_create(mfla_handle h, int limbs, int64_t factor_capacity, int64_t ldlt_block_capacity, mfla_workspace *out);
+int mfla_workspace_destroy(mfla_workspace ws);
+/* workspace_capacity (types.jl:131-145): out[0] factor, out[1] ldlt_block */
+int mfla_workspace_capacity(mfla_workspace ws, int64_t *out2);
+/* ensure_workspace_capacity! (types.jl:157-192): grows, never shrinks */
+int mfla_workspace_ensure(mfla_workspace ws, int64_t factor_capacity, int64_t ldlt_block_capacity);
+
+/* Elementwise Float64xN networks over planar vectors (limb stride = count):
+ * op 0 x+y, 1 x-y, 2 x*y, 3 x/y, 4 sqrt(x), 5 z+x*y, 6 mulacc_x3(z, x, y)
+ * (src/kernels/mulacc_x3.jl:61-66, limbs == 3 only), 7 inv(x), 8 z-x*y,
+ * 9 abs(x), 10 max(x, y), 11 min(x, y), 12 (x < y). mfla_arith runs the
+ * device networks on device pointers; mfla_host_arith runs the same networks
+ * on the host (used for O(n) factor diagnostics). These are MultiFloats.jl's
+ * scalar operators (+ - * / sqrt inv abs max min <) for the same element
+ * types; they have no single MFLA counterpart. */
+int mfla_arith(mfla_handle h, int limbs, int op, int64_t count, const double *x, const double *y, const double *z, double *out);
+int mfla_host_arith(int limbs, int op, int64_t count, const double *x, const double *y, const double *z, double *out);
+
+/* max |A| over a region (0 all, 1 lower i >= j, 2 upper i <= j) into the host
+ * limbs out[0..limbs-1] (_maximum_abs / _lower_maximum_abs, src/types.jl:317-331;
+ * the U scan of src/diagnostics.jl:74-81), and the finiteness scans
+ * _all_finite / _lower_triangle_finite (src/types.jl:298-315). */
+int mfla_absmax(mfla_handle h, int limbs, int region, mfla_matrix A, double *out);
+int mfla_all_finite(mfla_handle h, int limbs, int region, mfla_matrix A, int *out);
+
+/* AoS (Julia Matrix{Float64xN}, limbs interleaved) <-> limb-planar copies */
+int mfla_from_interleaved(mfla_handle h, int limbs, mfla_matrix dst, const double *src_dev, int64_t src_ld);
+int mfla_to_interleaved(mfla_handle h, int limbs, double *dst_dev, int64_t dst_ld, mfla_matrix src);
+
+/* ------------------------------------------------------------ BLAS 3 */
+/* gemm! (src/kernels/gemm.jl:747-774): C = alpha*A*B + beta*C */
+int mfla_gemm(mfla_handle h, int limbs, mfla_matrix C, mfla_matrix A, mfla_matrix B, const double *alpha, const double *beta,
+              const mfla_config *config);
+/* syrk! (src/kernels/syrk.jl:67-108): lower C = alpha*P^T*P + beta*C, P is k x n */
+int mfla_syrk(mfla_handle h, int limbs, int uplo, mfla_matrix C, mfla_matrix P, const double *alpha, const double *beta,
+              const mfla_config *config);
+/* gemmt! (src/kernels/gemmt.jl:69-108): lower C = alpha*L*R^T + beta*C */
+int mfla_gemmt(mfla_handle h, int limbs, mfla_matrix C, mfla_matrix L, mfla_matrix R, const double *alpha, const double *beta,
+               const mfla_config *config);
+/* trsm! (src/kernels/trsm.jl:224-302): op(A) X = alpha B or X op(A) = alpha B, in place */
+int mfla_trsm(mfla_handle h, int limbs, int side, int uplo, int trans, int diag, mfla_matrix B, mfla_matrix A, const double *alpha,
+              const mfla_config *config);
+
+/* ------------------------------------------------------------ BLAS 2 */
+/* trsv! (src/kernels/trsv.jl:19-73): x is an n x 1 matrix; same order as trsm! */
+int mfla_trsv(mfla_handle h, int limbs, int uplo, int trans, int diag, mfla_matrix x, mfla_matrix A, const double *alpha,
+              const mfla_config *config);
+/* gemv! (src/kernels/gemv.jl:111-167): y = alpha*op(A)*x + beta*y */
+int mfla_gemv(mfla_handle h, int limbs, int trans, mfla_matrix y, mfla_matrix A, mfla_matrix x, const double *alpha, const double *beta,
+              const mfla_config *config);
+/* symv! (src/kernels/symv.jl:53-89): y = alpha*A*x + beta*y from the uplo triangle */
+int mfla_symv(mfla_handle h, int limbs, int uplo, mfla_matrix y, mfla_matrix A, mfla_matrix x, const double *alpha, const double *beta,
+              const mfla_config *config);
+
+/* ---------------------------------------------------- factorizations */
+/* cholesky! (src/factorizations/cholesky.jl:68-107): lower, in place */
+int mfla_potrf(mfla_handle h, int limbs, mfla_matrix A, int64_t *info, const mfla_config *config);
+/* lu! (src/factorizations/lu.jl:128-195): ipiv_dev has min(m, n) int64 */
+int mfla_getrf(mfla_handle h, int limbs, mfla_matrix A, int64_t *ipiv_dev, double *original_maximum, int64_t *info,
+               const mfla_config *config, mfla_workspace ws);
+/* ldlt! (src/factorizations/ldlt.jl:654-717): dsub is an n x 1 device matrix;
+ * pivots_dev (int64) and blocks_dev (uint8) have n entries */
+int mfla_sytrf(mfla_handle h, int limbs, mfla_matrix A, mfla_matrix dsub, int64_t *pivots_dev, uint8_t *blocks_dev,
+               double *original_maximum, int64_t *info, const mfla_config *config, mfla_workspace ws);
+
+/* ------------------------------------------------------------ solves */
+/* ldiv!(B, F::MFCholesky) (src/solve.jl:23-83) */
+int mfla_potrs(mfla_handle h, int limbs, mfla_matrix F, int64_t info, mfla_matrix B, const mfla_config *config);
+/* ldiv!(B, F::MFLU) (src/solve.jl:115-181) */
+int mfla_getrs(mfla_handle h, int limbs, mfla_matrix F, const int64_t *ipiv_dev, int64_t info, mfla_matrix B,
+               const mfla_config *config);
+/* ldiv!(B, F::MFLDLT) (src/solve.jl:298-365) */
+int mfla_sytrs(mfla_handle h, int limbs, mfla_matrix F, mfla_matrix dsub, const int64_t *pivots_dev, const uint8_t *blocks_dev,
+               int64_t info, mfla_matrix B, const mfla_config *config);
+
+/* ------------------------------------------------ residual / refinement */
+/* residual!(R, A, X, B) for matrix right-hand sides (src/residual.jl:82-125,
+ * the gemm!/symv! path): R = B - A*X; R may be B itself */
+int mfla_residual(mfla_handle h, int limbs, int uplo, mfla_matrix R, mfla_matrix A, mfla_matrix X, mfla_matrix B,
+                  const mfla_config *config);
+/* residual!(r, A, x, b) for a vector right-hand side (src/residual.jl:55-80,
+ * the gemv!/symv! path); r, x, b are n x 1 matrices */
+int mfla_residual_vector(mfla_handle h, int limbs, int uplo, mfla_matrix r, mfla_matrix A, mfla_matrix x, mfla_matrix b,
+                         const mfla_config *config);
+/* residual_mixed! (src/residual.jl:431-496): R (res_limbs) = B - A*X with
+ * A, X, B (src_limbs) promoted first. Pairs: 2->3, 2->4, 3->4 (MFLA) and
+ * 1->2/3/4 (extension used by FP64-factor refinement). */
+int mfla_residual_mixed(mfla_handle h, int src_limbs, int res_limbs, int uplo, mfla_matrix R, mfla_matrix A, mfla_matrix X,
+                        mfla_matrix B, const mfla_config *config);
+/* normwise_backward_error (src/residual.jl:215-269): out is a host array of
+ * limbs x nrhs binary64 (limb-planar), one value per right-hand side */
+int mfla_backward_error(mfla_handle h, int limbs, int uplo, mfla_matrix A, mfla_matrix X, mfla_matrix B, mfla_matrix R, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MFLA_HIP_H */

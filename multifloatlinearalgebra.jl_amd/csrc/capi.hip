@@ -1,0 +1,839 @@
+// C ABI of the library (include/mfla_hip.h): argument validation with MFLA's
+// error semantics, and the host drivers of the blocked factorizations and
+// solves. Every numerical step runs on the GPU; the host only sequences
+// kernels, and reads back status words where MFLA returns them.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mfla_hip.h"
+#include "k_blas2.hpp"
+#include "k_blas3.hpp"
+#include "k_factor.hpp"
+#include "k_trsm.hpp"
+#include "mf_arith.hpp"
+
+using mfk::DV;
+using mfk::Status;
+
+struct mfla_handle_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Status *status = nullptr;   // device status word of the current call
+    int *flags = nullptr;       // small device flag array
+    void *scratch = nullptr;    // grows on demand (non-hot paths only)
+    size_t scratch_bytes = 0;
+    std::mutex lock;            // calls on one handle are serialised
+};
+
+struct mfla_workspace_s {
+    mfla_handle h = nullptr;
+    int limbs = 0;
+    int64_t factor_capacity = 0;
+    int64_t block_capacity = 0;
+    double *weighted = nullptr;  // limbs * factor_capacity * (block_capacity + 1)
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(MFLA_ERR_HIP, "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+bool limbs_ok(int N) { return N >= 1 && N <= 4; }
+int check_limbs(int N) { return limbs_ok(N) ? MFLA_OK : fail(MFLA_ERR_ARGUMENT, "only 1-4 limb MultiFloats are supported"); }
+
+DV dvn(const mfla_matrix &m) { return DV{m.data, 1, m.ld, m.limb_stride}; }        // (i, j) -> M[i, j]
+DV dvt(const mfla_matrix &m) { return DV{m.data, m.ld, 1, m.limb_stride}; }        // (i, j) -> M[j, i]
+DV sub(const DV &v, int64_t i, int64_t j) { return DV{v.p + i * v.rs + j * v.cs, v.rs, v.cs, v.ls}; }
+
+// Base.mightalias for limb-planar storage: overlapping address extents.
+bool extents_overlap(const mfla_matrix &a, const mfla_matrix &b, int N) {
+    if (a.rows * a.cols == 0 || b.rows * b.cols == 0) return false;
+    auto ext = [N](const mfla_matrix &m, const double *&lo, const double *&hi) {
+        lo = m.data;
+        hi = m.data + (int64_t)(N - 1) * m.limb_stride + (m.cols - 1) * m.ld + m.rows;
+    };
+    const double *alo, *ahi, *blo, *bhi;
+    ext(a, alo, ahi);
+    ext(b, blo, bhi);
+    return alo < bhi && blo < ahi;
+}
+bool same_matrix(const mfla_matrix &a, const mfla_matrix &b) {
+    return a.data == b.data && a.rows == b.rows && a.cols == b.cols && a.ld == b.ld && a.limb_stride == b.limb_stride;
+}
+int check_desc(const mfla_matrix &m, const char *name) {
+    if (m.rows < 0 || m.cols < 0) return fail(MFLA_ERR_DIMENSION, "%s has negative dimensions", name);
+    if (m.rows * m.cols > 0 && (m.data == nullptr || m.ld < m.rows)) return fail(MFLA_ERR_ARGUMENT, "%s has an invalid descriptor", name);
+    return MFLA_OK;
+}
+
+template <int N> mf::V<N> host_v(const double *p) { mf::V<N> v; for (int l = 0; l < N; ++l) v.x[l] = p[l]; return v; }
+bool scalar_is_zero(const double *s, int N) { for (int l = 0; l < N; ++l) if (s[l] != 0.0) return false; return true; }
+bool scalar_is_one(const double *s, int N) { if (s[0] != 1.0) return false; for (int l = 1; l < N; ++l) if (s[l] != 0.0) return false; return true; }
+
+const mfla_config &cfg_or_default(const mfla_config *c, mfla_config &tmp) {
+    if (c) return *c;
+    mfla_config_default(&tmp);
+    return tmp;
+}
+
+void *scratch(mfla_handle h, size_t bytes) {
+    if (h->scratch_bytes < bytes) {
+        if (h->scratch) { (void)hipStreamSynchronize(h->stream); (void)hipFree(h->scratch); }
+        h->scratch = nullptr;
+        h->scratch_bytes = 0;
+        if (hipMalloc(&h->scratch, bytes) != hipSuccess) return nullptr;
+        h->scratch_bytes = bytes;
+    }
+    return h->scratch;
+}
+
+int64_t read_info(mfla_handle h, int64_t *aux = nullptr) {
+    Status st{};
+    if (hipMemcpyAsync(&st, h->status, sizeof(Status), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return INT64_MIN;
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return INT64_MIN;
+    if (aux) *aux = st.aux;
+    return st.info;
+}
+
+const double kOne[4] = {1.0, 0.0, 0.0, 0.0};
+const double kMinusOne[4] = {-1.0, 0.0, 0.0, 0.0};
+const double kZero[4] = {0.0, 0.0, 0.0, 0.0};
+
+// GEMM body shared by gemm!, LU's trailing update and the matrix residual.
+// x3 auto/fused route: MFLA's four-row vector blocks use mulacc_x3 and the
+// m mod 4 remainder rows the standard network (gemm.jl:411-452).
+hipError_t gemm_launch(mfla_handle h, int N, int strategy, int tile, int64_t m, int64_t n, int64_t k, const DV &A, const DV &B,
+                       const DV &C, const double *alpha, const double *beta, const Status *st) {
+    const bool fused = (N == 3) && (strategy == MFLA_GEMM_AUTO || strategy == MFLA_GEMM_FUSED);
+    mfk::TileArgs t{};
+    t.mode = mfk::MODE_AB; t.lower = 0; t.beta_zero = scalar_is_zero(beta, N) ? 1 : 0; t.tile = tile; t.n = n; t.K = k;
+    if (fused) {
+        const int64_t m4 = (m / 4) * 4;
+        t.net = mf::NET_FUSED3; t.row_lo = 0; t.rows = m4;
+        hipError_t e = mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
+        if (e != hipSuccess) return e;
+        t.net = mf::NET_STD; t.row_lo = m4; t.rows = m - m4;
+        return mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
+    }
+    t.net = mf::NET_STD; t.row_lo = 0; t.rows = m;
+    return mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
+}
+
+// Lower-triangle update C = alpha*a*b + beta*C over i >= j (syrk!/gemmt!).
+hipError_t lower_launch(mfla_handle h, int N, int tile, int64_t n, int64_t k, const DV &A, const DV &B, const DV &C,
+                        const double *alpha, const double *beta, const Status *st) {
+    mfk::TileArgs t{};
+    t.mode = mfk::MODE_AB; t.net = mf::NET_STD; t.lower = 1; t.beta_zero = scalar_is_zero(beta, N) ? 1 : 0; t.tile = tile;
+    t.row_lo = 0; t.rows = n; t.n = n; t.K = k;
+    return mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
+}
+
+bool trsm_exact(const mfla_config &c, int64_t n) {
+    if (c.trsm_order == MFLA_TRSM_ORDER_EXACT) return true;
+    if (c.trsm_order == MFLA_TRSM_ORDER_BLOCKED) return false;
+    return n <= 256;
+}
+
+// trsm! core (trsm.jl:247-301) after argument checks; alpha scaling included.
+hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int diag, const mfla_matrix &B, const mfla_matrix &A,
+                     const double *alpha, const mfla_config &c, const Status *st) {
+    const int64_t n = A.rows;
+    if (alpha && !scalar_is_one(alpha, N)) {
+        hipError_t e = mfk::scale(N, h->stream, B.rows, B.cols, dvn(B), alpha, st);
+        if (e != hipSuccess) return e;
+    }
+    const bool transposed = trans == MFLA_TRANS;
+    const bool eff_lower = ((uplo == MFLA_LOWER) == !transposed);
+    mfk::TrsmArgs a{};
+    a.n = n;
+    a.unit = diag == MFLA_UNIT;
+    a.exact_order = trsm_exact(c, n) ? 1 : 0;
+    DV X, CA;
+    if (side == MFLA_LEFT) {
+        X = dvn(B); a.nrhs = B.cols;
+        CA = transposed ? dvt(A) : dvn(A);
+        a.forward = eff_lower;
+    } else {
+        X = dvt(B); a.nrhs = B.rows;
+        CA = transposed ? dvn(A) : dvt(A);
+        a.forward = !eff_lower;
+    }
+    return mfk::trsm_solve(N, h->stream, a, X, CA, dvn(A), st);
+}
+
+template <int N> void host_max_reduce(const double *colmax, int64_t count, double *out) {
+    mf::V<N> mx = mf::zero<N>();
+    for (int64_t c = 0; c < count; ++c) {
+        mf::V<N> v;
+        for (int l = 0; l < N; ++l) v.x[l] = colmax[l * count + c];
+        mx = mf::vmax(mx, v);
+    }
+    for (int l = 0; l < N; ++l) out[l] = mx.x[l];
+}
+void host_max(int N, const double *colmax, int64_t count, double *out) {
+    switch (N) {
+        case 1: host_max_reduce<1>(colmax, count, out); break;
+        case 2: host_max_reduce<2>(colmax, count, out); break;
+        case 3: host_max_reduce<3>(colmax, count, out); break;
+        default: host_max_reduce<4>(colmax, count, out); break;
+    }
+}
+
+// original_maximum (types.jl:317-331): max |A| over all (region 0), lower
+// (region 1) or upper (region 2) entries.
+int original_maximum(mfla_handle h, int N, const mfla_matrix &A, int lower_only, double *out) {
+    const int64_t cols = A.cols;
+    for (int l = 0; l < N; ++l) out[l] = 0.0;
+    if (A.rows * cols == 0) return MFLA_OK;
+    double *buf = (double *)scratch(h, sizeof(double) * N * cols);
+    if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
+    HIPCHK(mfk::col_absmax_region(N, h->stream, dvn(A), A.rows, cols, lower_only, DV{buf, 1, cols, cols}));
+    std::vector<double> host((size_t)N * cols);
+    HIPCHK(hipMemcpyAsync(host.data(), buf, sizeof(double) * N * cols, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    host_max(N, host.data(), cols, out);
+    return MFLA_OK;
+}
+
+template <int N> void ldlt_alpha_t(double *out) {
+    // alpha = (1 + sqrt(17)) / 8 (ldlt.jl:691)
+    mf::V<N> a = mf::div(mf::add(mf::from<N>(1.0), mf::sqrt(mf::from<N>(17.0))), mf::from<N>(8.0));
+    for (int l = 0; l < N; ++l) out[l] = a.x[l];
+}
+void ldlt_alpha(int N, double *out) {
+    switch (N) {
+        case 1: ldlt_alpha_t<1>(out); break;
+        case 2: ldlt_alpha_t<2>(out); break;
+        case 3: ldlt_alpha_t<3>(out); break;
+        default: ldlt_alpha_t<4>(out); break;
+    }
+}
+
+template <int N>
+void backward_error_host(const double *an, const double *xn, const double *bn, const double *rn, bool finite, double *out) {
+    // src/residual.jl:178-200
+    auto put = [&](const mf::V<N> &v) { for (int l = 0; l < N; ++l) out[l] = v.x[l]; };
+    if (!finite) { put(mf::from<N>(__builtin_nan(""))); return; }
+    mf::V<N> A = host_v<N>(an), X = host_v<N>(xn), B = host_v<N>(bn), R = host_v<N>(rn);
+    mf::V<N> ps = mf::vmax(A, B);
+    if (mf::is_zero(ps)) { put(mf::is_zero(R) ? mf::zero<N>() : mf::from<N>(__builtin_inf())); return; }
+    mf::V<N> ss = mf::vmax(X, mf::from<N>(1.0));
+    mf::V<N> den = mf::add(mf::mul(mf::div(A, ps), mf::div(X, ss)), mf::div(mf::div(B, ps), ss));
+    if (mf::is_zero(den)) { put(mf::is_zero(R) ? mf::zero<N>() : mf::from<N>(__builtin_inf())); return; }
+    mf::V<N> sr = mf::div(mf::div(R, ps), ss);
+    put(mf::div(sr, den));
+}
+
+}  // namespace
+
+// ======================================================================
+extern "C" {
+
+int mfla_version(void) { return 1; }
+const char *mfla_last_error(void) { return g_err.c_str(); }
+
+void mfla_config_default(mfla_config *c) {
+    std::memset(c, 0, sizeof(*c));
+    c->reduction_tile = 64;
+    c->column_tile = 16;
+    c->cholesky_block = 16;
+    c->lu_block = 16;
+    c->ldlt_block = 0;
+    c->ldlt_strategy = MFLA_LDLT_AUTO;
+    c->gemm_strategy = MFLA_GEMM_AUTO;
+    c->ldlt_blocked_crossover = 512;
+    c->thread_count = 1;
+    c->gemm_panel_columns = 0;
+    c->gemm_micro_columns = 0;
+    c->trsm_order = MFLA_TRSM_ORDER_AUTO;
+    c->gemm_tile = -1;
+}
+
+int mfla_handle_create(mfla_handle *out) {
+    if (!out) return fail(MFLA_ERR_ARGUMENT, "null handle pointer");
+    auto *h = new mfla_handle_s();
+    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return fail(MFLA_ERR_HIP, "no HIP device"); }
+    if (hipMalloc(&h->status, sizeof(Status)) != hipSuccess) { delete h; return fail(MFLA_ERR_HIP, "status allocation failed"); }
+    if (hipMalloc(&h->flags, 64 * sizeof(int)) != hipSuccess) { (void)hipFree(h->status); delete h; return fail(MFLA_ERR_HIP, "flag allocation failed"); }
+    *out = h;
+    return MFLA_OK;
+}
+
+int mfla_handle_destroy(mfla_handle h) {
+    if (!h) return MFLA_OK;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(h->status);
+    (void)hipFree(h->flags);
+    if (h->scratch) (void)hipFree(h->scratch);
+    delete h;
+    return MFLA_OK;
+}
+
+int mfla_handle_set_stream(mfla_handle h, void *stream) {
+    if (!h) return fail(MFLA_ERR_ARGUMENT, "null handle");
+    std::lock_guard<std::mutex> g(h->lock);
+    h->stream = (hipStream_t)stream;
+    return MFLA_OK;
+}
+
+int mfla_synchronize(mfla_handle h) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return MFLA_OK;
+}
+
+// ------------------------------------------------------------ workspace
+int mfla_workspace_create(mfla_handle h, int limbs, int64_t factor_capacity, int64_t ldlt_block_capacity, mfla_workspace *out) {
+    if (check_limbs(limbs)) return MFLA_ERR_ARGUMENT;
+    if (factor_capacity < 0) return fail(MFLA_ERR_ARGUMENT, "factor_capacity must be nonnegative");
+    if (ldlt_block_capacity < 0) return fail(MFLA_ERR_ARGUMENT, "ldlt_block_capacity must be nonnegative");
+    auto *w = new mfla_workspace_s();
+    w->h = h;
+    w->limbs = limbs;
+    *out = w;
+    int rc = mfla_workspace_ensure(w, factor_capacity, ldlt_block_capacity);
+    if (rc != MFLA_OK) { delete w; *out = nullptr; }
+    return rc;
+}
+
+int mfla_workspace_destroy(mfla_workspace w) {
+    if (!w) return MFLA_OK;
+    if (w->weighted) { (void)hipStreamSynchronize(w->h->stream); (void)hipFree(w->weighted); }
+    delete w;
+    return MFLA_OK;
+}
+
+int mfla_workspace_capacity(mfla_workspace w, int64_t *out2) {
+    out2[0] = w->factor_capacity;
+    out2[1] = w->block_capacity;
+    return MFLA_OK;
+}
+
+int mfla_workspace_ensure(mfla_workspace w, int64_t factor_capacity, int64_t ldlt_block_capacity) {
+    if (factor_capacity < 0 || ldlt_block_capacity < 0) return fail(MFLA_ERR_ARGUMENT, "workspace capacities must be nonnegative");
+    const int64_t f = std::max(w->factor_capacity, factor_capacity);
+    const int64_t b = std::max(w->block_capacity, ldlt_block_capacity);
+    if (f == w->factor_capacity && b == w->block_capacity) return MFLA_OK;
+    const size_t bytes = sizeof(double) * (size_t)w->limbs * (size_t)f * (size_t)(b > 0 ? b + 1 : 0);
+    double *p = nullptr;
+    if (bytes > 0 && hipMalloc(&p, bytes) != hipSuccess) return fail(MFLA_ERR_HIP, "workspace allocation failed");
+    if (w->weighted) { (void)hipStreamSynchronize(w->h->stream); (void)hipFree(w->weighted); }
+    w->weighted = p;
+    w->factor_capacity = f;
+    w->block_capacity = b;
+    return MFLA_OK;
+}
+
+// ------------------------------------------------------------ arithmetic
+int mfla_arith(mfla_handle h, int N, int op, int64_t count, const double *x, const double *y, const double *z, double *out) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (op < 0 || op > 12 || (op == 6 && N != 3)) return fail(MFLA_ERR_ARGUMENT, "unsupported arithmetic op %d for %d limbs", op, N);
+    HIPCHK(mfk::arith(N, h->stream, op, count, x, y, z, out));
+    return MFLA_OK;
+}
+
+int mfla_host_arith(int N, int op, int64_t count, const double *x, const double *y, const double *z, double *out) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (op < 0 || op > 12 || (op == 6 && N != 3)) return fail(MFLA_ERR_ARGUMENT, "unsupported arithmetic op %d for %d limbs", op, N);
+    return mfk::host_arith(N, op, count, x, y, z, out) == 0 ? MFLA_OK : fail(MFLA_ERR_ARGUMENT, "arithmetic failed");
+}
+
+// types.jl:298-331 helpers (_all_finite, _lower_triangle_finite,
+// _maximum_abs, _lower_maximum_abs) and the upper-triangle scan of
+// diagnostics.jl:74-81. region: 0 all, 1 lower (i >= j), 2 upper (i <= j).
+int mfla_absmax(mfla_handle h, int N, int region, mfla_matrix A, double *out) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A")) return MFLA_ERR_ARGUMENT;
+    if (region < 0 || region > 2) return fail(MFLA_ERR_ARGUMENT, "region must be 0, 1 or 2");
+    std::lock_guard<std::mutex> g(h->lock);
+    return original_maximum(h, N, A, region, out);
+}
+
+int mfla_all_finite(mfla_handle h, int N, int region, mfla_matrix A, int *out) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A")) return MFLA_ERR_ARGUMENT;
+    if (region < 0 || region > 2) return fail(MFLA_ERR_ARGUMENT, "region must be 0, 1 or 2");
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(mfk::status_reset(h->stream, h->status));
+    HIPCHK(mfk::finite_check(N, h->stream, A.rows, A.cols, dvn(A), region, h->status));
+    const int64_t r = read_info(h);
+    if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+    *out = r == 0 ? 1 : 0;
+    return MFLA_OK;
+}
+
+// ------------------------------------------------------------ conversions
+int mfla_from_interleaved(mfla_handle h, int limbs, mfla_matrix dst, const double *src, int64_t src_ld) {
+    if (check_limbs(limbs)) return MFLA_ERR_ARGUMENT;
+    HIPCHK(mfk::aos_to_planar(limbs, h->stream, dst.rows, dst.cols, dvn(dst), src, src_ld));
+    return MFLA_OK;
+}
+
+int mfla_to_interleaved(mfla_handle h, int limbs, double *dst, int64_t dst_ld, mfla_matrix src) {
+    if (check_limbs(limbs)) return MFLA_ERR_ARGUMENT;
+    HIPCHK(mfk::planar_to_aos(limbs, h->stream, src.rows, src.cols, dst, dst_ld, dvn(src)));
+    return MFLA_OK;
+}
+
+// ------------------------------------------------------------ BLAS 3
+int mfla_gemm(mfla_handle h, int N, mfla_matrix C, mfla_matrix A, mfla_matrix B, const double *alpha, const double *beta,
+              const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    const int64_t m = A.rows, k = A.cols, n = B.cols;
+    if (B.rows != k) return fail(MFLA_ERR_DIMENSION, "gemm! inner dimensions differ");
+    if (C.rows != m || C.cols != n) return fail(MFLA_ERR_DIMENSION, "gemm! output dimensions differ");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A") || check_desc(B, "B") || check_desc(C, "C")) return MFLA_ERR_ARGUMENT;
+    if (extents_overlap(C, A, N) || extents_overlap(C, B, N)) return fail(MFLA_ERR_ALIAS, "gemm! does not support output/input aliasing");
+    if (c.gemm_strategy < 0 || c.gemm_strategy > 3) return fail(MFLA_ERR_ARGUMENT, "gemm_strategy must be :auto, :direct, :packed, or :fused");
+    if (c.gemm_strategy == MFLA_GEMM_FUSED && N != 3) return fail(MFLA_ERR_ARGUMENT, "gemm_strategy=:fused is only available for Float64x3");
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(gemm_launch(h, N, c.gemm_strategy, c.gemm_tile, m, n, k, dvn(A), dvn(B), dvn(C), alpha ? alpha : kOne, beta ? beta : kZero,
+                       nullptr));
+    return MFLA_OK;
+}
+
+int mfla_syrk(mfla_handle h, int N, int uplo, mfla_matrix C, mfla_matrix P, const double *alpha, const double *beta,
+              const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (uplo != MFLA_LOWER) return fail(MFLA_ERR_ARGUMENT, "syrk! currently supports only uplo=:lower or :L");
+    const int64_t cols = P.cols;
+    if (C.rows != cols || C.cols != cols) return fail(MFLA_ERR_DIMENSION, "syrk! output dimensions differ");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(C, "C") || check_desc(P, "P")) return MFLA_ERR_ARGUMENT;
+    if (extents_overlap(C, P, N)) return fail(MFLA_ERR_ALIAS, "syrk! does not support output/input aliasing");
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(lower_launch(h, N, c.gemm_tile, cols, P.rows, dvt(P), dvn(P), dvn(C), alpha ? alpha : kOne, beta ? beta : kZero, nullptr));
+    return MFLA_OK;
+}
+
+int mfla_gemmt(mfla_handle h, int N, mfla_matrix C, mfla_matrix L, mfla_matrix R, const double *alpha, const double *beta,
+               const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    const int64_t rows = L.rows, red = L.cols;
+    if (R.rows != rows || R.cols != red) return fail(MFLA_ERR_DIMENSION, "gemmt! input dimensions differ");
+    if (C.rows != rows || C.cols != rows) return fail(MFLA_ERR_DIMENSION, "gemmt! output dimensions differ");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(C, "C") || check_desc(L, "L") || check_desc(R, "R")) return MFLA_ERR_ARGUMENT;
+    if (extents_overlap(C, L, N) || extents_overlap(C, R, N)) return fail(MFLA_ERR_ALIAS, "gemmt! does not support output/input aliasing");
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(lower_launch(h, N, c.gemm_tile, rows, red, dvn(L), dvt(R), dvn(C), alpha ? alpha : kOne, beta ? beta : kZero, nullptr));
+    return MFLA_OK;
+}
+
+static int check_tr_symbols(int side, int uplo, int trans, int diag) {
+    if (side != MFLA_LEFT && side != MFLA_RIGHT) return fail(MFLA_ERR_ARGUMENT, "side must be :left or :right");
+    if (uplo != MFLA_LOWER && uplo != MFLA_UPPER) return fail(MFLA_ERR_ARGUMENT, "uplo must be :lower or :upper");
+    if (trans != MFLA_NOTRANS && trans != MFLA_TRANS) return fail(MFLA_ERR_ARGUMENT, "trans must be :N or :T");
+    if (diag != MFLA_NONUNIT && diag != MFLA_UNIT) return fail(MFLA_ERR_ARGUMENT, "diag must be :unit or :nonunit");
+    return MFLA_OK;
+}
+
+int mfla_trsm(mfla_handle h, int N, int side, int uplo, int trans, int diag, mfla_matrix B, mfla_matrix A, const double *alpha,
+              const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "trsm! requires a square triangular matrix");
+    if (check_tr_symbols(side, uplo, trans, diag)) return MFLA_ERR_ARGUMENT;
+    if (side == MFLA_LEFT && B.rows != A.rows) return fail(MFLA_ERR_DIMENSION, "trsm! left dimensions differ");
+    if (side == MFLA_RIGHT && B.cols != A.rows) return fail(MFLA_ERR_DIMENSION, "trsm! right dimensions differ");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A") || check_desc(B, "B")) return MFLA_ERR_ARGUMENT;
+    if (extents_overlap(B, A, N)) return fail(MFLA_ERR_ALIAS, "trsm! does not support output/input aliasing");
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(trsm_core(h, N, side, uplo, trans, diag, B, A, alpha ? alpha : kOne, c, nullptr));
+    return MFLA_OK;
+}
+
+// ------------------------------------------------------------ BLAS 2
+int mfla_trsv(mfla_handle h, int N, int uplo, int trans, int diag, mfla_matrix x, mfla_matrix A, const double *alpha,
+              const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "trsv! requires a square triangular matrix");
+    if (x.rows != A.rows || x.cols != 1) return fail(MFLA_ERR_DIMENSION, "trsv! right-hand side length differs");
+    if (check_tr_symbols(MFLA_LEFT, uplo, trans, diag)) return MFLA_ERR_ARGUMENT;
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A") || check_desc(x, "x")) return MFLA_ERR_ARGUMENT;
+    if (extents_overlap(x, A, N)) return fail(MFLA_ERR_ALIAS, "trsv! does not support output/input aliasing");
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(trsm_core(h, N, MFLA_LEFT, uplo, trans, diag, x, A, alpha ? alpha : kOne, c, nullptr));
+    return MFLA_OK;
+}
+
+int mfla_gemv(mfla_handle h, int N, int trans, mfla_matrix y, mfla_matrix A, mfla_matrix x, const double *alpha, const double *beta,
+              const mfla_config *) {
+    const int64_t m = A.rows, n = A.cols;
+    if (trans != MFLA_NOTRANS && trans != MFLA_TRANS) return fail(MFLA_ERR_ARGUMENT, "trans must be :N or :T");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A") || check_desc(x, "x") || check_desc(y, "y")) return MFLA_ERR_ARGUMENT;
+    if (extents_overlap(y, A, N) || extents_overlap(y, x, N)) return fail(MFLA_ERR_ALIAS, "gemv! does not support output/input aliasing");
+    if (x.rows != (trans ? m : n) || x.cols != 1) return fail(MFLA_ERR_DIMENSION, "gemv! input length differs");
+    if (y.rows != (trans ? n : m) || y.cols != 1) return fail(MFLA_ERR_DIMENSION, "gemv! output length differs");
+    const double *al = alpha ? alpha : kOne, *be = beta ? beta : kZero;
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(mfk::gemv(N, h->stream, trans, m, n, dvn(A), dvn(x), dvn(y), al, be, scalar_is_zero(be, N) ? 1 : 0, nullptr));
+    return MFLA_OK;
+}
+
+int mfla_symv(mfla_handle h, int N, int uplo, mfla_matrix y, mfla_matrix A, mfla_matrix x, const double *alpha, const double *beta,
+              const mfla_config *) {
+    const int64_t n = A.rows;
+    if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "symv! requires a square matrix");
+    if (x.rows != n || x.cols != 1) return fail(MFLA_ERR_DIMENSION, "symv! input length differs");
+    if (y.rows != n || y.cols != 1) return fail(MFLA_ERR_DIMENSION, "symv! output length differs");
+    if (uplo != MFLA_LOWER && uplo != MFLA_UPPER) return fail(MFLA_ERR_ARGUMENT, "uplo must be :lower or :upper");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A") || check_desc(x, "x") || check_desc(y, "y")) return MFLA_ERR_ARGUMENT;
+    if (extents_overlap(y, A, N) || extents_overlap(y, x, N)) return fail(MFLA_ERR_ALIAS, "symv! does not support output/input aliasing");
+    const double *al = alpha ? alpha : kOne, *be = beta ? beta : kZero;
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(mfk::symv(N, h->stream, uplo == MFLA_UPPER, n, 1, dvn(A), dvn(x), dvn(y), al, be, scalar_is_zero(be, N) ? 1 : 0));
+    return MFLA_OK;
+}
+
+// ------------------------------------------------------ factorizations
+int mfla_potrf(mfla_handle h, int N, mfla_matrix A, int64_t *info, const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    const int64_t n = A.rows;
+    if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "cholesky! requires a square matrix");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A")) return MFLA_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    Status *st = h->status;
+    HIPCHK(mfk::status_reset(s, st));
+    HIPCHK(mfk::finite_check(N, s, n, n, dvn(A), 1, st));
+    const int64_t block = std::max<int64_t>(c.cholesky_block, 1);
+    const DV a = dvn(A);
+    for (int64_t first = 0; first < n; first += block) {
+        const int64_t last = std::min(first + block, n) - 1, nb = last - first + 1;
+        HIPCHK(mfk::chol_panel(N, s, a, first, last, st));
+        if (last + 1 < n) {
+            const int64_t rest = n - last - 1;
+            // trsm!(A21, L11; side=:right, uplo=:lower, trans=:T) -> forward over panel columns
+            mfk::TrsmArgs ta{};
+            ta.n = nb; ta.nrhs = rest; ta.forward = 1; ta.unit = 0; ta.exact_order = 1;
+            const DV A21 = sub(a, last + 1, first), L11 = sub(a, first, first);
+            HIPCHK(mfk::trsm_solve(N, s, ta, DV{A21.p, A21.cs, A21.rs, A21.ls}, L11, L11, st));
+            // syrk!(A22, transpose(A21), -1, 1)
+            HIPCHK(lower_launch(h, N, c.gemm_tile, rest, nb, A21, DV{A21.p, A21.cs, A21.rs, A21.ls}, sub(a, last + 1, last + 1),
+                                kMinusOne, kOne, st));
+        }
+    }
+    const int64_t r = read_info(h);
+    if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+    *info = r;
+    return MFLA_OK;
+}
+
+int mfla_getrf(mfla_handle h, int N, mfla_matrix A, int64_t *ipiv, double *origmax, int64_t *info, const mfla_config *config,
+               mfla_workspace) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    const int64_t m = A.rows, n = A.cols, kmax = std::min(m, n);
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A")) return MFLA_ERR_ARGUMENT;
+    if (c.gemm_strategy < 0 || c.gemm_strategy > 3) return fail(MFLA_ERR_ARGUMENT, "gemm_strategy must be :auto, :direct, :packed, or :fused");
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    Status *st = h->status;
+    HIPCHK(mfk::status_reset(s, st));
+    HIPCHK(mfk::iota1(s, ipiv, kmax));
+    HIPCHK(mfk::finite_check(N, s, m, n, dvn(A), 0, st));
+    double om[4] = {0, 0, 0, 0};
+    if (int rc = original_maximum(h, N, A, 0, om)) return rc;
+    const int64_t block = std::max<int64_t>(c.lu_block, 1);
+    const DV a = dvn(A);
+    for (int64_t first = 0; first < kmax; first += block) {
+        const int64_t last = std::min(first + block, kmax) - 1, nb = last - first + 1;
+        for (int64_t k = first; k <= last; ++k) {
+            HIPCHK(mfk::lu_pivot_swap(N, s, a, k, m, n, ipiv, st));
+            HIPCHK(mfk::lu_scale_update(N, s, a, k, last, m, st));
+        }
+        if (last + 1 < n) {
+            // trsm!(U12, L11; side=:left, uplo=:lower, trans=:N, diag=:unit)
+            mfk::TrsmArgs ta{};
+            ta.n = nb; ta.nrhs = n - last - 1; ta.forward = 1; ta.unit = 1; ta.exact_order = 1;
+            const DV U12 = sub(a, first, last + 1), L11 = sub(a, first, first);
+            HIPCHK(mfk::trsm_solve(N, s, ta, U12, L11, L11, st));
+            if (last + 1 < m)  // gemm!(A22, A21, U12, -1, 1)
+                HIPCHK(gemm_launch(h, N, c.gemm_strategy, c.gemm_tile, m - last - 1, n - last - 1, nb, sub(a, last + 1, first), U12,
+                                   sub(a, last + 1, last + 1), kMinusOne, kOne, st));
+        }
+    }
+    const int64_t r = read_info(h);
+    if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+    *info = r;
+    for (int l = 0; l < N; ++l) origmax[l] = (r == -1) ? 0.0 : om[l];
+    return MFLA_OK;
+}
+
+int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *pivots, uint8_t *blocks, double *origmax, int64_t *info,
+               const mfla_config *config, mfla_workspace ws) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    const int64_t n = A.rows;
+    if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "ldlt! requires a square matrix");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A") || check_desc(dsub, "dsub")) return MFLA_ERR_ARGUMENT;
+    if (dsub.rows != n || dsub.cols != 1) return fail(MFLA_ERR_DIMENSION, "ldlt! dsub length differs");
+    if (c.ldlt_strategy < 0 || c.ldlt_strategy > 2) return fail(MFLA_ERR_ARGUMENT, "ldlt_strategy must be :auto, :unblocked, or :blocked");
+    // ldlt_plan (ldlt.jl:329-350), block widths ldlt_tuning.jl:82-88
+    int64_t b = c.ldlt_block > 0 ? c.ldlt_block : (N <= 2 ? 16 : (N == 3 ? 12 : 8));
+    b = std::max<int64_t>(b, 1);
+    bool blocked;
+    if (c.ldlt_strategy == MFLA_LDLT_UNBLOCKED) blocked = false;
+    else if (c.ldlt_strategy == MFLA_LDLT_BLOCKED) { blocked = true; b = std::max<int64_t>(b, 2); }
+    else blocked = b > 1 && n >= std::max<int64_t>(c.ldlt_blocked_crossover, 1);
+
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    Status *st = h->status;
+    HIPCHK(mfk::status_reset(s, st));
+    HIPCHK(mfk::iota1(s, pivots, n));
+    HIPCHK(mfk::fill_u8(s, blocks, n, 0));
+    HIPCHK(mfk::zero_fill(N, s, n, 1, dvn(dsub)));
+    HIPCHK(mfk::finite_check(N, s, n, n, dvn(A), 1, st));
+    HIPCHK(mfk::mirror_lower(N, s, dvn(A), n, st));
+    double om[4] = {0, 0, 0, 0};
+    if (int rc = original_maximum(h, N, A, 1, om)) return rc;
+    double alpha[4];
+    ldlt_alpha(N, alpha);
+    const DV a = dvn(A), d = dvn(dsub);
+    if (!blocked) {
+        HIPCHK(mfk::ldlt_unblocked(N, s, a, n, d, pivots, blocks, alpha, st));
+    } else {
+        double *W = nullptr;
+        bool own = false;
+        const size_t need = sizeof(double) * (size_t)N * (size_t)n * (size_t)(b + 1);
+        if (ws && ws->limbs == N && ws->factor_capacity >= n && ws->block_capacity >= b) W = ws->weighted;
+        else {
+            if (hipMalloc(&W, need) != hipSuccess) return fail(MFLA_ERR_HIP, "weighted panel allocation failed");
+            own = true;
+        }
+        int64_t pfirst = 0;
+        while (pfirst < n) {
+            const int64_t reqlast = std::min(pfirst + b, n) - 1;
+            HIPCHK(mfk::ldlt_panel(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, st));
+            int64_t plast = 0;
+            const int64_t r = read_info(h, &plast);
+            if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+            if (r != 0) break;
+            if (plast < n - 1) {
+                const int64_t tfirst = plast + 1, tcount = n - plast - 1, width = plast - pfirst + 1;
+                const DV Wv{W, 1, tcount, tcount * (b + 1)};
+                HIPCHK(mfk::ldlt_weighted(N, s, a, pfirst, plast, tfirst, tcount, d, blocks, Wv, st));
+                HIPCHK(lower_launch(h, N, c.gemm_tile, tcount, width, sub(a, tfirst, pfirst), DV{W, tcount, 1, tcount * (b + 1)},
+                                    sub(a, tfirst, tfirst), kMinusOne, kOne, st));
+                HIPCHK(mfk::mirror_lower(N, s, sub(a, tfirst, tfirst), tcount, st));
+            }
+            pfirst = plast + 1;
+        }
+        if (own) { (void)hipStreamSynchronize(s); (void)hipFree(W); }
+    }
+    const int64_t r = read_info(h);
+    if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+    *info = r;
+    for (int l = 0; l < N; ++l) origmax[l] = (r == -1) ? 0.0 : om[l];
+    return MFLA_OK;
+}
+
+// ------------------------------------------------------------- solves
+static int check_solve(int N, const mfla_matrix &F, const mfla_matrix &B) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(F, "F") || check_desc(B, "B")) return MFLA_ERR_ARGUMENT;
+    if (F.rows != F.cols) return fail(MFLA_ERR_DIMENSION, "solve requires a square factor");
+    if (B.rows != F.rows) return fail(MFLA_ERR_DIMENSION, "right-hand side dimensions differ");
+    if (extents_overlap(B, F, N)) return fail(MFLA_ERR_ALIAS, "ldiv! does not support output/input aliasing");
+    return MFLA_OK;
+}
+
+int mfla_potrs(mfla_handle h, int N, mfla_matrix F, int64_t info, mfla_matrix B, const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (info != 0) return fail(MFLA_ERR_FAILED_FACTOR, "PosDefException(%lld)", (long long)info);
+    if (int rc = check_solve(N, F, B)) return rc;
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_NOTRANS, MFLA_NONUNIT, B, F, kOne, c, nullptr));
+    HIPCHK(trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_TRANS, MFLA_NONUNIT, B, F, kOne, c, nullptr));
+    return MFLA_OK;
+}
+
+int mfla_getrs(mfla_handle h, int N, mfla_matrix F, const int64_t *ipiv, int64_t info, mfla_matrix B, const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (info != 0) return fail(MFLA_ERR_FAILED_FACTOR, "SingularException(%lld)", (long long)info);
+    if (int rc = check_solve(N, F, B)) return rc;
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(mfk::apply_ipiv(N, h->stream, dvn(B), F.rows, B.cols, ipiv));
+    HIPCHK(trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_NOTRANS, MFLA_UNIT, B, F, kOne, c, nullptr));
+    HIPCHK(trsm_core(h, N, MFLA_LEFT, MFLA_UPPER, MFLA_NOTRANS, MFLA_NONUNIT, B, F, kOne, c, nullptr));
+    return MFLA_OK;
+}
+
+int mfla_sytrs(mfla_handle h, int N, mfla_matrix F, mfla_matrix dsub, const int64_t *pivots, const uint8_t *blocks, int64_t info,
+               mfla_matrix B, const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (info != 0) return fail(MFLA_ERR_FAILED_FACTOR, "SingularException(%lld)", (long long)info);
+    if (int rc = check_solve(N, F, B)) return rc;
+    const int64_t n = F.rows, nrhs = B.cols;
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    HIPCHK(mfk::status_reset(s, h->status));
+    HIPCHK(hipMemsetAsync(h->flags, 0, sizeof(int), s));
+    HIPCHK(mfk::ldlt_pivots(N, s, dvn(B), n, nrhs, pivots, blocks, 1, h->flags));
+    HIPCHK(trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_NOTRANS, MFLA_UNIT, B, F, kOne, c, nullptr));
+    HIPCHK(mfk::ldlt_dsolve(N, s, dvn(B), n, nrhs, dvn(F), dvn(dsub), blocks, h->status));
+    HIPCHK(trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_TRANS, MFLA_UNIT, B, F, kOne, c, nullptr));
+    HIPCHK(mfk::ldlt_pivots(N, s, dvn(B), n, nrhs, pivots, blocks, 0, h->flags));
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, h->flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    const int64_t r = read_info(h);
+    if (bad) return fail(MFLA_ERR_ARGUMENT, "invalid LDLT block structure");
+    if (r != 0) return fail(MFLA_ERR_FAILED_FACTOR, "SingularException(%lld)", (long long)r);
+    return MFLA_OK;
+}
+
+// --------------------------------------------------- residual / errors
+static int residual_common(mfla_handle h, int N, int uplo, mfla_matrix R, mfla_matrix A, mfla_matrix X, mfla_matrix B, bool vec,
+                           const mfla_config &c) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (uplo != MFLA_GENERAL && uplo != MFLA_LOWER && uplo != MFLA_UPPER)
+        return fail(MFLA_ERR_ARGUMENT, "uplo must be :general, :lower, or :upper");
+    if (check_desc(R, "R") || check_desc(A, "A") || check_desc(X, "X") || check_desc(B, "B")) return MFLA_ERR_ARGUMENT;
+    const int64_t rows = A.rows, cols = A.cols;
+    if (vec) {
+        if (R.rows != rows || R.cols != 1) return fail(MFLA_ERR_DIMENSION, "residual output length differs");
+        if (X.rows != cols || X.cols != 1) return fail(MFLA_ERR_DIMENSION, "residual solution length differs");
+        if (B.rows != rows || B.cols != 1) return fail(MFLA_ERR_DIMENSION, "residual right-hand side length differs");
+    } else {
+        if (R.rows != B.rows || R.cols != B.cols) return fail(MFLA_ERR_DIMENSION, "residual output dimensions differ");
+        if (R.rows != rows) return fail(MFLA_ERR_DIMENSION, "residual output row count differs");
+        if (X.rows != cols) return fail(MFLA_ERR_DIMENSION, "residual solution row count differs");
+        if (X.cols != R.cols) return fail(MFLA_ERR_DIMENSION, "residual right-hand-side count differs");
+    }
+    if (uplo != MFLA_GENERAL && rows != cols) return fail(MFLA_ERR_DIMENSION, "symmetric residual requires a square matrix");
+    if (extents_overlap(R, X, N)) return fail(MFLA_ERR_ALIAS, "residual output must not alias x");
+    if (extents_overlap(R, A, N)) return fail(MFLA_ERR_ALIAS, "residual output must not alias A");
+    const bool same = same_matrix(R, B);
+    if (!same && extents_overlap(R, B, N)) return fail(MFLA_ERR_ALIAS, "residual output may equal b but must not partially alias it");
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    if (!same) HIPCHK(mfk::copy(N, s, R.rows, R.cols, dvn(R), dvn(B)));
+    if (uplo == MFLA_GENERAL) {
+        if (vec) HIPCHK(mfk::gemv(N, s, 0, rows, cols, dvn(A), dvn(X), dvn(R), kMinusOne, kOne, 0, nullptr));
+        else HIPCHK(gemm_launch(h, N, c.gemm_strategy, c.gemm_tile, rows, R.cols, cols, dvn(A), dvn(X), dvn(R), kMinusOne, kOne, nullptr));
+    } else {
+        HIPCHK(mfk::symv(N, s, uplo == MFLA_UPPER, rows, R.cols, dvn(A), dvn(X), dvn(R), kMinusOne, kOne, 0));
+    }
+    return MFLA_OK;
+}
+
+int mfla_residual(mfla_handle h, int N, int uplo, mfla_matrix R, mfla_matrix A, mfla_matrix X, mfla_matrix B, const mfla_config *config) {
+    mfla_config tmp;
+    return residual_common(h, N, uplo, R, A, X, B, false, cfg_or_default(config, tmp));
+}
+
+int mfla_residual_vector(mfla_handle h, int N, int uplo, mfla_matrix r, mfla_matrix A, mfla_matrix x, mfla_matrix b,
+                         const mfla_config *config) {
+    mfla_config tmp;
+    return residual_common(h, N, uplo, r, A, x, b, true, cfg_or_default(config, tmp));
+}
+
+int mfla_residual_mixed(mfla_handle h, int NS, int NR, int uplo, mfla_matrix R, mfla_matrix A, mfla_matrix X, mfla_matrix B,
+                        const mfla_config *) {
+    const bool ok = (NS == 2 && (NR == 3 || NR == 4)) || (NS == 3 && NR == 4) || (NS == 1 && NR >= 2 && NR <= 4);
+    if (!ok) return fail(MFLA_ERR_ARGUMENT, "residual_mixed! supports only Float64x2 to Float64x3/x4 and Float64x3 to Float64x4");
+    if (uplo != MFLA_GENERAL && uplo != MFLA_LOWER && uplo != MFLA_UPPER)
+        return fail(MFLA_ERR_ARGUMENT, "uplo must be :general, :lower, or :upper");
+    const int64_t rows = A.rows, cols = A.cols;
+    if (R.rows != B.rows || R.cols != B.cols) return fail(MFLA_ERR_DIMENSION, "residual output dimensions differ");
+    if (R.rows != rows) return fail(MFLA_ERR_DIMENSION, "residual output row count differs");
+    if (X.rows != cols) return fail(MFLA_ERR_DIMENSION, "residual solution row count differs");
+    if (X.cols != R.cols) return fail(MFLA_ERR_DIMENSION, "residual right-hand-side count differs");
+    if (uplo != MFLA_GENERAL && rows != cols) return fail(MFLA_ERR_DIMENSION, "symmetric residual requires a square matrix");
+    if (extents_overlap(R, A, NR > NS ? NS : NR) || extents_overlap(R, X, NS) || extents_overlap(R, B, NS))
+        return fail(MFLA_ERR_ALIAS, "mixed residual output must not alias source operands");
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(mfk::residual_mixed(NS, NR, h->stream, uplo, rows, cols, R.cols, dvn(A), dvn(X), dvn(B), dvn(R)));
+    return MFLA_OK;
+}
+
+int mfla_backward_error(mfla_handle h, int N, int uplo, mfla_matrix A, mfla_matrix X, mfla_matrix B, mfla_matrix R, double *out) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (uplo != MFLA_GENERAL && uplo != MFLA_LOWER && uplo != MFLA_UPPER)
+        return fail(MFLA_ERR_ARGUMENT, "uplo must be :general, :lower, or :upper");
+    const int64_t m = A.rows, n = A.cols, nrhs = X.cols;
+    if (R.rows != B.rows || R.cols != B.cols || R.rows != m || X.rows != n || X.cols != R.cols)
+        return fail(MFLA_ERR_DIMENSION, "backward error dimensions differ");
+    if (uplo != MFLA_GENERAL && m != n) return fail(MFLA_ERR_DIMENSION, "symmetric residual requires a square matrix");
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    // scratch: row sums (m), three column-max vectors (nrhs each), flags
+    const size_t dbl = (size_t)N * (size_t)(m + 3 * nrhs);
+    char *base = (char *)scratch(h, sizeof(double) * dbl + sizeof(int) * 3 * (size_t)nrhs + 64);
+    if (!base) return fail(MFLA_ERR_HIP, "scratch allocation failed");
+    double *rows = (double *)base;
+    double *xm = rows + (size_t)N * m, *bm = xm + (size_t)N * nrhs, *rm = bm + (size_t)N * nrhs;
+    int *fin = (int *)(rm + (size_t)N * nrhs);
+    HIPCHK(mfk::status_reset(s, h->status));
+    HIPCHK(mfk::finite_check(N, s, m, n, dvn(A), uplo == MFLA_GENERAL ? 0 : (uplo == MFLA_LOWER ? 1 : 2), h->status));
+    HIPCHK(mfk::row_abs_sum(N, s, uplo, m, n, dvn(A), DV{rows, 1, m, m}));
+    HIPCHK(mfk::col_absmax(N, s, n, nrhs, dvn(X), DV{xm, 1, nrhs, nrhs}, fin));
+    HIPCHK(mfk::col_absmax(N, s, m, nrhs, dvn(B), DV{bm, 1, nrhs, nrhs}, fin + nrhs));
+    HIPCHK(mfk::col_absmax(N, s, m, nrhs, dvn(R), DV{rm, 1, nrhs, nrhs}, fin + 2 * nrhs));
+    std::vector<double> hb(dbl);
+    std::vector<int> hf(3 * (size_t)nrhs);
+    HIPCHK(hipMemcpyAsync(hb.data(), base, sizeof(double) * dbl, hipMemcpyDeviceToHost, s));
+    if (nrhs > 0) HIPCHK(hipMemcpyAsync(hf.data(), fin, sizeof(int) * 3 * nrhs, hipMemcpyDeviceToHost, s));
+    const int64_t st = read_info(h);
+    if (st == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+    const bool afin = st == 0;
+    double an[4];
+    host_max(N, hb.data(), m, an);
+    for (int64_t j = 0; j < nrhs; ++j) {
+        double xn[4], bn[4], rn[4], o[4];
+        for (int l = 0; l < N; ++l) {
+            xn[l] = hb[(size_t)N * m + l * nrhs + j];
+            bn[l] = hb[(size_t)N * (m + nrhs) + l * nrhs + j];
+            rn[l] = hb[(size_t)N * (m + 2 * nrhs) + l * nrhs + j];
+        }
+        const bool f = afin && hf[j] && hf[nrhs + j] && hf[2 * nrhs + j];
+        switch (N) {
+            case 1: backward_error_host<1>(an, xn, bn, rn, f, o); break;
+            case 2: backward_error_host<2>(an, xn, bn, rn, f, o); break;
+            case 3: backward_error_host<3>(an, xn, bn, rn, f, o); break;
+            default: backward_error_host<4>(an, xn, bn, rn, f, o); break;
+        }
+        for (int l = 0; l < N; ++l) out[l * nrhs + j] = o[l];
+    }
+    return MFLA_OK;
+}
+
+}  // extern "C"

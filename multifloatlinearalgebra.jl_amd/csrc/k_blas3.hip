@@ -1,0 +1,184 @@
+// GEMM-family tile kernel for Float64xN on CDNA4.
+//
+// One kernel body serves gemm!, syrk!, gemmt!, the trailing updates of the
+// factorizations and the running-value updates of the triangular solves:
+//
+//   MODE_AB : acc = 0; for k ascending acc = acc + a(i,k)*b(k,j);
+//             C = alpha*acc            (beta == 0, C never read)
+//             C = alpha*acc + beta*C   (otherwise)
+//             -- src/kernels/gemm.jl:151-181, syrk.jl:13-38, gemmt.jl:15-40
+//   MODE_SUB: v = C; for k ascending v = v - a(i,k)*b(k,j); C = v
+//             -- the running-value order of src/kernels/trsm.jl:40-47
+//
+// Each output element is owned by exactly one thread and reduced in ascending
+// k with one fixed network, so results are deterministic and independent of
+// the tiling. There is no split-K anywhere.
+//
+// Layout: a 256-thread workgroup computes a (16*TM) x (16*TN) tile; A and B
+// k-panels of depth BK are staged through LDS as [k][row][limb] so a thread's
+// TM (TN) consecutive elements are one contiguous LDS read. Operands are
+// strided views, so transposed operands (syrk's P^T P, gemmt's L R^T) need no
+// copies.
+#include "mf_common.hpp"
+#include "k_blas3.hpp"
+
+namespace mfk {
+
+template <int N, int NET, int TM, int TN, int BK, int MODE, bool LOWER>
+__global__ __launch_bounds__(256) void k_tile(int64_t n, int64_t K, DV A, DV B, DV C, S<N> alpha, S<N> beta,
+                                              int beta_zero, int64_t row_lo, int64_t row_hi, const Status *status) {
+    if (status && status->info != 0) return;
+    constexpr int BM = 16 * TM, BN = 16 * TN;
+    __shared__ double As[BK * BM * N];
+    __shared__ double Bs[BK * BN * N];
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int64_t i0 = row_lo + (int64_t)blockIdx.x * BM;
+    const int64_t j0 = (int64_t)blockIdx.y * BN;
+    if (LOWER && i0 + BM - 1 < j0) return;  // tile strictly above the diagonal
+
+    mf::V<N> acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            if (MODE == MODE_SUB) {
+                const int64_t i = i0 + tx * TM + a, j = j0 + ty * TN + b;
+                acc[a][b] = (i < row_hi && j < n) ? ld<N>(C, i, j) : mf::zero<N>();
+            } else {
+                acc[a][b] = mf::zero<N>();
+            }
+        }
+
+    const bool a_rowfast = (A.rs == 1);
+    const bool b_kfast = (B.rs == 1);
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+        // stage A panel: BM x BK elements
+        for (int e = tid; e < BM * BK; e += 256) {
+            int ii, kk;
+            if (a_rowfast) { ii = e % BM; kk = e / BM; } else { kk = e % BK; ii = e / BK; }
+            const int64_t gi = i0 + ii, gk = k0 + kk;
+            double *dst = &As[(kk * BM + ii) * N];
+            if (gi < row_hi && gk < K) {
+                const double *src = A.p + gi * A.rs + gk * A.cs;
+#pragma unroll
+                for (int l = 0; l < N; ++l) dst[l] = src[l * A.ls];
+            } else {
+#pragma unroll
+                for (int l = 0; l < N; ++l) dst[l] = 0.0;
+            }
+        }
+        // stage B panel: BK x BN elements
+        for (int e = tid; e < BN * BK; e += 256) {
+            int jj, kk;
+            if (b_kfast) { kk = e % BK; jj = e / BK; } else { jj = e % BN; kk = e / BN; }
+            const int64_t gj = j0 + jj, gk = k0 + kk;
+            double *dst = &Bs[(kk * BN + jj) * N];
+            if (gj < n && gk < K) {
+                const double *src = B.p + gk * B.rs + gj * B.cs;
+#pragma unroll
+                for (int l = 0; l < N; ++l) dst[l] = src[l * B.ls];
+            } else {
+#pragma unroll
+                for (int l = 0; l < N; ++l) dst[l] = 0.0;
+            }
+        }
+        __syncthreads();
+        const int kend = (K - k0) < BK ? (int)(K - k0) : BK;
+        for (int kk = 0; kk < kend; ++kk) {
+            mf::V<N> av[TM], bv[TN];
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int l = 0; l < N; ++l) av[a].x[l] = As[(kk * BM + tx * TM + a) * N + l];
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+#pragma unroll
+                for (int l = 0; l < N; ++l) bv[b].x[l] = Bs[(kk * BN + ty * TN + b) * N + l];
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b) {
+                    if (MODE == MODE_SUB) acc[a][b] = mf::msub(acc[a][b], av[a], bv[b]);
+                    else acc[a][b] = mf::Mac<N, NET>::apply(acc[a][b], av[a], bv[b]);
+                }
+        }
+        __syncthreads();
+    }
+
+    const mf::V<N> al = sv(alpha), be = sv(beta);
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            const int64_t i = i0 + tx * TM + a, j = j0 + ty * TN + b;
+            if (i >= row_hi || j >= n) continue;
+            if (LOWER && i < j) continue;
+            mf::V<N> r;
+            if (MODE == MODE_SUB) r = acc[a][b];
+            else if (beta_zero) r = mf::mul(al, acc[a][b]);
+            else r = mf::add(mf::mul(al, acc[a][b]), mf::mul(be, ld<N>(C, i, j)));
+            stv<N>(C, i, j, r);
+        }
+}
+
+// ---------------------------------------------------------------- launcher
+template <int N, int NET, int TM, int TN, int BK, int MODE, bool LOWER>
+static hipError_t go(hipStream_t s, int64_t rows, int64_t n, int64_t K, const DV &A, const DV &B, const DV &C,
+                     const S<N> &al, const S<N> &be, int bz, int64_t row_lo, const Status *st) {
+    constexpr int BM = 16 * TM, BN = 16 * TN;
+    dim3 grid((unsigned)cdiv(rows, BM), (unsigned)cdiv(n, BN));
+    hipLaunchKernelGGL((k_tile<N, NET, TM, TN, BK, MODE, LOWER>), grid, dim3(256), 0, s, n, K, A, B, C, al, be, bz, row_lo,
+                       row_lo + rows, st);
+    return hipGetLastError();
+}
+
+template <int N, int NET, int MODE, bool LOWER>
+static hipError_t pick(hipStream_t s, int tile, int64_t rows, int64_t n, int64_t K, const DV &A, const DV &B, const DV &C,
+                       const S<N> &al, const S<N> &be, int bz, int64_t row_lo, const Status *st) {
+    // tile 1: 32x32 tiles (2x2 per thread); tile 0: 16x16 tiles (1x1 per thread)
+    // for small outputs where 32x32 tiles would leave most CUs idle.
+    if (tile == 1) return go<N, NET, 2, 2, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st);
+    return go<N, NET, 1, 1, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st);
+}
+
+static int auto_tile(int64_t rows, int64_t n) {
+    // 32x32 tiles once they give at least ~2 workgroups per CU (256 CUs).
+    return (cdiv(rows, 32) * cdiv(n, 32) >= 512) ? 1 : 0;
+}
+
+template <int N>
+hipError_t tile_launch(hipStream_t s, const TileArgs &t, const DV &A, const DV &B, const DV &C, const double *alpha,
+                       const double *beta, const Status *st) {
+    if (t.rows <= 0 || t.n <= 0) return hipSuccess;
+    S<N> al, be;
+    for (int l = 0; l < N; ++l) { al.x[l] = alpha ? alpha[l] : 0.0; be.x[l] = beta ? beta[l] : 0.0; }
+    const int tile = t.tile >= 0 ? t.tile : auto_tile(t.rows, t.n);
+    const int bz = t.beta_zero;
+    if (t.mode == MODE_SUB) {
+        if (t.lower) return pick<N, mf::NET_STD, MODE_SUB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st);
+        return pick<N, mf::NET_STD, MODE_SUB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st);
+    }
+    if constexpr (N == 3) {
+        if (t.net == mf::NET_FUSED3) {
+            if (t.lower) return pick<3, mf::NET_FUSED3, MODE_AB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st);
+            return pick<3, mf::NET_FUSED3, MODE_AB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st);
+        }
+    }
+    if (t.lower) return pick<N, mf::NET_STD, MODE_AB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st);
+    return pick<N, mf::NET_STD, MODE_AB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st);
+}
+
+hipError_t tile_launch_n(int N, hipStream_t s, const TileArgs &t, const DV &A, const DV &B, const DV &C, const double *alpha,
+                         const double *beta, const Status *st) {
+    switch (N) {
+        case 1: return tile_launch<1>(s, t, A, B, C, alpha, beta, st);
+        case 2: return tile_launch<2>(s, t, A, B, C, alpha, beta, st);
+        case 3: return tile_launch<3>(s, t, A, B, C, alpha, beta, st);
+        case 4: return tile_launch<4>(s, t, A, B, C, alpha, beta, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace mfk

@@ -1,0 +1,26 @@
+// Host-side interface of the GEMM-family tile kernel (k_blas3.hip).
+#pragma once
+#include "mf_common.hpp"
+
+namespace mfk {
+
+enum TileMode { MODE_AB = 0, MODE_SUB = 1 };
+
+struct TileArgs {
+    int mode;          // MODE_AB or MODE_SUB
+    int net;           // mf::NET_STD or mf::NET_FUSED3 (x3, MODE_AB only)
+    int lower;         // write only elements with i >= j (absolute indices)
+    int beta_zero;     // MODE_AB: C is not read
+    int tile;          // -1 auto, 0: 16x16 tiles, 1: 32x32 tiles
+    int64_t row_lo;    // first output row handled by this launch
+    int64_t rows;      // number of output rows handled (from row_lo)
+    int64_t n;         // output columns
+    int64_t K;         // reduction length
+};
+
+// C(i, j) for i in [row_lo, row_lo + rows), j in [0, n) from a(i, k) = A(i, k)
+// and b(k, j) = B(k, j), all strided views. alpha/beta are host limb arrays.
+hipError_t tile_launch_n(int N, hipStream_t s, const TileArgs &t, const DV &A, const DV &B, const DV &C,
+                         const double *alpha, const double *beta, const Status *st);
+
+}  // namespace mfk

@@ -1,0 +1,605 @@
+// Factorization kernels for Float64xN: Cholesky panel, partial-pivoting LU
+// panel steps, Bunch-Kaufman LDLT (unblocked and lazy blocked panel), and the
+// pivot / block-diagonal steps of the solves.
+//
+// All kernels share the call's device Status word: they return immediately
+// once a previous kernel recorded a failure, so the matrix is left exactly as
+// MFLA leaves it when the factorization stops.
+#include "mf_common.hpp"
+#include "k_factor.hpp"
+#include <climits>
+
+namespace mfk {
+
+// ------------------------------------------------------------ utilities
+template <int N> __device__ __forceinline__ void lds_put(double *s, const mf::V<N> &v) {
+#pragma unroll
+    for (int l = 0; l < N; ++l) s[l] = v.x[l];
+}
+template <int N> __device__ __forceinline__ mf::V<N> lds_get(const double *s) {
+    mf::V<N> v;
+#pragma unroll
+    for (int l = 0; l < N; ++l) v.x[l] = s[l];
+    return v;
+}
+template <int N> __device__ __forceinline__ void swap_el(const DV &A, int64_t i1, int64_t j1, int64_t i2, int64_t j2) {
+    mf::V<N> t = ld<N>(A, i1, j1);
+    stv<N>(A, i1, j1, ld<N>(A, i2, j2));
+    stv<N>(A, i2, j2, t);
+}
+
+// Workgroup argmax of |value| with MFLA's first-strict-maximum rule: the
+// largest value wins, ties go to the smallest index (lu.jl:93-100,
+// ldlt.jl:192-200). Every thread passes its best (value, index); idx < 0 means
+// "no candidate". Result broadcast through LDS.
+template <int N>
+__device__ void wg_argmax(mf::V<N> &val, int64_t &idx, double *sv_buf, int64_t *si_buf) {
+    const int t = threadIdx.x, T = blockDim.x;
+    lds_put<N>(&sv_buf[t * N], val);
+    si_buf[t] = idx;
+    __syncthreads();
+    for (int s = T / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            mf::V<N> a = lds_get<N>(&sv_buf[t * N]), b = lds_get<N>(&sv_buf[(t + s) * N]);
+            int64_t ia = si_buf[t], ib = si_buf[t + s];
+            bool take_b;
+            if (ib < 0) take_b = false;
+            else if (ia < 0) take_b = true;
+            else if (mf::gt(b, a)) take_b = true;
+            else if (mf::eq(a, b)) take_b = ib < ia;
+            else take_b = false;
+            if (take_b) { lds_put<N>(&sv_buf[t * N], b); si_buf[t] = ib; }
+        }
+        __syncthreads();
+    }
+    val = lds_get<N>(&sv_buf[0]);
+    idx = si_buf[0];
+    __syncthreads();
+}
+
+// Workgroup max (Julia max semantics, value only).
+template <int N>
+__device__ mf::V<N> wg_max(mf::V<N> val, double *sv_buf) {
+    const int t = threadIdx.x, T = blockDim.x;
+    lds_put<N>(&sv_buf[t * N], val);
+    __syncthreads();
+    for (int s = T / 2; s > 0; s >>= 1) {
+        if (t < s) lds_put<N>(&sv_buf[t * N], mf::vmax(lds_get<N>(&sv_buf[t * N]), lds_get<N>(&sv_buf[(t + s) * N])));
+        __syncthreads();
+    }
+    mf::V<N> r = lds_get<N>(&sv_buf[0]);
+    __syncthreads();
+    return r;
+}
+
+__device__ int64_t wg_min_i64(int64_t v, int64_t *si_buf) {
+    const int t = threadIdx.x, T = blockDim.x;
+    si_buf[t] = v;
+    __syncthreads();
+    for (int s = T / 2; s > 0; s >>= 1) {
+        if (t < s && si_buf[t + s] < si_buf[t]) si_buf[t] = si_buf[t + s];
+        __syncthreads();
+    }
+    int64_t r = si_buf[0];
+    __syncthreads();
+    return r;
+}
+
+// -------------------------------------------------------------- Cholesky
+// src/factorizations/cholesky.jl:10-34, one workgroup per panel.
+template <int N>
+__global__ __launch_bounds__(256) void k_chol_panel(DV A, int64_t first, int64_t last, Status *st) {
+    if (st->info != 0) return;
+    __shared__ double piv_s[N];
+    __shared__ int fail_s;
+    for (int64_t c = first; c <= last; ++c) {
+        if (threadIdx.x == 0) {
+            mf::V<N> d = ld<N>(A, c, c);
+            for (int64_t k = first; k < c; ++k) { mf::V<N> v = ld<N>(A, c, k); d = mf::msub(d, v, v); }
+            if (!mf::gt(d, mf::zero<N>())) {
+                fail_s = 1;
+                st->info = c + 1;
+            } else {
+                fail_s = 0;
+                mf::V<N> p = mf::sqrt(d);
+                stv<N>(A, c, c, p);
+                lds_put<N>(piv_s, p);
+            }
+        }
+        __syncthreads();
+        if (fail_s) return;
+        const mf::V<N> p = lds_get<N>(piv_s);
+        for (int64_t r = c + 1 + threadIdx.x; r <= last; r += blockDim.x) {
+            mf::V<N> v = ld<N>(A, r, c);
+            for (int64_t k = first; k < c; ++k) v = mf::msub(v, ld<N>(A, r, k), ld<N>(A, c, k));
+            stv<N>(A, r, c, mf::div(v, p));
+        }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------------- LU
+// lu.jl:91-102: pivot search over rows k..m-1 of column k, record ipiv, then
+// swap whole rows across all n columns. One workgroup.
+template <int N>
+__global__ __launch_bounds__(1024) void k_lu_pivot_swap(DV A, int64_t k, int64_t m, int64_t n, int64_t *ipiv, Status *st) {
+    if (st->info != 0) return;
+    __shared__ double sv_buf[1024 * N];
+    __shared__ int64_t si_buf[1024];
+    mf::V<N> best = mf::zero<N>();
+    int64_t bi = -1;
+    for (int64_t r = k + threadIdx.x; r < m; r += blockDim.x) {
+        mf::V<N> c = mf::vabs(ld<N>(A, r, k));
+        if (bi < 0 || mf::gt(c, best)) { best = c; bi = r; }
+    }
+    wg_argmax<N>(best, bi, sv_buf, si_buf);
+    const int64_t piv = bi;
+    if (threadIdx.x == 0) { ipiv[k] = piv + 1; st->aux = piv; }
+    if (piv != k)
+        for (int64_t j = threadIdx.x; j < n; j += blockDim.x) swap_el<N>(A, k, j, piv, j);
+}
+
+// lu.jl:104-109 + rank-1 update lu.jl:47-82 restricted to the panel columns.
+template <int N>
+__global__ __launch_bounds__(256) void k_lu_scale_update(DV A, int64_t k, int64_t last, int64_t m, Status *st) {
+    if (st->info != 0) return;
+    const mf::V<N> akk = ld<N>(A, k, k);
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (mf::is_zero(akk)) {
+        if (gid == 0) st->info = k + 1;
+        return;
+    }
+    const int64_t r = k + 1 + gid;
+    if (r >= m) return;
+    const mf::V<N> inv = mf::inv(akk);
+    mf::V<N> l = mf::mul(ld<N>(A, r, k), inv);
+    stv<N>(A, r, k, l);
+    for (int64_t c = k + 1; c <= last; ++c) stv<N>(A, r, c, mf::msub(ld<N>(A, r, c), l, ld<N>(A, k, c)));
+}
+
+// Sequential row interchanges on right-hand sides (solve.jl:85-113).
+template <int N>
+__global__ void k_apply_ipiv(DV B, int64_t count, int64_t nrhs, const int64_t *ipiv) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nrhs) return;
+    for (int64_t k = 0; k < count; ++k) {
+        const int64_t p = ipiv[k] - 1;
+        if (p != k) swap_el<N>(B, k, c, p, c);
+    }
+}
+
+// ------------------------------------------------------------------ LDLT
+// ldlt.jl:65-101 (scaled 2x2 elimination)
+template <int N>
+__device__ bool solve2x2(const mf::V<N> &d11, const mf::V<N> &d21, const mf::V<N> &d22, const mf::V<N> &f, const mf::V<N> &s,
+                         mf::V<N> &x1, mf::V<N> &x2) {
+    mf::V<N> scale = mf::vmax(mf::vmax(mf::vabs(d11), mf::vabs(d21)), mf::vabs(d22));
+    x1 = mf::zero<N>(); x2 = mf::zero<N>();
+    if (mf::is_zero(scale) || !mf::is_finite(scale)) return false;
+    mf::V<N> a = mf::div(d11, scale), b = mf::div(d21, scale), c = mf::div(d22, scale);
+    mf::V<N> r1 = mf::div(f, scale), r2 = mf::div(s, scale);
+    if (mf::ge(mf::vabs(a), mf::vabs(b))) {
+        if (mf::is_zero(a)) return false;
+        mf::V<N> t = mf::div(b, a);
+        mf::V<N> u = mf::msub(c, t, b);
+        if (mf::is_zero(u) || !mf::is_finite(u)) return false;
+        x2 = mf::div(mf::msub(r2, t, r1), u);
+        x1 = mf::div(mf::msub(r1, b, x2), a);
+        return true;
+    }
+    if (mf::is_zero(b)) return false;
+    mf::V<N> t = mf::div(a, b);
+    mf::V<N> u = mf::msub(b, t, c);
+    if (mf::is_zero(u) || !mf::is_finite(u)) return false;
+    x2 = mf::div(mf::msub(r1, t, r2), u);
+    x1 = mf::div(mf::msub(r2, c, x2), b);
+    return true;
+}
+
+// ldlt.jl:155-181, executed cooperatively by one workgroup.
+template <int N>
+__device__ void wg_sym_swap(const DV &A, int64_t n, int64_t f, int64_t s, int64_t active) {
+    if (f == s) return;
+    for (int64_t c = threadIdx.x; c < n; c += blockDim.x) swap_el<N>(A, f, c, s, c);
+    __syncthreads();
+    for (int64_t r = active + threadIdx.x; r < n; r += blockDim.x) swap_el<N>(A, r, f, r, s);
+    __syncthreads();
+}
+
+// Shared pieces of the Bunch-Kaufman pivot choice (ldlt.jl:183-219 and the
+// lazy variant :451-508). Entry(i1, i2) evaluates the current Schur entry.
+struct BKChoice { int bs; int64_t pivot; };
+
+template <int N, typename Entry>
+__device__ BKChoice wg_select_bk(int64_t n, int64_t k, const mf::V<N> &alpha, Entry entry, double *sv_buf, int64_t *si_buf) {
+    const mf::V<N> absakk = mf::vabs(entry(k, k));
+    if (k == n - 1) return BKChoice{mf::is_zero(absakk) ? 0 : 1, k};
+    mf::V<N> best = mf::zero<N>();
+    int64_t bi = -1;
+    for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) {
+        mf::V<N> c = mf::vabs(entry(r, k));
+        if (bi < 0 || mf::gt(c, best)) { best = c; bi = r; }
+    }
+    wg_argmax<N>(best, bi, sv_buf, si_buf);
+    const mf::V<N> colmax = best;
+    const int64_t imax = bi;
+    if (mf::eq(mf::vmax(absakk, colmax), mf::zero<N>())) return BKChoice{0, k};
+    if (mf::ge(absakk, mf::mul(alpha, colmax))) return BKChoice{1, k};
+    mf::V<N> rm = mf::zero<N>();
+    for (int64_t c = k + threadIdx.x; c < imax; c += blockDim.x) rm = mf::vmax(rm, mf::vabs(entry(imax, c)));
+    for (int64_t r = imax + 1 + threadIdx.x; r < n; r += blockDim.x) rm = mf::vmax(rm, mf::vabs(entry(r, imax)));
+    const mf::V<N> rowmax = wg_max<N>(rm, sv_buf);
+    if (mf::ge(absakk, mf::mul(mf::mul(alpha, colmax), mf::div(colmax, rowmax)))) return BKChoice{1, k};
+    if (mf::ge(mf::vabs(entry(imax, imax)), mf::mul(alpha, rowmax))) return BKChoice{1, imax};
+    return BKChoice{2, imax};
+}
+
+// Unblocked right-looking Bunch-Kaufman LDLT (ldlt.jl:352-412), one
+// workgroup per matrix. Matrices are addressed through a batch stride so the
+// same body serves single and batched factorizations.
+template <int N>
+__device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks, const mf::V<N> &alpha,
+                                    int64_t *info_out, double *sv_buf, int64_t *si_buf) {
+    auto entry = [&](int64_t i, int64_t j) { return ld<N>(A, i, j); };
+    int64_t k = 0;
+    while (k < n) {
+        BKChoice ch = wg_select_bk<N>(n, k, alpha, entry, sv_buf, si_buf);
+        if (ch.bs == 0) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+        if (ch.bs == 1) {
+            if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 1; }
+            wg_sym_swap<N>(A, n, k, ch.pivot, k);
+            const mf::V<N> d = ld<N>(A, k, k);
+            if (mf::is_zero(d)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+            for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(ld<N>(A, r, k), d));
+            __syncthreads();
+            // rank-1 update (ldlt.jl:221-262), both triangles written
+            const int64_t w = n - k - 1;
+            for (int64_t e = threadIdx.x; e < w * w; e += blockDim.x) {
+                const int64_t col = k + 1 + e / w, row = k + 1 + e % w;
+                if (row < col) continue;
+                const mf::V<N> coef = mf::mul(d, ld<N>(A, col, k));
+                const mf::V<N> v = mf::msub(ld<N>(A, row, col), ld<N>(A, row, k), coef);
+                stv<N>(A, row, col, v);
+                stv<N>(A, col, row, v);
+            }
+            __syncthreads();
+            k += 1;
+        } else {
+            if (!(k < n - 1)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+            if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 2; blocks[k + 1] = 0; }
+            wg_sym_swap<N>(A, n, k + 1, ch.pivot, k);
+            const mf::V<N> d11 = ld<N>(A, k, k), d21 = ld<N>(A, k + 1, k), d22 = ld<N>(A, k + 1, k + 1);
+            if (threadIdx.x == 0) stv<N>(dsub, k, 0, d21);
+            mf::V<N> s1, s2;
+            if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+            // rows k+2..n-1: find the first failing row, write only rows before it
+            int64_t myfail = INT64_MAX;
+            for (int64_t r = k + 2 + threadIdx.x; r < n; r += blockDim.x) {
+                bool ok = solve2x2<N>(d11, d21, d22, ld<N>(A, r, k), ld<N>(A, r, k + 1), s1, s2);
+                if (!(ok && mf::is_finite(s1) && mf::is_finite(s2)) && r < myfail) myfail = r;
+            }
+            const int64_t firstfail = wg_min_i64(myfail, si_buf);
+            for (int64_t r = k + 2 + threadIdx.x; r < n && r < firstfail; r += blockDim.x) {
+                solve2x2<N>(d11, d21, d22, ld<N>(A, r, k), ld<N>(A, r, k + 1), s1, s2);
+                stv<N>(A, r, k, s1);
+                stv<N>(A, r, k + 1, s2);
+            }
+            __syncthreads();
+            if (firstfail != INT64_MAX) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+            // rank-2 update (ldlt.jl:264-319): vector form on four-row blocks,
+            // scalar form on the remainder rows of each column
+            if (k + 1 < n - 1) {
+                const int64_t w = n - k - 2;
+                for (int64_t e = threadIdx.x; e < w * w; e += blockDim.x) {
+                    const int64_t col = k + 2 + e / w, row = k + 2 + e % w;
+                    if (row < col) continue;
+                    const mf::V<N> l1 = ld<N>(A, col, k), l2 = ld<N>(A, col, k + 1);
+                    const mf::V<N> c1 = mf::add(mf::mul(d11, l1), mf::mul(d21, l2));
+                    const mf::V<N> c2 = mf::add(mf::mul(d21, l1), mf::mul(d22, l2));
+                    const int64_t vec_end = col + ((n - col) / 4) * 4;
+                    mf::V<N> v;
+                    if (row < vec_end) v = mf::sub(ld<N>(A, row, col), mf::add(mf::mul(ld<N>(A, row, k), c1), mf::mul(ld<N>(A, row, k + 1), c2)));
+                    else v = mf::msub(mf::msub(ld<N>(A, row, col), ld<N>(A, row, k), c1), ld<N>(A, row, k + 1), c2);
+                    stv<N>(A, row, col, v);
+                    stv<N>(A, col, row, v);
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) { stv<N>(A, k + 1, k, mf::zero<N>()); stv<N>(A, k, k + 1, mf::zero<N>()); }
+            __syncthreads();
+            k += 2;
+        }
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(512) void k_ldlt_unblocked(DV A, int64_t n, DV dsub, int64_t *pivots, uint8_t *blocks, S<N> alpha,
+                                                        Status *st) {
+    if (st->info != 0) return;
+    __shared__ double sv_buf[512 * N];
+    __shared__ int64_t si_buf[512];
+    __shared__ int64_t info_s;
+    if (threadIdx.x == 0) info_s = 0;
+    __syncthreads();
+    ldlt_unblocked_body<N>(A, n, dsub, pivots, blocks, sv(alpha), &info_s, sv_buf, si_buf);
+    __syncthreads();
+    if (threadIdx.x == 0 && info_s != 0) st->info = info_s;
+}
+
+// Lazy blocked panel (ldlt.jl:414-593). Writes info to st->info and the
+// (possibly extended) last panel column to st->aux.
+template <int N>
+__global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfirst, int64_t reqlast, DV dsub, int64_t *pivots,
+                                                    uint8_t *blocks, S<N> alpha_s, Status *st) {
+    if (st->info != 0) return;
+    __shared__ double sv_buf[512 * N];
+    __shared__ int64_t si_buf[512];
+    const mf::V<N> alpha = sv(alpha_s);
+    int64_t plast = reqlast;
+    int64_t k = pfirst;
+    int64_t info = 0;
+    while (k <= plast) {
+        // entry(i1, i2) against pivots pfirst..k-1 of this panel
+        const int64_t kk = k;
+        auto entry = [&](int64_t i1, int64_t i2) {
+            const int64_t row = i1 > i2 ? i1 : i2, col = i1 > i2 ? i2 : i1;
+            mf::V<N> v = ld<N>(A, row, col);
+            int64_t q = pfirst;
+            while (q < kk) {
+                if (blocks[q] == 1) {
+                    v = mf::sub(v, mf::mul(ld<N>(A, row, q), mf::mul(ld<N>(A, q, q), ld<N>(A, col, q))));
+                    q += 1;
+                } else {
+                    const mf::V<N> rf = ld<N>(A, row, q), rs = ld<N>(A, row, q + 1), cf = ld<N>(A, col, q), cs = ld<N>(A, col, q + 1);
+                    const mf::V<N> dq = ld<N>(dsub, q, 0);
+                    const mf::V<N> c1 = mf::add(mf::mul(ld<N>(A, q, q), cf), mf::mul(dq, cs));
+                    const mf::V<N> c2 = mf::add(mf::mul(dq, cf), mf::mul(ld<N>(A, q + 1, q + 1), cs));
+                    v = mf::sub(v, mf::add(mf::mul(rf, c1), mf::mul(rs, c2)));
+                    q += 2;
+                }
+            }
+            return v;
+        };
+        BKChoice ch = wg_select_bk<N>(n, k, alpha, entry, sv_buf, si_buf);
+        if (ch.bs == 0) { info = k + 1; break; }
+        if (ch.bs == 2 && k == plast) {
+            if (!(k < n - 1)) { info = k + 1; break; }
+            plast += 1;
+        }
+        if (ch.bs == 1) {
+            if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 1; }
+            __syncthreads();
+            wg_sym_swap<N>(A, n, k, ch.pivot, k);
+            const mf::V<N> d = entry(k, k);
+            if (mf::is_zero(d)) { info = k + 1; break; }
+            __syncthreads();
+            if (threadIdx.x == 0) stv<N>(A, k, k, d);
+            for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(entry(r, k), d));
+            __syncthreads();
+            k += 1;
+        } else {
+            if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 2; blocks[k + 1] = 0; }
+            __syncthreads();
+            wg_sym_swap<N>(A, n, k + 1, ch.pivot, k);
+            const mf::V<N> d11 = entry(k, k), d21 = entry(k + 1, k), d22 = entry(k + 1, k + 1);
+            __syncthreads();
+            if (threadIdx.x == 0) { stv<N>(A, k, k, d11); stv<N>(A, k + 1, k + 1, d22); stv<N>(dsub, k, 0, d21); }
+            __syncthreads();
+            mf::V<N> s1, s2;
+            if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) { info = k + 1; break; }
+            int64_t myfail = INT64_MAX;
+            for (int64_t r = k + 2 + threadIdx.x; r < n; r += blockDim.x) {
+                bool ok = solve2x2<N>(d11, d21, d22, entry(r, k), entry(r, k + 1), s1, s2);
+                if (!(ok && mf::is_finite(s1) && mf::is_finite(s2)) && r < myfail) myfail = r;
+            }
+            const int64_t firstfail = wg_min_i64(myfail, si_buf);
+            for (int64_t r = k + 2 + threadIdx.x; r < n && r < firstfail; r += blockDim.x) {
+                mf::V<N> f = entry(r, k), s = entry(r, k + 1);
+                solve2x2<N>(d11, d21, d22, f, s, s1, s2);
+                stv<N>(A, r, k, s1);
+                stv<N>(A, r, k + 1, s2);
+            }
+            __syncthreads();
+            if (firstfail != INT64_MAX) { info = k + 1; break; }
+            if (threadIdx.x == 0) { stv<N>(A, k + 1, k, mf::zero<N>()); stv<N>(A, k, k + 1, mf::zero<N>()); }
+            __syncthreads();
+            k += 2;
+        }
+    }
+    if (threadIdx.x == 0) { st->info = info; st->aux = plast; }
+}
+
+// ldlt.jl:595-632: weighted panel W(r, local) = L21 * D for the trailing rows.
+template <int N>
+__global__ void k_ldlt_weighted(DV A, int64_t pfirst, int64_t plast, int64_t tfirst, int64_t tcount, DV dsub, const uint8_t *blocks,
+                                DV W, const Status *st) {
+    if (st->info != 0) return;
+    const int64_t lr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lr >= tcount) return;
+    const int64_t row = tfirst + lr;
+    int64_t q = pfirst;
+    while (q <= plast) {
+        const int64_t lc = q - pfirst;
+        if (blocks[q] == 1) {
+            stv<N>(W, lr, lc, mf::mul(ld<N>(A, row, q), ld<N>(A, q, q)));
+            q += 1;
+        } else {
+            const mf::V<N> d11 = ld<N>(A, q, q), d21 = ld<N>(dsub, q, 0), d22 = ld<N>(A, q + 1, q + 1);
+            const mf::V<N> f = ld<N>(A, row, q), s = ld<N>(A, row, q + 1);
+            stv<N>(W, lr, lc, mf::add(mf::mul(f, d11), mf::mul(s, d21)));
+            stv<N>(W, lr, lc + 1, mf::add(mf::mul(f, d21), mf::mul(s, d22)));
+            q += 2;
+        }
+    }
+}
+
+// ldlt.jl:141-153: A[c, r] = A[r, c] for r > c over an n x n view.
+template <int N>
+__global__ void k_mirror_lower(DV A, int64_t n, const Status *st) {
+    if (st && st->info != 0) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t r = e % n, c = e / n;
+    if (r > c) stv<N>(A, c, r, ld<N>(A, r, c));
+}
+
+// solve.jl:207-241: LDLT interchanges on right-hand sides, one thread per column.
+template <int N>
+__global__ void k_ldlt_pivots(DV B, int64_t n, int64_t nrhs, const int64_t *pivots, const uint8_t *blocks, int forward, int *bad) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nrhs) return;
+    if (forward) {
+        int64_t k = 0;
+        while (k < n) {
+            if (blocks[k] == 1) { int64_t p = pivots[k] - 1; if (p != k) swap_el<N>(B, k, c, p, c); k += 1; }
+            else if (blocks[k] == 2) { int64_t p = pivots[k] - 1; if (p != k + 1) swap_el<N>(B, k + 1, c, p, c); k += 2; }
+            else { *bad = 1; return; }
+        }
+    } else {
+        int64_t k = n - 1;
+        while (k >= 0) {
+            if (blocks[k] == 1) { int64_t p = pivots[k] - 1; if (p != k) swap_el<N>(B, k, c, p, c); k -= 1; }
+            else if (blocks[k] == 0 && k > 0 && blocks[k - 1] == 2) {
+                int64_t s0 = k - 1, p = pivots[s0] - 1;
+                if (p != s0 + 1) swap_el<N>(B, s0 + 1, c, p, c);
+                k -= 2;
+            } else { *bad = 1; return; }
+        }
+    }
+}
+
+// solve.jl:268-296: block-diagonal solve, one thread per (block start, column).
+template <int N>
+__global__ void k_ldlt_dsolve(DV B, int64_t n, int64_t nrhs, DV F, DV dsub, const uint8_t *blocks, Status *st) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * nrhs) return;
+    const int64_t k = e % n, c = e / n;
+    if (blocks[k] == 1) {
+        stv<N>(B, k, c, mf::div(ld<N>(B, k, c), ld<N>(F, k, k)));
+    } else if (blocks[k] == 2) {
+        mf::V<N> s1, s2;
+        if (!solve2x2<N>(ld<N>(F, k, k), ld<N>(dsub, k, 0), ld<N>(F, k + 1, k + 1), ld<N>(B, k, c), ld<N>(B, k + 1, c), s1, s2)) {
+            st->info = k + 1;
+            return;
+        }
+        stv<N>(B, k, c, s1);
+        stv<N>(B, k + 1, c, s2);
+    }
+}
+
+// Column-wise max |A| over rows [rlo(c), m): rlo = c for the lower triangle.
+template <int N>
+__global__ void k_col_absmax_region(DV A, int64_t m, int lower_only, DV out) {
+    const int64_t c = blockIdx.x;
+    __shared__ double sv_buf[256 * N];
+    mf::V<N> mx = mf::zero<N>();
+    // lower_only: 0 all rows, 1 rows >= c, 2 rows <= c
+    const int64_t lo = lower_only == 1 ? c : 0;
+    const int64_t hi = lower_only == 2 ? (c + 1 < m ? c + 1 : m) : m;
+    for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) mx = mf::vmax(mx, mf::vabs(ld<N>(A, r, c)));
+    mx = wg_max<N>(mx, sv_buf);
+    if (threadIdx.x == 0) stv<N>(out, c, 0, mx);
+}
+
+__global__ void k_iota1(int64_t *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i + 1;
+}
+__global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ----------------------------------------------------------- launchers
+#define MF_DISPATCH(N, ...)                                        \
+    switch (N) {                                                   \
+        case 1: { constexpr int NN = 1; __VA_ARGS__; } break;      \
+        case 2: { constexpr int NN = 2; __VA_ARGS__; } break;      \
+        case 3: { constexpr int NN = 3; __VA_ARGS__; } break;      \
+        case 4: { constexpr int NN = 4; __VA_ARGS__; } break;      \
+        default: return hipErrorInvalidValue;                      \
+    }
+static inline unsigned nb(int64_t work, int b) { return (unsigned)cdiv(work > 0 ? work : 1, b); }
+template <int N> static S<N> mk(const double *v) { S<N> s; for (int l = 0; l < N; ++l) s.x[l] = v[l]; return s; }
+
+hipError_t chol_panel(int N, hipStream_t s, const DV &A, int64_t first, int64_t last, Status *st) {
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_chol_panel<NN>, dim3(1), dim3(256), 0, s, A, first, last, st));
+    return hipGetLastError();
+}
+
+hipError_t lu_pivot_swap(int N, hipStream_t s, const DV &A, int64_t k, int64_t m, int64_t n, int64_t *ipiv, Status *st) {
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_pivot_swap<NN>, dim3(1), dim3(1024), 0, s, A, k, m, n, ipiv, st));
+    return hipGetLastError();
+}
+
+hipError_t lu_scale_update(int N, hipStream_t s, const DV &A, int64_t k, int64_t last, int64_t m, Status *st) {
+    const int64_t rows = m - k - 1;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_scale_update<NN>, dim3(nb(rows, 256)), dim3(256), 0, s, A, k, last, m, st));
+    return hipGetLastError();
+}
+
+hipError_t apply_ipiv(int N, hipStream_t s, const DV &B, int64_t count, int64_t nrhs, const int64_t *ipiv) {
+    if (nrhs == 0 || count == 0) return hipSuccess;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_apply_ipiv<NN>, dim3(nb(nrhs, 64)), dim3(64), 0, s, B, count, nrhs, ipiv));
+    return hipGetLastError();
+}
+
+hipError_t ldlt_unblocked(int N, hipStream_t s, const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks,
+                          const double *alpha, Status *st) {
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_ldlt_unblocked<NN>, dim3(1), dim3(512), 0, s, A, n, dsub, pivots, blocks, mk<NN>(alpha), st));
+    return hipGetLastError();
+}
+
+hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
+                      uint8_t *blocks, const double *alpha, Status *st) {
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_ldlt_panel<NN>, dim3(1), dim3(512), 0, s, A, n, pfirst, reqlast, dsub, pivots, blocks,
+                                      mk<NN>(alpha), st));
+    return hipGetLastError();
+}
+
+hipError_t ldlt_weighted(int N, hipStream_t s, const DV &A, int64_t pfirst, int64_t plast, int64_t tfirst, int64_t tcount,
+                         const DV &dsub, const uint8_t *blocks, const DV &W, const Status *st) {
+    if (tcount <= 0) return hipSuccess;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_ldlt_weighted<NN>, dim3(nb(tcount, 256)), dim3(256), 0, s, A, pfirst, plast, tfirst, tcount,
+                                      dsub, blocks, W, st));
+    return hipGetLastError();
+}
+
+hipError_t mirror_lower(int N, hipStream_t s, const DV &A, int64_t n, const Status *st) {
+    if (n <= 1) return hipSuccess;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_mirror_lower<NN>, dim3(nb(n * n, 256)), dim3(256), 0, s, A, n, st));
+    return hipGetLastError();
+}
+
+hipError_t ldlt_pivots(int N, hipStream_t s, const DV &B, int64_t n, int64_t nrhs, const int64_t *pivots, const uint8_t *blocks,
+                       int forward, int *bad) {
+    if (nrhs == 0 || n == 0) return hipSuccess;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_ldlt_pivots<NN>, dim3(nb(nrhs, 64)), dim3(64), 0, s, B, n, nrhs, pivots, blocks, forward, bad));
+    return hipGetLastError();
+}
+
+hipError_t ldlt_dsolve(int N, hipStream_t s, const DV &B, int64_t n, int64_t nrhs, const DV &F, const DV &dsub, const uint8_t *blocks,
+                       Status *st) {
+    if (nrhs == 0 || n == 0) return hipSuccess;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_ldlt_dsolve<NN>, dim3(nb(n * nrhs, 256)), dim3(256), 0, s, B, n, nrhs, F, dsub, blocks, st));
+    return hipGetLastError();
+}
+
+hipError_t col_absmax_region(int N, hipStream_t s, const DV &A, int64_t m, int64_t ncols, int lower_only, const DV &out) {
+    if (ncols == 0) return hipSuccess;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_col_absmax_region<NN>, dim3((unsigned)ncols), dim3(256), 0, s, A, m, lower_only, out));
+    return hipGetLastError();
+}
+
+hipError_t iota1(hipStream_t s, int64_t *p, int64_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iota1, dim3(nb(n, 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+hipError_t fill_u8(hipStream_t s, uint8_t *p, int64_t n, uint8_t v) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u8, dim3(nb(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace mfk

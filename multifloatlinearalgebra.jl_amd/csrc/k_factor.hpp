@@ -1,0 +1,26 @@
+// Host-side interface of the factorization kernels (k_factor.hip).
+#pragma once
+#include "mf_common.hpp"
+
+namespace mfk {
+
+hipError_t chol_panel(int N, hipStream_t s, const DV &A, int64_t first, int64_t last, Status *st);
+hipError_t lu_pivot_swap(int N, hipStream_t s, const DV &A, int64_t k, int64_t m, int64_t n, int64_t *ipiv, Status *st);
+hipError_t lu_scale_update(int N, hipStream_t s, const DV &A, int64_t k, int64_t last, int64_t m, Status *st);
+hipError_t apply_ipiv(int N, hipStream_t s, const DV &B, int64_t count, int64_t nrhs, const int64_t *ipiv);
+hipError_t ldlt_unblocked(int N, hipStream_t s, const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks,
+                          const double *alpha, Status *st);
+hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
+                      uint8_t *blocks, const double *alpha, Status *st);
+hipError_t ldlt_weighted(int N, hipStream_t s, const DV &A, int64_t pfirst, int64_t plast, int64_t tfirst, int64_t tcount,
+                         const DV &dsub, const uint8_t *blocks, const DV &W, const Status *st);
+hipError_t mirror_lower(int N, hipStream_t s, const DV &A, int64_t n, const Status *st);
+hipError_t ldlt_pivots(int N, hipStream_t s, const DV &B, int64_t n, int64_t nrhs, const int64_t *pivots, const uint8_t *blocks,
+                       int forward, int *bad);
+hipError_t ldlt_dsolve(int N, hipStream_t s, const DV &B, int64_t n, int64_t nrhs, const DV &F, const DV &dsub, const uint8_t *blocks,
+                       Status *st);
+hipError_t col_absmax_region(int N, hipStream_t s, const DV &A, int64_t m, int64_t ncols, int lower_only, const DV &out);
+hipError_t iota1(hipStream_t s, int64_t *p, int64_t n);
+hipError_t fill_u8(hipStream_t s, uint8_t *p, int64_t n, uint8_t v);
+
+}  // namespace mfk

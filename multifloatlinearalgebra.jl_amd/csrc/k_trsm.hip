@@ -1,0 +1,133 @@
+// Triangular solves for Float64xN (trsm!/trsv!, src/kernels/trsm.jl,
+// src/kernels/trsv.jl).
+//
+// Every solve is expressed on a strided "solution view" X(p, c) (p: unknown
+// index, c: right-hand side) and a coefficient view CA(p, q) (the coefficient
+// of X(q) in equation p), so left/right and N/T variants are the same kernels
+// with different strides. MFLA's element order is
+//     v = X(p); for q in substitution order (ascending): v = v - CA(p,q)*X(q);
+//     v = v / A(p,p) unless unit
+// (trsm.jl:40-50, 63-73, 136-149, 159-172).
+//
+// Forward substitution (dependencies on q < p) is blocked right-looking:
+// a systolic diagonal-block solve (one pivot per step, all later rows of the
+// block subtract it) followed by a MODE_SUB tile update of the rows below that
+// subtracts the block's terms in ascending q into the running values. Every
+// element therefore sees exactly MFLA's ascending-q chain: bitwise equal.
+//
+// Backward substitution with MFLA's ascending-q chain has an O(n^2) serial
+// dependency per right-hand side (row p's first term needs X(p+1), the last
+// unknown solved). Two orders are provided: the exact MFLA order
+// (k_trsm_seq, one thread per right-hand side) and a blocked right-looking
+// order (descending blocks) whose rounding differs from MFLA's but stays
+// within the same c*n*u bound.
+#include "mf_common.hpp"
+#include "k_blas3.hpp"
+#include "k_trsm.hpp"
+
+namespace mfk {
+
+constexpr int TB = 64;   // diagonal block rows (threads in x)
+constexpr int TC = 4;    // right-hand sides per workgroup (threads in y)
+
+template <int N, bool FWD>
+__global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit,
+                                                      const Status *st) {
+    if (st && st->info != 0) return;
+    __shared__ double xs[TC * N];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int64_t c = (int64_t)blockIdx.x * TC + ty;
+    const bool valid = (tx < nbk) && (c < nrhs);
+    const int64_t p = p0 + tx;
+    mf::V<N> v = valid ? ld<N>(X, p, c) : mf::zero<N>();
+    for (int s0 = 0; s0 < nbk; ++s0) {
+        const int s = FWD ? s0 : nbk - 1 - s0;
+        const int64_t q = p0 + s;
+        if (tx == s && valid) {
+            if (!unit) v = mf::div(v, ld<N>(D, q, q));
+            stv<N>(X, q, c, v);
+#pragma unroll
+            for (int l = 0; l < N; ++l) xs[ty * N + l] = v.x[l];
+        }
+        __syncthreads();
+        if (valid && (FWD ? tx > s : tx < s)) {
+            mf::V<N> xq;
+#pragma unroll
+            for (int l = 0; l < N; ++l) xq.x[l] = xs[ty * N + l];
+            v = mf::msub(v, ld<N>(CA, p, q), xq);
+        }
+        __syncthreads();
+    }
+}
+
+// Exact MFLA order for backward substitution: one thread per right-hand side.
+template <int N>
+__global__ void k_trsm_seq_bwd(int64_t n, int64_t nrhs, DV X, DV CA, DV D, int unit, const Status *st) {
+    if (st && st->info != 0) return;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nrhs) return;
+    for (int64_t p = n - 1; p >= 0; --p) {
+        mf::V<N> v = ld<N>(X, p, c);
+        for (int64_t q = p + 1; q < n; ++q) v = mf::msub(v, ld<N>(CA, p, q), ld<N>(X, q, c));
+        if (!unit) v = mf::div(v, ld<N>(D, p, p));
+        stv<N>(X, p, c, v);
+    }
+}
+
+static DV sub(const DV &v, int64_t i, int64_t j) { return DV{v.p + i * v.rs + j * v.cs, v.rs, v.cs, v.ls}; }
+
+template <int N>
+static hipError_t diag(hipStream_t s, bool fwd, int64_t p0, int nbk, int64_t nrhs, const DV &X, const DV &CA, const DV &D, int unit,
+                       const Status *st) {
+    dim3 grid((unsigned)cdiv(nrhs, TC)), block(TB, TC);
+    if (fwd) hipLaunchKernelGGL((k_trsm_diag<N, true>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, st);
+    else hipLaunchKernelGGL((k_trsm_diag<N, false>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, st);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const DV &CA, const DV &D, const Status *st) {
+    const int64_t n = a.n, nrhs = a.nrhs;
+    if (n == 0 || nrhs == 0) return hipSuccess;
+    hipError_t e = hipSuccess;
+    TileArgs t{};
+    t.mode = MODE_SUB; t.net = mf::NET_STD; t.lower = 0; t.beta_zero = 0; t.tile = -1; t.row_lo = 0; t.n = nrhs;
+    if (a.forward) {
+        for (int64_t p0 = 0; p0 < n; p0 += TB) {
+            const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
+            if ((e = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, st)) != hipSuccess) return e;
+            const int64_t p1 = p0 + nbk;
+            if (p1 < n) {
+                t.rows = n - p1; t.K = nbk;
+                if ((e = tile_launch_n(N, s, t, sub(CA, p1, p0), sub(X, p0, 0), sub(X, p1, 0), nullptr, nullptr, st)) != hipSuccess) return e;
+            }
+        }
+        return hipSuccess;
+    }
+    if (a.exact_order) {
+        hipLaunchKernelGGL(k_trsm_seq_bwd<N>, dim3((unsigned)cdiv(nrhs, 64)), dim3(64), 0, s, n, nrhs, X, CA, D, a.unit, st);
+        return hipGetLastError();
+    }
+    for (int64_t p1 = n; p1 > 0; p1 -= TB) {
+        const int64_t p0 = p1 > TB ? p1 - TB : 0;
+        const int nbk = (int)(p1 - p0);
+        if ((e = diag<N>(s, false, p0, nbk, nrhs, X, CA, D, a.unit, st)) != hipSuccess) return e;
+        if (p0 > 0) {
+            t.rows = p0; t.K = nbk;
+            if ((e = tile_launch_n(N, s, t, sub(CA, 0, p0), sub(X, p0, 0), X, nullptr, nullptr, st)) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+hipError_t trsm_solve(int N, hipStream_t s, const TrsmArgs &a, const DV &X, const DV &CA, const DV &D, const Status *st) {
+    switch (N) {
+        case 1: return solve_t<1>(s, a, X, CA, D, st);
+        case 2: return solve_t<2>(s, a, X, CA, D, st);
+        case 3: return solve_t<3>(s, a, X, CA, D, st);
+        case 4: return solve_t<4>(s, a, X, CA, D, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace mfk

@@ -1,0 +1,19 @@
+// Host-side interface of the triangular-solve kernels (k_trsm.hip).
+#pragma once
+#include "mf_common.hpp"
+
+namespace mfk {
+
+struct TrsmArgs {
+    int64_t n;        // unknowns per right-hand side
+    int64_t nrhs;     // right-hand sides
+    int forward;      // 1: dependencies on q < p; 0: on q > p
+    int unit;         // unit diagonal (no division)
+    int exact_order;  // backward only: 1 = MFLA's ascending-q chain, 0 = blocked
+};
+
+// Solves in place on the solution view X(p, c) with coefficient view CA(p, q)
+// and diagonal view D(p, p).
+hipError_t trsm_solve(int N, hipStream_t s, const TrsmArgs &a, const DV &X, const DV &CA, const DV &D, const Status *st);
+
+}  // namespace mfk

@@ -1,0 +1,16 @@
+"""mfla -- MI355X (gfx950) device backend for MultiFloatLinearAlgebra.jl's API.
+
+Python host mirror of the reference's public surface; every numerical
+operation runs in libmfla_hip.so on the GPU (see include/mfla_hip.h).
+"""
+from ._lib import LIB_PATH, load  # noqa: F401
+from .api import (  # noqa: F401
+    ArgumentError, DimensionMismatch, DomainError, KernelConfig, MFCholesky, MFLDLT, MFLU, MFWorkspace, PosDefException,
+    SingularException, arith, cholesky_, ensure_workspace_capacity_, from_interleaved, gemm_, gemmt_, gemv_, host_arith, ldiv_, ldlt_,
+    lu_, normwise_backward_error, refinement_correction_, residual_, residual_mixed_, solve, symv_, synchronize, syrk_, to_interleaved,
+    trsm_, trsv_, workspace_capacity,
+)
+from .factor import (  # noqa: F401
+    factor_blocks, factor_diagnostics, factor_inertia, factor_kind, factor_matrix, factor_permutation, factor_pivots,
+    factor_precision, factor_provider, factor_state, factor_status, issuccess, numerical_rank,
+)

@@ -1,0 +1,239 @@
+"""GPU parity of the BLAS-like kernels against the oracle (bitwise).
+
+Every kernel here reduces each output element in MFLA's ascending order with
+the same multi-double network as the oracle, so any difference is a bug.
+Shapes include odd sizes, sizes that are not multiples of the tiles, empty
+dimensions and the beta == 0 NaN-destination case (runtests.jl:315-396).
+"""
+import numpy as np
+import pytest
+import torch
+
+from mfutil import assert_bitwise, eps, rand_mf
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def mfla():
+    import mfla as M
+
+    assert torch.cuda.is_available()
+    return M
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_device_networks_equal_oracle(mfla, oracle, N):
+    rng = np.random.default_rng(900 + N)
+    a, b, c = (rand_mf(rng, N, 5000, spread=30) for _ in range(3))
+    b[:, :300] = -a[:, :300]
+    pos = a * np.sign(a[0])
+    for op in ("add", "sub", "mul", "div", "mac", "msub", "inv"):
+        got = host(mfla.arith(op, dev(a), dev(b), dev(c)))
+        assert_bitwise(got, oracle.arith(op, a, b, c), f"x{N} {op}")
+    assert_bitwise(host(mfla.arith("sqrt", dev(pos))), oracle.arith("sqrt", pos), f"x{N} sqrt")
+    if N == 3:
+        assert_bitwise(host(mfla.arith("mulacc_x3", dev(a), dev(b), dev(c))), oracle.arith("mulacc_x3", a, b, c), "fused")
+
+
+def test_device_proof_vectors(mfla):
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "mulacc_x3_proof_vectors.json")) as f:
+        pv = json.load(f)
+    f64 = lambda k: np.stack([np.array(v[k], dtype=np.uint64).view(np.float64) for v in pv], axis=1)  # noqa: E731
+    acc, x, y, out = f64("acc"), f64("x"), f64("y"), f64("output")
+    assert_bitwise(host(mfla.arith("mac", dev(x), dev(y), dev(acc))), out, "standard x3 mac")
+    assert_bitwise(host(mfla.arith("mulacc_x3", dev(x), dev(y), dev(acc))), out, "mulacc_x3")
+
+
+GEMM_SHAPES = [(1, 1, 1), (5, 3, 7), (33, 17, 31), (64, 64, 64), (70, 45, 129), (130, 9, 66), (4, 0, 5), (0, 3, 4)]
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("shape", GEMM_SHAPES)
+def test_gemm_matches_oracle(mfla, oracle, N, shape):
+    m, k, n = shape
+    rng = np.random.default_rng(m * 1000 + k * 10 + n + N)
+    A, B, C = rand_mf(rng, N, m, k), rand_mf(rng, N, k, n), rand_mf(rng, N, m, n)
+    alpha = rand_mf(rng, N, 1)[:, 0]
+    beta = rand_mf(rng, N, 1)[:, 0]
+    for a_, b_ in ((1.0, 0.0), (alpha, beta), (-1.0, 1.0)):
+        strategies = ["auto", "direct", "fused"] if N == 3 else ["auto"]
+        for strat in strategies:
+            C0 = C.copy()
+            if np.all(np.asarray(b_) == 0):
+                C0[:] = np.nan  # beta == 0: the destination is never read
+            Cd = dev(C0)
+            mfla.gemm_(Cd, dev(A), dev(B), a_, b_, config=mfla.KernelConfig(gemm_strategy=strat))
+            ref = oracle.gemm(C0.copy(), A, B, a_, b_, fused=(strat != "direct"))
+            assert_bitwise(host(Cd), ref, f"x{N} {shape} {strat}")
+
+
+@pytest.mark.parametrize("tile", [0, 1])
+def test_gemm_tile_shapes_bitwise_identical(mfla, tile):
+    rng = np.random.default_rng(5)
+    A, B = rand_mf(rng, 4, 97, 40), rand_mf(rng, 4, 40, 75)
+    out = []
+    for t in (tile, 1 - tile):
+        C = torch.zeros((4, 75, 97), dtype=torch.float64, device="cuda")
+        mfla.gemm_(C, dev(A), dev(B), config=mfla.KernelConfig(gemm_tile=t))
+        out.append(host(C))
+    assert_bitwise(out[0], out[1], "tile independence")
+
+
+def test_gemm_dimension_and_alias_errors(mfla):
+    A = torch.zeros((2, 3, 4), dtype=torch.float64, device="cuda")
+    B = torch.zeros((2, 5, 3), dtype=torch.float64, device="cuda")
+    with pytest.raises(mfla.DimensionMismatch):
+        mfla.gemm_(torch.zeros((2, 5, 5), dtype=torch.float64, device="cuda"), A, B)
+    with pytest.raises(mfla.DimensionMismatch):
+        mfla.gemm_(torch.zeros((2, 5, 4), dtype=torch.float64, device="cuda"), A, A)
+    S = torch.zeros((2, 4, 4), dtype=torch.float64, device="cuda")
+    with pytest.raises(mfla.ArgumentError):
+        mfla.gemm_(S, S, S)
+    with pytest.raises(mfla.ArgumentError):
+        mfla.gemm_(torch.zeros((2, 5, 4), dtype=torch.float64, device="cuda"), A, B, config=mfla.KernelConfig(gemm_strategy="fused"))
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("n,k", [(1, 1), (23, 5), (70, 33), (97, 16)])
+def test_syrk_gemmt_lower_only(mfla, oracle, N, n, k):
+    rng = np.random.default_rng(77 + n + k + N)
+    P = rand_mf(rng, N, k, n)
+    Lm, R = rand_mf(rng, N, n, k), rand_mf(rng, N, n, k)
+    C = rand_mf(rng, N, n, n)
+    iu = np.triu_indices(n, 1)
+    C[:, iu[1], iu[0]] = np.nan           # upper triangle (row < col) must be preserved
+    for a_, b_ in ((1.0, 0.0), (-1.0, 1.0), (rand_mf(rng, N, 1)[:, 0], rand_mf(rng, N, 1)[:, 0])):
+        Cd = dev(C)
+        mfla.syrk_(Cd, dev(P), a_, b_)
+        got = host(Cd)
+        ref = oracle.syrk(C.copy(), P, a_, b_)
+        assert_bitwise(got, ref, f"syrk x{N}")
+        assert np.all(np.isnan(got[:, iu[1], iu[0]]))
+        Cd = dev(C)
+        mfla.gemmt_(Cd, dev(Lm), dev(R), a_, b_)
+        assert_bitwise(host(Cd), oracle.gemmt(C.copy(), Lm, R, a_, b_), f"gemmt x{N}")
+
+
+def _triangular(rng, N, n, uplo, unit):
+    A = rand_mf(rng, N, n, n)
+    r, c = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")   # A[:, col, row]
+    inactive = (c < r) if uplo == "lower" else (c > r)              # planar index [l, j, i]
+    A[:, inactive] = 0.0
+    idx = np.arange(n)
+    A[0, idx, idx] = np.abs(A[0, idx, idx]) + 3.0          # well conditioned diagonal
+    A[1:, idx, idx] = 0.0
+    return A
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+@pytest.mark.parametrize("side", ["left", "right"])
+@pytest.mark.parametrize("uplo", ["lower", "upper"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["nonunit", "unit"])
+def test_trsm_all_variants_exact_order(mfla, oracle, N, side, uplo, trans, diag):
+    rng = np.random.default_rng(hash((N, side, uplo, trans, diag)) % 2**32)
+    n, other = 70, 9
+    A = _triangular(rng, N, n, uplo, diag == "unit")
+    B = rand_mf(rng, N, n, other) if side == "left" else rand_mf(rng, N, other, n)
+    alpha = 1.0 if diag == "unit" else rand_mf(rng, N, 1)[:, 0]
+    Bd = dev(B)
+    mfla.trsm_(Bd, dev(A), alpha, side=side, uplo=uplo, trans=trans, diag=diag, config=mfla.KernelConfig(trsm_order="exact"))
+    ref = oracle.trsm(B.copy(), A, alpha, side, uplo, trans, diag)
+    assert_bitwise(host(Bd), ref, f"trsm x{N} {side} {uplo} {trans} {diag}")
+
+
+@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("uplo,trans", [("upper", "N"), ("lower", "T")])
+def test_trsm_blocked_backward_within_tolerance(mfla, oracle, N, uplo, trans):
+    rng = np.random.default_rng(31 + N)
+    n, nrhs = 200, 5
+    A = _triangular(rng, N, n, uplo, False)
+    B = rand_mf(rng, N, n, nrhs)
+    Bd = dev(B)
+    mfla.trsm_(Bd, dev(A), side="left", uplo=uplo, trans=trans, config=mfla.KernelConfig(trsm_order="blocked"))
+    X = host(Bd)
+    ref = oracle.trsm(B.copy(), A, 1.0, "left", uplo, trans, "nonunit")
+    # blocked order rounds differently: compare to the oracle and check the
+    # exact residual B - op(A) X with the long accumulator
+    rows = np.repeat(np.arange(n), nrhs)
+    cols = np.tile(np.arange(nrhs), n)
+    At = np.ascontiguousarray(np.transpose(A, (0, 2, 1))) if trans == "T" else A
+    val, scale = oracle.exact_axpb(At, X, B, sab=-1, sc=1, rows=rows, cols=cols)
+    assert np.max(np.abs(val) / scale) <= 4096 * n * eps(N)
+    d = np.abs(X.sum(0) - ref.sum(0)) / np.maximum(np.abs(ref.sum(0)), 1)
+    assert d.max() <= 4096 * n * eps(N) * 1e3
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_trsv_equals_one_column_trsm(mfla, oracle, N):
+    # runtests.jl:683-690
+    rng = np.random.default_rng(60 + N)
+    n = 90
+    for uplo in ("lower", "upper"):
+        for trans in ("N", "T"):
+            A = _triangular(rng, N, n, uplo, False)
+            b = rand_mf(rng, N, n)
+            x = dev(b)
+            mfla.trsv_(x, dev(A), uplo=uplo, trans=trans)
+            Bm = dev(b.reshape(N, 1, n))
+            mfla.trsm_(Bm, dev(A), uplo=uplo, trans=trans)
+            assert_bitwise(host(x), host(Bm).reshape(N, n), "trsv vs trsm")
+            assert_bitwise(host(x), oracle.trsv(b.copy(), A, 1.0, uplo, trans), "trsv vs oracle")
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_gemv_symv(mfla, oracle, N):
+    rng = np.random.default_rng(11 + N)
+    m, n = 77, 45
+    A = rand_mf(rng, N, m, n)
+    for trans in ("N", "T"):
+        x = rand_mf(rng, N, n if trans == "N" else m)
+        y = rand_mf(rng, N, m if trans == "N" else n)
+        for a_, b_ in ((1.0, 0.0), (-1.0, 1.0)):
+            y0 = y.copy()
+            if b_ == 0.0:
+                y0[:] = np.nan
+            yd = dev(y0)
+            mfla.gemv_(yd, dev(A), dev(x), a_, b_, trans=trans)
+            assert_bitwise(host(yd), oracle.gemv(y0.copy(), A, x, a_, b_, trans), f"gemv {trans}")
+    S = rand_mf(rng, N, 60, 60)
+    x = rand_mf(rng, N, 60)
+    for uplo in ("lower", "upper"):
+        S0 = S.copy()
+        il = np.tril_indices(60, -1) if uplo == "upper" else np.triu_indices(60, 1)
+        S0[:, il[1], il[0]] = np.nan       # inactive triangle is never read
+        y = rand_mf(rng, N, 60)
+        yd = dev(y)
+        mfla.symv_(yd, dev(S0), dev(x), -1.0, 1.0, uplo=uplo)
+        assert_bitwise(host(yd), oracle.symv(y.copy(), S0, x, -1.0, 1.0, uplo), f"symv {uplo}")
+
+
+def test_gemm_x4_n1024_sampled_columns(mfla, oracle):
+    # Full-size property at n = 1024: sampled output columns equal the oracle
+    # bitwise (per-element results do not depend on other columns).
+    rng = np.random.default_rng(1024)
+    n = 1024
+    A, B = rand_mf(rng, 4, n, n), rand_mf(rng, 4, n, n)
+    Cd = torch.empty((4, n, n), dtype=torch.float64, device="cuda")
+    mfla.gemm_(Cd, dev(A), dev(B))
+    C = host(Cd)
+    cols = [0, 1, 511, 1023]
+    Bs = np.ascontiguousarray(B[:, cols, :])
+    ref = oracle.gemm(np.zeros((4, len(cols), n)), A, Bs)
+    assert_bitwise(C[:, cols, :], ref, "n=1024 sampled columns")
+    val, scale = oracle.exact_axpb(A, Bs, ref, sab=1, sc=-1, rows=np.arange(0, n, 97).repeat(len(cols)),
+                                   cols=np.tile(np.arange(len(cols)), len(range(0, n, 97))))
+    assert np.max(np.abs(val) / scale) <= 8 * n * eps(4)

@@ -1,0 +1,219 @@
+"""GPU parity of the factorizations, solves and residual path.
+
+* Cholesky / LU / LDLT on seeded matrices: factors, pivots, blocks, D and
+  status bitwise equal to the oracle (same per-element order and networks),
+  across limb counts, block sizes and sizes that are not block multiples.
+* Solves bitwise equal to the oracle in MFLA's exact substitution order.
+* The reference's known-answer tests (tests/kat_cases.py) on the GPU.
+* Residual, mixed residual and backward error bitwise equal to the oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from backends import GpuBackend
+from kat_cases import run_all
+from mfutil import assert_bitwise, rand_mf
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mfla
+
+    assert torch.cuda.is_available()
+    return mfla
+
+
+def spd(oracle, rng, N, n):
+    """S = R R^T + n I formed in Float64xN by the oracle (full symmetric)."""
+    R = rand_mf(rng, N, n, n)
+    S = np.zeros((N, n, n))
+    oracle.gemm(S, R, np.ascontiguousarray(np.transpose(R, (0, 2, 1))), 1.0, 0.0, fused=False)
+    shift = np.zeros((N, n))
+    shift[0] = float(n)
+    diag = np.ascontiguousarray(S[:, np.arange(n), np.arange(n)])
+    S[:, np.arange(n), np.arange(n)] = oracle.arith("add", diag, shift)
+    return S
+
+
+def kkt(rng, N, m):
+    p = m // 2
+    H = rand_mf(rng, N, m, m)
+    Hs = H + np.transpose(H, (0, 2, 1))
+    Hs[0, np.arange(m), np.arange(m)] += 2.0 * m
+    A = rand_mf(rng, N, p, m)
+    n = m + p
+    K = np.zeros((N, n, n))
+    K[:, :m, :m] = Hs
+    K[:, :m, m:] = A                            # rows m.., columns 0..m-1: A
+    K[:, m:, :m] = np.transpose(A, (0, 2, 1))   # rows 0..m-1, columns m..: A^T
+    K[0, np.arange(m, n), np.arange(m, n)] = -1e-8
+    return K
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_reference_kats_on_gpu(M, N):
+    res = run_all(GpuBackend(), N)
+    failed = [m for ok, m in res if not ok]
+    assert not failed, failed
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("n,block", [(37, 16), (64, 16), (100, 7), (130, 32)])
+def test_cholesky_bitwise(M, oracle, N, n, block):
+    rng = np.random.default_rng(n * 10 + N + block)
+    S = spd(oracle, rng, N, n)
+    S[:, np.triu_indices(n, 1)[1], np.triu_indices(n, 1)[0]] = np.nan     # only the lower triangle is read
+    ref = S.copy()
+    info_ref = oracle.potrf(ref, block)
+    Sd = dev(S)
+    F = M.cholesky_(Sd, config=M.KernelConfig(cholesky_block=block))
+    assert F.info == info_ref == 0
+    assert_bitwise(host(Sd), ref, f"cholesky x{N} n={n} b={block}")
+    B = rand_mf(rng, N, n, 3)
+    Xref = oracle.potrs(ref, B.copy())
+    X = M.solve(F, dev(B), config=M.KernelConfig(trsm_order="exact"))
+    assert_bitwise(host(X), Xref, "potrs")
+    b = rand_mf(rng, N, n)
+    x = M.solve(F, dev(b), config=M.KernelConfig(trsm_order="exact"))
+    assert_bitwise(host(x), oracle.potrs(ref, b.copy()), "potrs vector")
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("m,n,block", [(40, 40, 16), (67, 67, 16), (90, 50, 8), (50, 90, 16), (129, 129, 32)])
+def test_lu_bitwise(M, oracle, N, m, n, block):
+    rng = np.random.default_rng(m * 100 + n + N + block)
+    A = rand_mf(rng, N, m, n)
+    ref = A.copy()
+    info_ref, ipiv_ref, om_ref = oracle.getrf(ref, block)
+    Ad = dev(A)
+    F = M.lu_(Ad, config=M.KernelConfig(lu_block=block))
+    assert F.info == info_ref == 0
+    assert np.array_equal(F.ipiv.cpu().numpy(), ipiv_ref)
+    assert_bitwise(F.original_maximum, om_ref, "original maximum")
+    assert_bitwise(host(Ad), ref, f"lu x{N} {m}x{n} b={block}")
+    if m == n:
+        B = rand_mf(rng, N, n, 4)
+        X = M.solve(F, dev(B), config=M.KernelConfig(trsm_order="exact"))
+        assert_bitwise(host(X), oracle.getrs(ref, ipiv_ref, B.copy()), "getrs")
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+@pytest.mark.parametrize("m,strategy,block", [(20, "unblocked", 0), (60, "unblocked", 0), (60, "blocked", 0), (90, "blocked", 5)])
+def test_ldlt_kkt_bitwise(M, oracle, N, m, strategy, block):
+    rng = np.random.default_rng(m + N * 7 + block)
+    K = kkt(rng, N, m)
+    n = K.shape[1]
+    ref = K.copy()
+    info_ref, dsub_ref, piv_ref, blk_ref, om_ref = oracle.sytrf(ref, strategy, block, 1)
+    Kd = dev(K)
+    F = M.ldlt_(Kd, config=M.KernelConfig(ldlt_strategy=strategy, ldlt_block=block, ldlt_blocked_crossover=1))
+    assert F.info == info_ref == 0
+    assert np.array_equal(F.pivots.cpu().numpy(), piv_ref), "pivots"
+    assert np.array_equal(F.blocks.cpu().numpy(), blk_ref), "blocks"
+    assert_bitwise(host(F.dsub), dsub_ref, "dsub")
+    assert_bitwise(host(Kd), ref, f"ldlt x{N} n={n} {strategy}")
+    B = rand_mf(rng, N, n, 4)
+    X = M.solve(F, dev(B), config=M.KernelConfig(trsm_order="exact"))
+    assert_bitwise(host(X), oracle.sytrs(ref, dsub_ref, piv_ref, blk_ref, B.copy()), "sytrs")
+    inertia = M.factor_inertia(F)
+    assert inertia["positive"] == m and inertia["negative"] == n - m and inertia["zero"] == 0
+
+
+def test_status_codes_and_check(M):
+    A = torch.zeros((2, 3, 3), dtype=torch.float64, device="cuda")
+    A[0, 1, 2] = float("nan")                       # (row 2, col 1): lower triangle
+    F = M.cholesky_(A.clone(), check=False)
+    assert F.info == -1 and M.factor_state(F) == "nonfinite_input"
+    with pytest.raises(M.DomainError):
+        M.cholesky_(A.clone())
+    A2 = torch.zeros((2, 3, 3), dtype=torch.float64, device="cuda")
+    A2[0, 2, 0] = float("nan")                      # (row 0, col 2): upper -- never read
+    A2[0, [0, 1, 2], [0, 1, 2]] = 1.0
+    assert M.cholesky_(A2.clone()).info == 0
+    L = torch.zeros((2, 3, 3), dtype=torch.float64, device="cuda")
+    L[0, 0, 0] = 1.0
+    Fl = M.lu_(L, check=False)
+    assert Fl.info == 2 and M.factor_state(Fl) == "singular"
+    with pytest.raises(M.SingularException):
+        M.lu_(L.clone())
+    # a failed factor raises before the destination is touched (solve.jl:381-400)
+    b = torch.ones((2, 3), dtype=torch.float64, device="cuda")
+    dst = torch.full((2, 3), 7.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(M.SingularException):
+        M.ldiv_(dst, Fl, b)
+    assert torch.all(dst == 7.0)
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_residual_paths_bitwise(M, oracle, N):
+    rng = np.random.default_rng(55 + N)
+    m, n = 41, 29
+    A, x, b = rand_mf(rng, N, m, n), rand_mf(rng, N, n), rand_mf(rng, N, m)
+    r = torch.full((N, m), float("nan"), dtype=torch.float64, device="cuda")
+    M.residual_(r, dev(A), dev(x), dev(b))
+    assert_bitwise(host(r), oracle.gemv(b.copy(), A, x, -1.0, 1.0), "residual vector (gemv)")
+    X, B = rand_mf(rng, N, n, 3), rand_mf(rng, N, m, 3)
+    R = torch.full((N, 3, m), float("nan"), dtype=torch.float64, device="cuda")
+    M.residual_(R, dev(A), dev(X), dev(B))
+    assert_bitwise(host(R), oracle.gemm(B.copy(), A, X, -1.0, 1.0, fused=True), "residual matrix (gemm)")
+    S = rand_mf(rng, N, n, n)
+    rs = dev(b[:, :n].copy())
+    M.residual_(rs, dev(S), dev(x), rs, uplo="lower")          # in place on b
+    assert_bitwise(host(rs), oracle.symv(b[:, :n].copy(), S, x, -1.0, 1.0, "lower"), "residual symmetric")
+    be = M.normwise_backward_error(dev(A), dev(X), dev(B), R)
+    ref = oracle.backward_error(A, X, B, host(R))
+    assert_bitwise(be, ref, "backward error")
+    if N < 4:
+        for NR in range(N + 1, 5):
+            Rm = torch.zeros((NR, 3, m), dtype=torch.float64, device="cuda")
+            M.residual_mixed_(Rm, dev(A), dev(X), dev(B))
+            assert_bitwise(host(Rm), oracle.residual_mixed(np.zeros((NR, 3, m)), A, X, B), f"mixed x{N}->x{NR}")
+
+
+def test_workspace_reuse_bitwise(M, oracle):
+    rng = np.random.default_rng(3)
+    ws = M.MFWorkspace(3, factor_capacity=10, ldlt_block_capacity=4)
+    K = kkt(rng, 3, 40)
+    outs = []
+    for w in (None, ws, ws):
+        Kd = dev(K)
+        F = M.ldlt_(Kd, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_block=6, ldlt_blocked_crossover=1), workspace=w)
+        outs.append(host(Kd))
+        assert F.info == 0
+    M.ensure_workspace_capacity_(ws, factor_capacity=100, ldlt_block_capacity=6)
+    assert M.workspace_capacity(ws) == {"factor": 100, "ldlt_block": 6}
+    Kd = dev(K)
+    M.ldlt_(Kd, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_block=6, ldlt_blocked_crossover=1), workspace=ws)
+    outs.append(host(Kd))
+    for o in outs[1:]:
+        assert_bitwise(o, outs[0], "workspace reuse")
+
+
+def test_diagnostics_fields(M, oracle):
+    rng = np.random.default_rng(8)
+    S = spd(oracle, rng, 4, 30)
+    F = M.cholesky_(dev(S))
+    d = M.factor_diagnostics(F)
+    assert d["kind"] == "cholesky" and d["success"] and d["accepted_pivots"] == 30 and d["finite"]
+    assert d["state"] == "success" and d["failure_location"] is None
+    spread = M.host_arith("div", d["maximum_diagonal"].reshape(4, 1), d["minimum_diagonal"].reshape(4, 1))[:, 0]
+    assert np.array_equal(spread, d["diagonal_spread"])
+    A = rand_mf(rng, 4, 20, 20)
+    A[0, np.arange(20), np.arange(20)] += 4.0
+    Fl = M.lu_(dev(A))
+    dl = M.factor_diagnostics(Fl)
+    info, ipiv, om = oracle.getrf(A.copy(), 16)
+    assert np.array_equal(dl["original_maximum"], om) and np.array_equal(dl["pivots"], ipiv)
+    assert M.numerical_rank(Fl) == 20
