@@ -205,6 +205,11 @@ int mfla_residual_vector(mfla_handle h, int limbs, int uplo, mfla_matrix r, mfla
  * 1->2/3/4 (extension used by FP64-factor refinement). */
 int mfla_residual_mixed(mfla_handle h, int src_limbs, int res_limbs, int uplo, mfla_matrix R, mfla_matrix A, mfla_matrix X,
                         mfla_matrix B, const mfla_config *config);
+/* X = X + D with D (src_limbs <= dst_limbs) zero-extended to X's precision:
+ * the correction-accumulation step of iterative refinement. MFLA leaves the
+ * refinement loop to the caller (residual.jl:294-300, approximation .+=
+ * correction in test/runtests.jl:2132); this is that update on the device. */
+int mfla_add_mixed(mfla_handle h, int dst_limbs, int src_limbs, mfla_matrix X, mfla_matrix D);
 /* normwise_backward_error (src/residual.jl:215-269): out is a host array of
  * limbs x nrhs binary64 (limb-planar), one value per right-hand side */
 int mfla_backward_error(mfla_handle h, int limbs, int uplo, mfla_matrix A, mfla_matrix X, mfla_matrix B, mfla_matrix R, double *out);

@@ -149,6 +149,9 @@ hipError_t lower_launch(mfla_handle h, int N, int tile, int64_t n, int64_t k, co
     return mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
 }
 
+// Kernel-side symmetric-access codes: 0 general, 1 lower, 2 upper.
+int kernel_uplo(int uplo) { return uplo == MFLA_GENERAL ? 0 : (uplo == MFLA_LOWER ? 1 : 2); }
+
 bool trsm_exact(const mfla_config &c, int64_t n) {
     if (c.trsm_order == MFLA_TRSM_ORDER_EXACT) return true;
     if (c.trsm_order == MFLA_TRSM_ORDER_BLOCKED) return false;
@@ -781,7 +784,18 @@ int mfla_residual_mixed(mfla_handle h, int NS, int NR, int uplo, mfla_matrix R, 
     if (extents_overlap(R, A, NR > NS ? NS : NR) || extents_overlap(R, X, NS) || extents_overlap(R, B, NS))
         return fail(MFLA_ERR_ALIAS, "mixed residual output must not alias source operands");
     std::lock_guard<std::mutex> g(h->lock);
-    HIPCHK(mfk::residual_mixed(NS, NR, h->stream, uplo, rows, cols, R.cols, dvn(A), dvn(X), dvn(B), dvn(R)));
+    HIPCHK(mfk::residual_mixed(NS, NR, h->stream, kernel_uplo(uplo), rows, cols, R.cols, dvn(A), dvn(X), dvn(B), dvn(R)));
+    return MFLA_OK;
+}
+
+int mfla_add_mixed(mfla_handle h, int dst_limbs, int src_limbs, mfla_matrix X, mfla_matrix D) {
+    if (check_limbs(dst_limbs) || check_limbs(src_limbs)) return MFLA_ERR_ARGUMENT;
+    if (src_limbs > dst_limbs) return fail(MFLA_ERR_ARGUMENT, "correction precision exceeds the solution precision");
+    if (X.rows != D.rows || X.cols != D.cols) return fail(MFLA_ERR_DIMENSION, "correction dimensions differ");
+    if (check_desc(X, "X") || check_desc(D, "D")) return MFLA_ERR_ARGUMENT;
+    if (extents_overlap(X, D, src_limbs)) return fail(MFLA_ERR_ALIAS, "correction must not alias the solution");
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(mfk::add_mixed(src_limbs, dst_limbs, h->stream, X.rows, X.cols, dvn(X), dvn(D)));
     return MFLA_OK;
 }
 
@@ -803,8 +817,8 @@ int mfla_backward_error(mfla_handle h, int N, int uplo, mfla_matrix A, mfla_matr
     double *xm = rows + (size_t)N * m, *bm = xm + (size_t)N * nrhs, *rm = bm + (size_t)N * nrhs;
     int *fin = (int *)(rm + (size_t)N * nrhs);
     HIPCHK(mfk::status_reset(s, h->status));
-    HIPCHK(mfk::finite_check(N, s, m, n, dvn(A), uplo == MFLA_GENERAL ? 0 : (uplo == MFLA_LOWER ? 1 : 2), h->status));
-    HIPCHK(mfk::row_abs_sum(N, s, uplo, m, n, dvn(A), DV{rows, 1, m, m}));
+    HIPCHK(mfk::finite_check(N, s, m, n, dvn(A), kernel_uplo(uplo), h->status));
+    HIPCHK(mfk::row_abs_sum(N, s, kernel_uplo(uplo), m, n, dvn(A), DV{rows, 1, m, m}));
     HIPCHK(mfk::col_absmax(N, s, n, nrhs, dvn(X), DV{xm, 1, nrhs, nrhs}, fin));
     HIPCHK(mfk::col_absmax(N, s, m, nrhs, dvn(B), DV{bm, 1, nrhs, nrhs}, fin + nrhs));
     HIPCHK(mfk::col_absmax(N, s, m, nrhs, dvn(R), DV{rm, 1, nrhs, nrhs}, fin + 2 * nrhs));

@@ -162,6 +162,20 @@ __global__ __launch_bounds__(256) void k_residual_mixed(int uplo, int64_t m, int
     stv<NR>(R, row, c, acc);
 }
 
+// X (NR limbs) = X + promote(D) (NS <= NR limbs): correction accumulation of
+// iterative refinement in the higher precision.
+template <int NS, int NR>
+__global__ void k_add_mixed(int64_t m, int64_t n, DV X, DV D) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const int64_t i = e % m, j = e / m;
+    mf::V<NS> d = ld<NS>(D, i, j);
+    mf::V<NR> p = mf::zero<NR>();
+#pragma unroll
+    for (int l = 0; l < NS; ++l) p.x[l] = d.x[l];
+    stv<NR>(X, i, j, mf::add(ld<NR>(X, i, j), p));
+}
+
 // ----------------------------------------------------- norm reductions
 // Row sums of |A| (ascending column, src/residual.jl:143-164) per row.
 template <int N>
@@ -321,6 +335,25 @@ hipError_t residual_mixed(int NS, int NR, hipStream_t s, int uplo, int64_t m, in
     else if (NS == 1 && NR == 4) RM(1, 4);
     else return hipErrorInvalidValue;
 #undef RM
+    return hipGetLastError();
+}
+
+hipError_t add_mixed(int NS, int NR, hipStream_t s, int64_t m, int64_t n, const DV &X, const DV &D) {
+    if (m * n == 0) return hipSuccess;
+    const dim3 g(nb256(m * n));
+#define AM(a, b) hipLaunchKernelGGL((k_add_mixed<a, b>), g, dim3(256), 0, s, m, n, X, D)
+    if (NS == 1 && NR == 2) AM(1, 2);
+    else if (NS == 1 && NR == 3) AM(1, 3);
+    else if (NS == 1 && NR == 4) AM(1, 4);
+    else if (NS == 2 && NR == 3) AM(2, 3);
+    else if (NS == 2 && NR == 4) AM(2, 4);
+    else if (NS == 3 && NR == 4) AM(3, 4);
+    else if (NS == NR && NS == 1) AM(1, 1);
+    else if (NS == NR && NS == 2) AM(2, 2);
+    else if (NS == NR && NS == 3) AM(3, 3);
+    else if (NS == NR && NS == 4) AM(4, 4);
+    else return hipErrorInvalidValue;
+#undef AM
     return hipGetLastError();
 }
 

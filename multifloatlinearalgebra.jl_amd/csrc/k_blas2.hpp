@@ -20,6 +20,7 @@ hipError_t symv(int N, hipStream_t s, int upper, int64_t n, int64_t nrhs, const 
                 const double *beta, int beta_zero);
 hipError_t residual_mixed(int NS, int NR, hipStream_t s, int uplo, int64_t m, int64_t n, int64_t nrhs, const DV &A, const DV &X,
                           const DV &B, const DV &R);
+hipError_t add_mixed(int NS, int NR, hipStream_t s, int64_t m, int64_t n, const DV &X, const DV &D);
 hipError_t row_abs_sum(int N, hipStream_t s, int uplo, int64_t m, int64_t n, const DV &A, const DV &out);
 hipError_t col_absmax(int N, hipStream_t s, int64_t m, int64_t ncols, const DV &X, const DV &out, int *fin);
 

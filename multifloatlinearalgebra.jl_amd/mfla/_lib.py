@@ -90,6 +90,7 @@ PROTOTYPES = {
     "mfla_residual": (_I, [_P, _I, _I, _M, _M, _M, _M, _CP]),
     "mfla_residual_vector": (_I, [_P, _I, _I, _M, _M, _M, _M, _CP]),
     "mfla_residual_mixed": (_I, [_P, _I, _I, _I, _M, _M, _M, _M, _CP]),
+    "mfla_add_mixed": (_I, [_P, _I, _I, _M, _M]),
     "mfla_backward_error": (_I, [_P, _I, _I, _M, _M, _M, _M, _DP]),
 }
 

@@ -430,6 +430,13 @@ def residual_mixed_(r, A, x, b, *, uplo="general", config=None):
     return r
 
 
+def add_mixed_(X, D):
+    """X .+= D with D promoted to X's limb count (refinement update)."""
+    NR, NS = X.shape[0], D.shape[0]
+    _raise(L.load().mfla_add_mixed(_handle(), NR, NS, _mat(_as_matrix(X)), _mat(_as_matrix(D))))
+    return X
+
+
 def normwise_backward_error(A, x, b, r, *, uplo="general"):
     """normwise_backward_error (src/residual.jl:215-269). Returns the limbs of
     the value (vector RHS) or an (N, nrhs) array (matrix RHS)."""

@@ -1,0 +1,156 @@
+"""FP64-factor + Float64xN iterative refinement, and its benchmark.
+
+MFLA provides the pieces -- residual!, refinement_correction! (exactly one
+ldiv!), normwise_backward_error -- and leaves the loop to the caller
+(src/residual.jl:294-300, docs/SDPX_PROVIDER_READINESS.md). This module is
+that loop on the device, with the factorization done in FP64:
+
+    F64 = factor(hi limb of A)                      # FP64 Cholesky / LU
+    X = 0, R = B
+    repeat:  D = F64 \\ hi(R)      (FP64 solve of the residual's leading limb)
+             X = X + D             (Float64xN accumulation, add_mixed_)
+             R = B - A*X           (Float64xN residual, residual_)
+             until normwise_backward_error <= tol or it stops improving
+
+Convergence needs kappa(A) * u_64 < 1 (kappa <= 1e8 here); each step gains
+~ -log10(kappa * u_64) digits until the Float64xN residual floor.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from . import api as M
+
+
+def _eye(N, n):
+    Q = torch.zeros((N, n, n), dtype=torch.float64, device="cuda")
+    Q[0].diagonal().fill_(1.0)
+    return Q
+
+
+def householder_orthogonal(N, n, reflectors, rng):
+    """Q = H_1 ... H_r, H = I - tau v v^T with v ~ N(0, 1) (FP64 values) and
+    tau = 2 / (v^T v) in Float64xN, formed with gemv!/gemm! in Float64xN."""
+    Q = _eye(N, n)
+    for _ in range(reflectors):
+        v = torch.zeros((N, 1, n), dtype=torch.float64, device="cuda")
+        v[0, 0] = torch.from_numpy(rng.standard_normal(n)).cuda()
+        vt = v.reshape(N, n, 1).contiguous()                       # 1 x n
+        vv = torch.zeros((N, 1, 1), dtype=torch.float64, device="cuda")
+        M.gemm_(vv, vt, v)
+        two = np.zeros((N, 1))
+        two[0, 0] = 2.0
+        tau = M.host_arith("div", two, vv.reshape(N, 1).cpu().numpy())[:, 0]
+        w = torch.zeros((N, 1, n), dtype=torch.float64, device="cuda")
+        M.gemm_(w, Q, v)
+        M.gemm_(Q, w, vt, -tau, 1.0)
+    return Q
+
+
+def _scale_columns(Q, sigma):
+    """Q * diag(sigma) elementwise in Float64xN (sigma: FP64 values)."""
+    N, n, _ = Q.shape
+    S = torch.zeros_like(Q)
+    S[0] = torch.from_numpy(np.ascontiguousarray(np.repeat(sigma[:, None], n, axis=1))).cuda()
+    return M.arith("mul", Q.reshape(N, -1), S.reshape(N, -1)).reshape(N, n, n)
+
+
+def _symmetrize_lower(A):
+    N, n, _ = A.shape
+    idx = torch.arange(n, device="cuda")
+    lower = (idx[None, :] >= idx[:, None])          # planar [col, row]: row >= col
+    return torch.where(lower[None], A, A.transpose(1, 2)).contiguous()
+
+
+def spd_kappa(n, kappa=1e8, reflectors=4, seed=20260812, N=4):
+    """A = Q diag(sigma) Q^T, sigma log-spaced over [1, kappa], exactly symmetric."""
+    rng = np.random.default_rng(seed)
+    Q = householder_orthogonal(N, n, reflectors, rng)
+    sigma = 10.0 ** (np.log10(kappa) * np.arange(n) / max(n - 1, 1))
+    W = _scale_columns(Q, sigma)
+    A = torch.zeros_like(Q)
+    M.gemm_(A, W, Q.transpose(1, 2).contiguous())
+    return _symmetrize_lower(A)
+
+
+def general_kappa(n, kappa=1e8, reflectors=4, seed=20260813, N=4):
+    """A = U diag(sigma) V^T with independent Householder U, V."""
+    rng = np.random.default_rng(seed)
+    U = householder_orthogonal(N, n, reflectors, rng)
+    V = householder_orthogonal(N, n, reflectors, rng)
+    sigma = 10.0 ** (np.log10(kappa) * np.arange(n) / max(n - 1, 1))
+    W = _scale_columns(U, sigma)
+    A = torch.zeros_like(U)
+    M.gemm_(A, W, V.transpose(1, 2).contiguous())
+    return A
+
+
+def _max_be(be):
+    return float(np.max(np.abs(be[0]))) if be.size else 0.0
+
+
+def refine_solve(A, B, kind="cholesky", *, block=64, tol=None, max_iter=30, uplo="general"):
+    """Solve A X = B (A, B Float64xN on the device) with an FP64 factor and
+    Float64xN refinement. Returns (X, history) where history holds the
+    backward error after each correction."""
+    N = A.shape[0]
+    tol = tol if tol is not None else 64.0 * 2.0 ** (-52 * N)
+    F64 = A[0:1].clone()
+    if kind == "cholesky":
+        F = M.cholesky_(F64, config=M.KernelConfig(cholesky_block=block))
+    else:
+        F = M.lu_(F64, config=M.KernelConfig(lu_block=block))
+    X = torch.zeros_like(B)
+    R = B.clone()
+    hist = []
+    for _ in range(max_iter):
+        D = R[0:1]                                     # leading limb of the residual, in place
+        M.ldiv_(D, F, config=M.KernelConfig(trsm_order="blocked"))
+        M.add_mixed_(X, D)
+        M.residual_(R, A, X, B, uplo=uplo)
+        e = _max_be(M.normwise_backward_error(A, X, B, R, uplo=uplo))
+        hist.append(e)
+        if e <= tol or (len(hist) > 1 and e > 0.5 * hist[-2]):
+            break
+    return X, hist
+
+
+def plain_solve(A, B, kind="cholesky", *, block=16):
+    """The plain Float64xN factor + solve (MFLA's own cholesky!/lu! + ldiv!)."""
+    F0 = A.clone()
+    if kind == "cholesky":
+        F = M.cholesky_(F0, config=M.KernelConfig(cholesky_block=block))
+    else:
+        F = M.lu_(F0, config=M.KernelConfig(lu_block=block))
+    return M.solve(F, B, config=M.KernelConfig(trsm_order="blocked"))
+
+
+def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812):
+    """Times the refined and the plain Float64xN solve on the same matrix and
+    right-hand sides; reports times and Float64xN backward errors."""
+    A = spd_kappa(n, kappa, N=N, seed=seed) if kind == "cholesky" else general_kappa(n, kappa, N=N, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    Bh = np.zeros((N, nrhs, n))
+    Bh[0] = rng.standard_normal((nrhs, n))
+    for l in range(1, N):
+        Bh[l] = Bh[l - 1] * 2.0**-53 * rng.uniform(-1, 1, (nrhs, n))
+    B = torch.from_numpy(Bh).cuda()
+    out = {"n": n, "nrhs": nrhs, "kind": kind, "kappa": kappa, "limbs": N}
+    for name, fn in (("refined", lambda: refine_solve(A, B, kind)), ("plain", lambda: (plain_solve(A, B, kind), None))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X, hist = fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        R = torch.zeros_like(B)
+        M.residual_(R, A, X, B)
+        be = _max_be(M.normwise_backward_error(A, X, B, R))
+        out[name] = {"seconds": round(dt, 4), "backward_error": be}
+        if hist is not None:
+            out[name]["iterations"] = len(hist)
+            out[name]["history"] = [float(f"{h:.3e}") for h in hist]
+    out["speedup"] = round(out["plain"]["seconds"] / out["refined"]["seconds"], 3)
+    return out
