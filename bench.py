@@ -109,9 +109,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--refine", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="MiB copied once by a plain device copy before the GEMM (scales HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it (see DESIGN.md on sharding)"}))
+        print(json.dumps({"error": "one GPU only: this library has no multi-GPU path (runs with --gpus > 1 are not supported)"}))
         sys.exit(2)
 
     import torch
@@ -121,6 +123,12 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU (the HIP library has no CPU fallback)")
     M.load()
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration * 2**20 // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     g = bench_gemm(args, M, torch)
     flops, instr = NETWORK_COST[args.limbs]
     out = {

@@ -141,6 +141,10 @@ int mfla_host_arith(int limbs, int op, int64_t count, const double *x, const dou
 int mfla_absmax(mfla_handle h, int limbs, int region, mfla_matrix A, double *out);
 int mfla_all_finite(mfla_handle h, int limbs, int region, mfla_matrix A, int *out);
 
+/* copyto!(dst, src) for limb-planar matrices as a device kernel (usable
+ * inside a captured HIP graph, unlike a memcpy node). */
+int mfla_copy(mfla_handle h, int limbs, mfla_matrix dst, mfla_matrix src);
+
 /* AoS (Julia Matrix{Float64xN}, limbs interleaved) <-> limb-planar copies */
 int mfla_from_interleaved(mfla_handle h, int limbs, mfla_matrix dst, const double *src_dev, int64_t src_ld);
 int mfla_to_interleaved(mfla_handle h, int limbs, double *dst_dev, int64_t dst_ld, mfla_matrix src);
@@ -213,6 +217,37 @@ int mfla_add_mixed(mfla_handle h, int dst_limbs, int src_limbs, mfla_matrix X, m
 /* normwise_backward_error (src/residual.jl:215-269): out is a host array of
  * limbs x nrhs binary64 (limb-planar), one value per right-hand side */
 int mfla_backward_error(mfla_handle h, int limbs, int uplo, mfla_matrix A, mfla_matrix X, mfla_matrix B, mfla_matrix R, double *out);
+
+/* ------------------------------------------------- batched small systems */
+/* MFLA has no batched API (callers loop cholesky!/ldlt! + ldiv! per system);
+ * this runs many uneven systems in one persistent launch. Each system is
+ * factored with MFLA's own algorithm (cholesky!: blocked, block
+ * config->cholesky_block; ldlt!: the unblocked Bunch-Kaufman route MFLA uses
+ * below its crossover) and solved in place on its right-hand sides. A system
+ * occupies pool storage at the entry's offsets: A (limbs planes of n x n,
+ * limb stride n*n), B (limbs planes of n x nrhs), and for LDLT dsub (limbs
+ * planes of n), pivots and blocks (n entries at meta_offset). info[s] gets the
+ * MFLA status (or -2 if n > mfla_batched_max_order() or nrhs >
+ * mfla_batched_max_rhs()); origmax[s*limbs..] the LDLT original_maximum.
+ * The call allocates nothing and never synchronises: it can be captured in a
+ * HIP graph. `counter` is one device unsigned int (reset by the call). */
+#define MFLA_BATCH_CHOLESKY 0
+#define MFLA_BATCH_LDLT 1
+typedef struct mfla_batch_entry {
+    int64_t n;
+    int32_t kind;        /* MFLA_BATCH_* */
+    int32_t nrhs;
+    int64_t a_offset;    /* elements into A_pool */
+    int64_t b_offset;    /* elements into B_pool */
+    int64_t meta_offset; /* elements into dsub/pivot/block pools */
+} mfla_batch_entry;
+
+int mfla_batched_solve(mfla_handle h, int limbs, const mfla_batch_entry *entries_dev, int64_t count, double *A_pool,
+                       double *B_pool, double *dsub_pool, int64_t *pivot_pool, uint8_t *block_pool, int64_t *info_dev,
+                       double *origmax_dev, unsigned int *counter_dev, int64_t workgroups, int exact_backward,
+                       const mfla_config *config);
+int mfla_batched_max_order(void);
+int mfla_batched_max_rhs(void);
 
 #ifdef __cplusplus
 }

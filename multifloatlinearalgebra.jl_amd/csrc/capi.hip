@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/mfla_hip.h"
+#include "k_batched.hpp"
 #include "k_blas2.hpp"
 #include "k_blas3.hpp"
 #include "k_factor.hpp"
@@ -382,6 +383,15 @@ int mfla_all_finite(mfla_handle h, int N, int region, mfla_matrix A, int *out) {
     const int64_t r = read_info(h);
     if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
     *out = r == 0 ? 1 : 0;
+    return MFLA_OK;
+}
+
+int mfla_copy(mfla_handle h, int N, mfla_matrix dst, mfla_matrix src) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (dst.rows != src.rows || dst.cols != src.cols) return fail(MFLA_ERR_DIMENSION, "copyto! dimensions differ");
+    if (check_desc(dst, "dst") || check_desc(src, "src")) return MFLA_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(mfk::copy(N, h->stream, dst.rows, dst.cols, dvn(dst), dvn(src)));
     return MFLA_OK;
 }
 
@@ -798,6 +808,30 @@ int mfla_add_mixed(mfla_handle h, int dst_limbs, int src_limbs, mfla_matrix X, m
     HIPCHK(mfk::add_mixed(src_limbs, dst_limbs, h->stream, X.rows, X.cols, dvn(X), dvn(D)));
     return MFLA_OK;
 }
+
+int mfla_batched_solve(mfla_handle h, int N, const mfla_batch_entry *entries, int64_t count, double *A_pool, double *B_pool,
+                       double *dsub_pool, int64_t *pivot_pool, uint8_t *block_pool, int64_t *info, double *origmax,
+                       unsigned int *counter, int64_t workgroups, int exact_backward, const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (count < 0) return fail(MFLA_ERR_ARGUMENT, "negative batch count");
+    if (count > 0 && (!entries || !A_pool || !B_pool || !info || !counter || !origmax))
+        return fail(MFLA_ERR_ARGUMENT, "null batch pool");
+    mfk::BatchArgs a{};
+    a.entries = entries; a.count = count; a.A = A_pool; a.B = B_pool; a.dsub = dsub_pool; a.pivots = pivot_pool;
+    a.blocks = block_pool; a.info = info; a.origmax = origmax; a.counter = counter;
+    ldlt_alpha(N, a.alpha);
+    a.chol_block = std::max<int64_t>(c.cholesky_block, 1);
+    a.exact_backward = exact_backward ? 1 : 0;
+    a.workgroups = workgroups > 0 ? workgroups : 1024;
+    std::lock_guard<std::mutex> g(h->lock);
+    HIPCHK(mfk::batched_solve(N, h->stream, a));
+    return MFLA_OK;
+}
+
+int mfla_batched_max_order(void) { return mfk::batched_max_order(); }
+int mfla_batched_max_rhs(void) { return mfk::batched_max_rhs(); }
 
 int mfla_backward_error(mfla_handle h, int N, int uplo, mfla_matrix A, mfla_matrix X, mfla_matrix B, mfla_matrix R, double *out) {
     if (check_limbs(N)) return MFLA_ERR_ARGUMENT;

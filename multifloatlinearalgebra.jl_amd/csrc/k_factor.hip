@@ -7,83 +7,9 @@
 // MFLA leaves it when the factorization stops.
 #include "mf_common.hpp"
 #include "k_factor.hpp"
-#include <climits>
+#include "k_factor_dev.hpp"
 
 namespace mfk {
-
-// ------------------------------------------------------------ utilities
-template <int N> __device__ __forceinline__ void lds_put(double *s, const mf::V<N> &v) {
-#pragma unroll
-    for (int l = 0; l < N; ++l) s[l] = v.x[l];
-}
-template <int N> __device__ __forceinline__ mf::V<N> lds_get(const double *s) {
-    mf::V<N> v;
-#pragma unroll
-    for (int l = 0; l < N; ++l) v.x[l] = s[l];
-    return v;
-}
-template <int N> __device__ __forceinline__ void swap_el(const DV &A, int64_t i1, int64_t j1, int64_t i2, int64_t j2) {
-    mf::V<N> t = ld<N>(A, i1, j1);
-    stv<N>(A, i1, j1, ld<N>(A, i2, j2));
-    stv<N>(A, i2, j2, t);
-}
-
-// Workgroup argmax of |value| with MFLA's first-strict-maximum rule: the
-// largest value wins, ties go to the smallest index (lu.jl:93-100,
-// ldlt.jl:192-200). Every thread passes its best (value, index); idx < 0 means
-// "no candidate". Result broadcast through LDS.
-template <int N>
-__device__ void wg_argmax(mf::V<N> &val, int64_t &idx, double *sv_buf, int64_t *si_buf) {
-    const int t = threadIdx.x, T = blockDim.x;
-    lds_put<N>(&sv_buf[t * N], val);
-    si_buf[t] = idx;
-    __syncthreads();
-    for (int s = T / 2; s > 0; s >>= 1) {
-        if (t < s) {
-            mf::V<N> a = lds_get<N>(&sv_buf[t * N]), b = lds_get<N>(&sv_buf[(t + s) * N]);
-            int64_t ia = si_buf[t], ib = si_buf[t + s];
-            bool take_b;
-            if (ib < 0) take_b = false;
-            else if (ia < 0) take_b = true;
-            else if (mf::gt(b, a)) take_b = true;
-            else if (mf::eq(a, b)) take_b = ib < ia;
-            else take_b = false;
-            if (take_b) { lds_put<N>(&sv_buf[t * N], b); si_buf[t] = ib; }
-        }
-        __syncthreads();
-    }
-    val = lds_get<N>(&sv_buf[0]);
-    idx = si_buf[0];
-    __syncthreads();
-}
-
-// Workgroup max (Julia max semantics, value only).
-template <int N>
-__device__ mf::V<N> wg_max(mf::V<N> val, double *sv_buf) {
-    const int t = threadIdx.x, T = blockDim.x;
-    lds_put<N>(&sv_buf[t * N], val);
-    __syncthreads();
-    for (int s = T / 2; s > 0; s >>= 1) {
-        if (t < s) lds_put<N>(&sv_buf[t * N], mf::vmax(lds_get<N>(&sv_buf[t * N]), lds_get<N>(&sv_buf[(t + s) * N])));
-        __syncthreads();
-    }
-    mf::V<N> r = lds_get<N>(&sv_buf[0]);
-    __syncthreads();
-    return r;
-}
-
-__device__ int64_t wg_min_i64(int64_t v, int64_t *si_buf) {
-    const int t = threadIdx.x, T = blockDim.x;
-    si_buf[t] = v;
-    __syncthreads();
-    for (int s = T / 2; s > 0; s >>= 1) {
-        if (t < s && si_buf[t + s] < si_buf[t]) si_buf[t] = si_buf[t + s];
-        __syncthreads();
-    }
-    int64_t r = si_buf[0];
-    __syncthreads();
-    return r;
-}
 
 // -------------------------------------------------------------- Cholesky
 // src/factorizations/cholesky.jl:10-34, one workgroup per panel.
@@ -165,150 +91,6 @@ __global__ void k_apply_ipiv(DV B, int64_t count, int64_t nrhs, const int64_t *i
     for (int64_t k = 0; k < count; ++k) {
         const int64_t p = ipiv[k] - 1;
         if (p != k) swap_el<N>(B, k, c, p, c);
-    }
-}
-
-// ------------------------------------------------------------------ LDLT
-// ldlt.jl:65-101 (scaled 2x2 elimination)
-template <int N>
-__device__ bool solve2x2(const mf::V<N> &d11, const mf::V<N> &d21, const mf::V<N> &d22, const mf::V<N> &f, const mf::V<N> &s,
-                         mf::V<N> &x1, mf::V<N> &x2) {
-    mf::V<N> scale = mf::vmax(mf::vmax(mf::vabs(d11), mf::vabs(d21)), mf::vabs(d22));
-    x1 = mf::zero<N>(); x2 = mf::zero<N>();
-    if (mf::is_zero(scale) || !mf::is_finite(scale)) return false;
-    mf::V<N> a = mf::div(d11, scale), b = mf::div(d21, scale), c = mf::div(d22, scale);
-    mf::V<N> r1 = mf::div(f, scale), r2 = mf::div(s, scale);
-    if (mf::ge(mf::vabs(a), mf::vabs(b))) {
-        if (mf::is_zero(a)) return false;
-        mf::V<N> t = mf::div(b, a);
-        mf::V<N> u = mf::msub(c, t, b);
-        if (mf::is_zero(u) || !mf::is_finite(u)) return false;
-        x2 = mf::div(mf::msub(r2, t, r1), u);
-        x1 = mf::div(mf::msub(r1, b, x2), a);
-        return true;
-    }
-    if (mf::is_zero(b)) return false;
-    mf::V<N> t = mf::div(a, b);
-    mf::V<N> u = mf::msub(b, t, c);
-    if (mf::is_zero(u) || !mf::is_finite(u)) return false;
-    x2 = mf::div(mf::msub(r1, t, r2), u);
-    x1 = mf::div(mf::msub(r2, c, x2), b);
-    return true;
-}
-
-// ldlt.jl:155-181, executed cooperatively by one workgroup.
-template <int N>
-__device__ void wg_sym_swap(const DV &A, int64_t n, int64_t f, int64_t s, int64_t active) {
-    if (f == s) return;
-    for (int64_t c = threadIdx.x; c < n; c += blockDim.x) swap_el<N>(A, f, c, s, c);
-    __syncthreads();
-    for (int64_t r = active + threadIdx.x; r < n; r += blockDim.x) swap_el<N>(A, r, f, r, s);
-    __syncthreads();
-}
-
-// Shared pieces of the Bunch-Kaufman pivot choice (ldlt.jl:183-219 and the
-// lazy variant :451-508). Entry(i1, i2) evaluates the current Schur entry.
-struct BKChoice { int bs; int64_t pivot; };
-
-template <int N, typename Entry>
-__device__ BKChoice wg_select_bk(int64_t n, int64_t k, const mf::V<N> &alpha, Entry entry, double *sv_buf, int64_t *si_buf) {
-    const mf::V<N> absakk = mf::vabs(entry(k, k));
-    if (k == n - 1) return BKChoice{mf::is_zero(absakk) ? 0 : 1, k};
-    mf::V<N> best = mf::zero<N>();
-    int64_t bi = -1;
-    for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) {
-        mf::V<N> c = mf::vabs(entry(r, k));
-        if (bi < 0 || mf::gt(c, best)) { best = c; bi = r; }
-    }
-    wg_argmax<N>(best, bi, sv_buf, si_buf);
-    const mf::V<N> colmax = best;
-    const int64_t imax = bi;
-    if (mf::eq(mf::vmax(absakk, colmax), mf::zero<N>())) return BKChoice{0, k};
-    if (mf::ge(absakk, mf::mul(alpha, colmax))) return BKChoice{1, k};
-    mf::V<N> rm = mf::zero<N>();
-    for (int64_t c = k + threadIdx.x; c < imax; c += blockDim.x) rm = mf::vmax(rm, mf::vabs(entry(imax, c)));
-    for (int64_t r = imax + 1 + threadIdx.x; r < n; r += blockDim.x) rm = mf::vmax(rm, mf::vabs(entry(r, imax)));
-    const mf::V<N> rowmax = wg_max<N>(rm, sv_buf);
-    if (mf::ge(absakk, mf::mul(mf::mul(alpha, colmax), mf::div(colmax, rowmax)))) return BKChoice{1, k};
-    if (mf::ge(mf::vabs(entry(imax, imax)), mf::mul(alpha, rowmax))) return BKChoice{1, imax};
-    return BKChoice{2, imax};
-}
-
-// Unblocked right-looking Bunch-Kaufman LDLT (ldlt.jl:352-412), one
-// workgroup per matrix. Matrices are addressed through a batch stride so the
-// same body serves single and batched factorizations.
-template <int N>
-__device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks, const mf::V<N> &alpha,
-                                    int64_t *info_out, double *sv_buf, int64_t *si_buf) {
-    auto entry = [&](int64_t i, int64_t j) { return ld<N>(A, i, j); };
-    int64_t k = 0;
-    while (k < n) {
-        BKChoice ch = wg_select_bk<N>(n, k, alpha, entry, sv_buf, si_buf);
-        if (ch.bs == 0) { if (threadIdx.x == 0) *info_out = k + 1; return; }
-        if (ch.bs == 1) {
-            if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 1; }
-            wg_sym_swap<N>(A, n, k, ch.pivot, k);
-            const mf::V<N> d = ld<N>(A, k, k);
-            if (mf::is_zero(d)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
-            for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(ld<N>(A, r, k), d));
-            __syncthreads();
-            // rank-1 update (ldlt.jl:221-262), both triangles written
-            const int64_t w = n - k - 1;
-            for (int64_t e = threadIdx.x; e < w * w; e += blockDim.x) {
-                const int64_t col = k + 1 + e / w, row = k + 1 + e % w;
-                if (row < col) continue;
-                const mf::V<N> coef = mf::mul(d, ld<N>(A, col, k));
-                const mf::V<N> v = mf::msub(ld<N>(A, row, col), ld<N>(A, row, k), coef);
-                stv<N>(A, row, col, v);
-                stv<N>(A, col, row, v);
-            }
-            __syncthreads();
-            k += 1;
-        } else {
-            if (!(k < n - 1)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
-            if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 2; blocks[k + 1] = 0; }
-            wg_sym_swap<N>(A, n, k + 1, ch.pivot, k);
-            const mf::V<N> d11 = ld<N>(A, k, k), d21 = ld<N>(A, k + 1, k), d22 = ld<N>(A, k + 1, k + 1);
-            if (threadIdx.x == 0) stv<N>(dsub, k, 0, d21);
-            mf::V<N> s1, s2;
-            if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
-            // rows k+2..n-1: find the first failing row, write only rows before it
-            int64_t myfail = INT64_MAX;
-            for (int64_t r = k + 2 + threadIdx.x; r < n; r += blockDim.x) {
-                bool ok = solve2x2<N>(d11, d21, d22, ld<N>(A, r, k), ld<N>(A, r, k + 1), s1, s2);
-                if (!(ok && mf::is_finite(s1) && mf::is_finite(s2)) && r < myfail) myfail = r;
-            }
-            const int64_t firstfail = wg_min_i64(myfail, si_buf);
-            for (int64_t r = k + 2 + threadIdx.x; r < n && r < firstfail; r += blockDim.x) {
-                solve2x2<N>(d11, d21, d22, ld<N>(A, r, k), ld<N>(A, r, k + 1), s1, s2);
-                stv<N>(A, r, k, s1);
-                stv<N>(A, r, k + 1, s2);
-            }
-            __syncthreads();
-            if (firstfail != INT64_MAX) { if (threadIdx.x == 0) *info_out = k + 1; return; }
-            // rank-2 update (ldlt.jl:264-319): vector form on four-row blocks,
-            // scalar form on the remainder rows of each column
-            if (k + 1 < n - 1) {
-                const int64_t w = n - k - 2;
-                for (int64_t e = threadIdx.x; e < w * w; e += blockDim.x) {
-                    const int64_t col = k + 2 + e / w, row = k + 2 + e % w;
-                    if (row < col) continue;
-                    const mf::V<N> l1 = ld<N>(A, col, k), l2 = ld<N>(A, col, k + 1);
-                    const mf::V<N> c1 = mf::add(mf::mul(d11, l1), mf::mul(d21, l2));
-                    const mf::V<N> c2 = mf::add(mf::mul(d21, l1), mf::mul(d22, l2));
-                    const int64_t vec_end = col + ((n - col) / 4) * 4;
-                    mf::V<N> v;
-                    if (row < vec_end) v = mf::sub(ld<N>(A, row, col), mf::add(mf::mul(ld<N>(A, row, k), c1), mf::mul(ld<N>(A, row, k + 1), c2)));
-                    else v = mf::msub(mf::msub(ld<N>(A, row, col), ld<N>(A, row, k), c1), ld<N>(A, row, k + 1), c2);
-                    stv<N>(A, row, col, v);
-                    stv<N>(A, col, row, v);
-                }
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) { stv<N>(A, k + 1, k, mf::zero<N>()); stv<N>(A, k, k + 1, mf::zero<N>()); }
-            __syncthreads();
-            k += 2;
-        }
     }
 }
 

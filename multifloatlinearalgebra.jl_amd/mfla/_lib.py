@@ -72,6 +72,7 @@ PROTOTYPES = {
     "mfla_host_arith": (_I, [_I, _I, _I64, _DP, _DP, _DP, _DP]),
     "mfla_absmax": (_I, [_P, _I, _I, _M, _DP]),
     "mfla_all_finite": (_I, [_P, _I, _I, _M, ctypes.POINTER(_I)]),
+    "mfla_copy": (_I, [_P, _I, _M, _M]),
     "mfla_from_interleaved": (_I, [_P, _I, _M, _P, _I64]),
     "mfla_to_interleaved": (_I, [_P, _I, _P, _I64, _M]),
     "mfla_gemm": (_I, [_P, _I, _M, _M, _M, _DP, _DP, _CP]),
@@ -92,6 +93,9 @@ PROTOTYPES = {
     "mfla_residual_mixed": (_I, [_P, _I, _I, _I, _M, _M, _M, _M, _CP]),
     "mfla_add_mixed": (_I, [_P, _I, _I, _M, _M]),
     "mfla_backward_error": (_I, [_P, _I, _I, _M, _M, _M, _M, _DP]),
+    "mfla_batched_solve": (_I, [_P, _I, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I, _CP]),
+    "mfla_batched_max_order": (_I, []),
+    "mfla_batched_max_rhs": (_I, []),
 }
 
 _lib = None

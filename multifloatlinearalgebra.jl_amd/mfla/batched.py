@@ -1,0 +1,188 @@
+"""Batched factor + solve of many small uneven systems (one persistent launch).
+
+MFLA has no batched API: a caller loops ``cholesky!``/``ldlt!`` and
+``ldiv!`` over its systems. ``Batch`` packs such a loop's systems into device
+pools and runs them all with ``mfla_batched_solve``: every system gets MFLA's
+own factorization (bitwise equal to the single-system call) and an in-place
+solve of its right-hand sides. The launch allocates nothing and never
+synchronises, so ``Batch.graph()`` can capture it (restore + solve) in a HIP
+graph and replay it.
+
+The autotuner picks the launch shape -- persistent workgroup count and queue
+order -- by timing candidates on the batch itself; neither changes any result
+(each system is computed by one workgroup in a fixed order).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .api import KernelConfig, _cfg, _handle, _mat, _raise
+
+CHOLESKY, LDLT = 0, 1
+ENTRY_DTYPE = np.dtype([("n", "<i8"), ("kind", "<i4"), ("nrhs", "<i4"), ("a_offset", "<i8"), ("b_offset", "<i8"),
+                        ("meta_offset", "<i8")])
+_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "mfla_batched_tuning.json")
+
+
+class Batch:
+    """Systems given as (kind, A, B) with planar numpy arrays A (N, n, n) and
+    B (N, nrhs, n); kind is CHOLESKY (lower triangle read) or LDLT."""
+
+    def __init__(self, systems, limbs, order=None):
+        self.N = limbs
+        self.systems = systems
+        n_list = [A.shape[1] for _, A, _ in systems]
+        self.count = len(systems)
+        ent = np.zeros(self.count, dtype=ENTRY_DTYPE)
+        a_off = b_off = m_off = 0
+        a_parts, b_parts = [], []
+        for i, (kind, A, B) in enumerate(systems):
+            n, nrhs = A.shape[1], B.shape[1]
+            ent[i] = (n, kind, nrhs, a_off, b_off, m_off)
+            a_parts.append(np.ascontiguousarray(A).reshape(-1))
+            b_parts.append(np.ascontiguousarray(B).reshape(-1))
+            a_off += self.N * n * n
+            b_off += self.N * n * nrhs
+            m_off += self.N * n
+        self.entries_host = ent
+        dev = "cuda"
+        self.A0 = torch.from_numpy(np.concatenate(a_parts) if a_parts else np.zeros(1)).to(dev)
+        self.B0 = torch.from_numpy(np.concatenate(b_parts) if b_parts else np.zeros(1)).to(dev)
+        self.A = self.A0.clone()
+        self.B = self.B0.clone()
+        self.dsub = torch.zeros(max(m_off, 1), dtype=torch.float64, device=dev)
+        self.piv = torch.zeros(max(m_off, 1), dtype=torch.int64, device=dev)
+        self.blk = torch.zeros(max(m_off, 1), dtype=torch.uint8, device=dev)
+        self.info = torch.zeros(max(self.count, 1), dtype=torch.int64, device=dev)
+        self.origmax = torch.zeros(max(self.count, 1) * self.N, dtype=torch.float64, device=dev)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.set_order(order)
+        self.max_n = max(n_list) if n_list else 0
+
+    def set_order(self, order=None):
+        """Queue order: None keeps the given order, "desc" runs the largest
+        systems first (longest-processing-time first)."""
+        ent = self.entries_host
+        if order == "desc":
+            ent = ent[np.argsort(-ent["n"], kind="stable")]
+        self.order = order
+        self._perm = ent
+        self.entries = torch.from_numpy(ent.view(np.uint8).copy()).cuda()
+        # info/origmax are indexed by queue position: keep the permutation
+        self._qpos = {int(o): i for i, o in enumerate(ent["a_offset"])}
+
+    def _flat(self, t, rows, cols):
+        return L.MFMatrix(t.data_ptr(), rows, cols, max(rows, 1), max(rows * cols, 1))
+
+    def restore(self):
+        """Copies the pristine inputs back with device kernels (graph-safe)."""
+        lib = L.load()
+        h = _handle()
+        for src, dst in ((self.A0, self.A), (self.B0, self.B)):
+            nel = src.numel()
+            _raise(lib.mfla_copy(h, 1, L.MFMatrix(dst.data_ptr(), nel, 1, nel, nel), L.MFMatrix(src.data_ptr(), nel, 1, nel, nel)))
+
+    def solve(self, workgroups=0, exact_backward=False, config=None):
+        if self.max_n > L.load().mfla_batched_max_order():
+            raise ValueError(f"batched systems need n <= {L.load().mfla_batched_max_order()}")
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        _raise(L.load().mfla_batched_solve(_handle(), self.N, p(self.entries), self.count, p(self.A), p(self.B), p(self.dsub), p(self.piv),
+                                           p(self.blk), p(self.info), p(self.origmax), p(self.counter), int(workgroups),
+                                           1 if exact_backward else 0, _cfg(config)))
+
+    def graph(self, workgroups=0, exact_backward=False, config=None):
+        """Captures restore + solve in a HIP graph; returns the graph (replay())."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.restore()
+            self.solve(workgroups, exact_backward, config)        # warm up outside capture
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.restore()
+            self.solve(workgroups, exact_backward, config)
+        torch.cuda.synchronize()
+        return g
+
+    # --------------------------------------------------------- results
+    def results(self):
+        """Per system: dict(info, A (factor), X, pivots, blocks, dsub, original_maximum)."""
+        torch.cuda.synchronize()
+        A = self.A.cpu().numpy()
+        B = self.B.cpu().numpy()
+        info = self.info.cpu().numpy()
+        om = self.origmax.cpu().numpy()
+        dsub = self.dsub.cpu().numpy()
+        piv = self.piv.cpu().numpy()
+        blk = self.blk.cpu().numpy()
+        out = []
+        for i, e in enumerate(self.entries_host):
+            n, nrhs, ao, bo, mo = int(e["n"]), int(e["nrhs"]), int(e["a_offset"]), int(e["b_offset"]), int(e["meta_offset"])
+            q = self._qpos[ao]
+            r = {"info": int(info[q]), "A": A[ao: ao + self.N * n * n].reshape(self.N, n, n),
+                 "X": B[bo: bo + self.N * n * nrhs].reshape(self.N, nrhs, n), "original_maximum": om[q * self.N: (q + 1) * self.N]}
+            if e["kind"] == LDLT:
+                r["pivots"] = piv[mo: mo + n].copy()
+                r["blocks"] = blk[mo: mo + n].copy()
+                r["dsub"] = dsub[mo: mo + self.N * n].reshape(self.N, n)
+            out.append(r)
+        return out
+
+
+def _time(fn, reps=3):
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def autotune(batch: Batch, workgroups=(256, 512, 1024, 2048), orders=(None, "desc"), use_cache=True):
+    """Times every (workgroups, order) candidate on the batch (restore + solve)
+    and keeps the fastest; the choice is cached per (limbs, count, size sum)."""
+    key = f"x{batch.N}:{batch.count}:{int(batch.entries_host['n'].sum())}"
+    if use_cache and os.path.exists(_CACHE):
+        try:
+            with open(_CACHE) as f:
+                c = json.load(f)
+            if key in c:
+                wg, order = c[key]["workgroups"], c[key]["order"]
+                batch.set_order(order)
+                return {"workgroups": wg, "order": order, "cached": True}
+        except (OSError, ValueError):
+            pass
+    best = None
+    trials = []
+    for order in orders:
+        batch.set_order(order)
+        for wg in workgroups:
+            ms = _time(lambda: (batch.restore(), batch.solve(wg)))
+            trials.append({"workgroups": wg, "order": order, "ms": round(ms, 3)})
+            if best is None or ms < best[0]:
+                best = (ms, wg, order)
+    batch.set_order(best[2])
+    res = {"workgroups": best[1], "order": best[2], "ms": round(best[0], 3), "trials": trials}
+    try:
+        os.makedirs(os.path.dirname(_CACHE), exist_ok=True)
+        c = {}
+        if os.path.exists(_CACHE):
+            with open(_CACHE) as f:
+                c = json.load(f)
+        c[key] = {"workgroups": best[1], "order": best[2]}
+        with open(_CACHE, "w") as f:
+            json.dump(c, f)
+    except (OSError, ValueError):
+        pass
+    return res

@@ -92,12 +92,36 @@ def _max_be(be):
     return float(np.max(np.abs(be[0]))) if be.size else 0.0
 
 
-def refine_solve(A, B, kind="cholesky", *, block=64, tol=None, max_iter=30, uplo="general"):
+class _Timer:
+    """Accumulates device time per stage with CUDA events (optional)."""
+
+    def __init__(self, on):
+        self.on, self.t, self.ev = on, {}, []
+
+    def mark(self, name):
+        if self.on:
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            self.ev.append((name, e))
+
+    def done(self):
+        if not self.on:
+            return None
+        torch.cuda.synchronize()
+        for (_, a), (name, b) in zip(self.ev, self.ev[1:]):
+            self.t[name] = self.t.get(name, 0.0) + a.elapsed_time(b)
+        return {k: round(v, 3) for k, v in self.t.items()}
+
+
+def refine_solve(A, B, kind="cholesky", *, block=64, tol=None, max_iter=30, uplo="general", stages=False):
     """Solve A X = B (A, B Float64xN on the device) with an FP64 factor and
     Float64xN refinement. Returns (X, history) where history holds the
-    backward error after each correction."""
+    backward error after each correction; with stages=True also a dict of
+    milliseconds per stage (factor, solve, update, residual, check)."""
     N = A.shape[0]
     tol = tol if tol is not None else 64.0 * 2.0 ** (-52 * N)
+    T = _Timer(stages)
+    T.mark("start")
     F64 = A[0:1].clone()
     if kind == "cholesky":
         F = M.cholesky_(F64, config=M.KernelConfig(cholesky_block=block))
@@ -105,16 +129,23 @@ def refine_solve(A, B, kind="cholesky", *, block=64, tol=None, max_iter=30, uplo
         F = M.lu_(F64, config=M.KernelConfig(lu_block=block))
     X = torch.zeros_like(B)
     R = B.clone()
+    T.mark("factor")
     hist = []
     for _ in range(max_iter):
         D = R[0:1]                                     # leading limb of the residual, in place
         M.ldiv_(D, F, config=M.KernelConfig(trsm_order="blocked"))
+        T.mark("solve")
         M.add_mixed_(X, D)
+        T.mark("update")
         M.residual_(R, A, X, B, uplo=uplo)
+        T.mark("residual")
         e = _max_be(M.normwise_backward_error(A, X, B, R, uplo=uplo))
+        T.mark("check")
         hist.append(e)
         if e <= tol or (len(hist) > 1 and e > 0.5 * hist[-2]):
             break
+    if stages:
+        return X, hist, T.done()
     return X, hist
 
 
@@ -153,4 +184,14 @@ def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812
             out[name]["iterations"] = len(hist)
             out[name]["history"] = [float(f"{h:.3e}") for h in hist]
     out["speedup"] = round(out["plain"]["seconds"] / out["refined"]["seconds"], 3)
+    _, _, st = refine_solve(A, B, kind, stages=True)
+    out["refined"]["stage_ms"] = st
+    P = _Timer(True)
+    P.mark("start")
+    F0 = A.clone()
+    F = M.cholesky_(F0) if kind == "cholesky" else M.lu_(F0)
+    P.mark("factor")
+    M.solve(F, B, config=M.KernelConfig(trsm_order="blocked"))
+    P.mark("solve")
+    out["plain"]["stage_ms"] = P.done()
     return out

@@ -1,0 +1,116 @@
+"""Batched persistent factor + solve: every system bitwise equal to the oracle
+(factor, pivots, blocks, D, status), solutions bitwise in MFLA's exact order,
+systolic-order solutions within c*n*u of the exact residual, and a captured
+HIP graph replayed several times (with eager work in between) reproducing the
+eager result."""
+import numpy as np
+import pytest
+import torch
+
+from mfutil import assert_bitwise, eps, rand_mf
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(oracle, rng, N, count, nmin, nmax, nrhs=4):
+    systems = []
+    for i in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        if i % 2 == 0:
+            R = rand_mf(rng, N, n, n)
+            S = np.zeros((N, n, n))
+            oracle.gemm(S, R, np.ascontiguousarray(np.transpose(R, (0, 2, 1))), fused=False)
+            S[0, np.arange(n), np.arange(n)] += n
+            systems.append((0, S, rand_mf(rng, N, n, nrhs)))
+        else:
+            m = max(2, (2 * n) // 3)
+            p = n - m
+            H = rand_mf(rng, N, m, m)
+            K = np.zeros((N, n, n))
+            K[:, :m, :m] = H + np.transpose(H, (0, 2, 1))
+            K[0, np.arange(m), np.arange(m)] += 2.0 * m
+            A = rand_mf(rng, N, p, m)
+            K[:, :m, m:] = A
+            K[:, m:, :m] = np.transpose(A, (0, 2, 1))
+            K[0, np.arange(m, n), np.arange(m, n)] = -1e-8
+            systems.append((1, K, rand_mf(rng, N, n, nrhs)))
+    return systems
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_batched_bitwise_and_exact(oracle, N):
+    from mfla.batched import Batch
+
+    rng = np.random.default_rng(400 + N)
+    systems = make_batch(oracle, rng, N, 24, 3, 90)
+    b = Batch(systems, N)
+    b.solve(exact_backward=True)
+    res = b.results()
+    for (kind, A, B), r in zip(systems, res):
+        n = A.shape[1]
+        if kind == 0:
+            F = A.copy()
+            info = oracle.potrf(F, 16)
+            assert r["info"] == info == 0
+            assert_bitwise(r["A"], F, f"batched cholesky n={n}")
+            assert_bitwise(r["X"], oracle.potrs(F, B.copy()), f"batched potrs n={n}")
+        else:
+            F = A.copy()
+            info, dsub, piv, blk, om = oracle.sytrf(F, "unblocked")
+            assert r["info"] == info == 0
+            assert np.array_equal(r["pivots"], piv) and np.array_equal(r["blocks"], blk), n
+            assert_bitwise(r["dsub"], dsub, "dsub")
+            assert_bitwise(r["A"], F, f"batched ldlt n={n}")
+            assert_bitwise(r["original_maximum"], om, "original maximum")
+            assert_bitwise(r["X"], oracle.sytrs(F, dsub, piv, blk, B.copy()), f"batched sytrs n={n}")
+
+
+def test_batched_systolic_order_and_graph(oracle):
+    from mfla.batched import Batch, autotune
+
+    N = 3
+    rng = np.random.default_rng(77)
+    systems = make_batch(oracle, rng, N, 40, 16, 128)
+    b = Batch(systems, N)
+    tune = autotune(b, workgroups=(64, 256), use_cache=False)
+    assert tune["workgroups"] in (64, 256)
+    b.restore()
+    b.solve(tune["workgroups"])
+    eager = b.results()
+    for (kind, A, B), r in zip(systems, eager):
+        n, nrhs = A.shape[1], B.shape[1]
+        assert r["info"] == 0
+        Af = A.copy()
+        if kind == 0:                       # symmetric from the lower triangle
+            il = np.triu_indices(n, 1)
+            Af[:, il[1], il[0]] = Af[:, il[0], il[1]]
+        rows = np.repeat(np.arange(n), nrhs)
+        cols = np.tile(np.arange(nrhs), n)
+        val, scale = oracle.exact_axpb(Af, r["X"], B, sab=-1, sc=1, rows=rows, cols=cols)
+        assert np.max(np.abs(val) / scale) <= 4096 * n * eps(N), n
+    g = b.graph(tune["workgroups"])
+    for rep in range(3):
+        junk = torch.randn(1000, device="cuda") * 2          # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        again = b.results()
+        for r0, r1 in zip(eager, again):
+            assert r0["info"] == r1["info"]
+            assert_bitwise(r1["X"], r0["X"], f"graph replay {rep}")
+            assert_bitwise(r1["A"], r0["A"], f"graph replay {rep}")
+        del junk
+
+
+def test_batched_rejects_oversize_and_reports_failures(oracle):
+    from mfla.batched import Batch
+
+    N = 2
+    bad = np.zeros((N, 3, 3))
+    bad[0] = np.array([[1.0, 2, 0], [2, 1, 0], [0, 0, 1]])      # not SPD: fails at 2
+    sing = np.ones((N, 2, 2))
+    sing[1:] = 0.0
+    b = Batch([(0, bad, np.ones((N, 1, 3))), (1, sing, np.ones((N, 1, 2)))], N)
+    b.solve()
+    r = b.results()
+    assert r[0]["info"] == 2 and r[1]["info"] == 2
