@@ -8,11 +8,11 @@ One step = one gemm! call = one kernel launch. The roofline entry prices the
 kernel against MI355X's FP64 peak (78.6 TFLOP/s); `issue_frac` gives FP64
 instruction issue against 39.3 T lane-instructions/s.
 
-`--refine` adds the solve comparison: FP64 Cholesky + Float64x4 iterative
+The run also reports the solve comparison (skip with --no-refine): FP64 Cholesky + Float64x4 iterative
 refinement against the plain Float64x4 Cholesky solve on the same SPD matrix
 (A = Q diag(sigma) Q^T, kappa = 1e8), reporting both times and backward errors.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--refine]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--no-refine] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -32,6 +32,11 @@ NETWORK_COST = {1: (2, 2), 2: (30, 29), 3: (111, 108), 4: (239, 233)}
 NETWORK_COST_X3_FUSED = (102, 99)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP64_INSTR = 39.3e12
+# HBM bytes per launch of the Float64x4 n=4096 GEMM kernel: rocprofv3 FETCH_SIZE
+# and WRITE_SIZE of one dispatch, each scaled by the same counter's reading for
+# a plain 512 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
+MEASURED_TRAFFIC = {(4, 4096): {"bytes": 39.52e9, "fetch_kib": 19033320, "write_kib": 524288, "fetch_scale": 2.0,
+                                "write_scale": 1.0}}
 
 
 def rand_mf(rng, N, rows, cols):
@@ -108,7 +113,7 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--refine", action="store_true")
+    ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the GEMM (scales HBM counters)")
     args = ap.parse_args()
@@ -147,14 +152,16 @@ def main():
         "config": {"workload": f"Float64x{args.limbs} square gemm! n={args.n}, C = A*B (beta = 0)", "n": args.n,
                    "limbs": args.limbs, "eft_flops_per_mac": flops, "fp64_instr_per_mac": instr},
         "roofline": {"bound": "mfma", "achieved": round(g["tflops"], 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(g["tflops"] / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "frac": round(g["tflops"] / PEAK_FP64_TFLOPS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((args.limbs, args.n), {}).get("bytes"),
                      "compute_unit": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s)",
-                     "issue_frac": round(g["issue_frac"], 4)},
+                     "issue_frac": round(g["issue_frac"], 4),
+                     "compulsory_bytes": 8 * args.limbs * 3 * args.n * args.n},
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(g["A"], g["B"], args.limbs, args.n, args.cpu_cols, threads)
-    if args.refine:
+    if not args.no_refine:
         from mfla import refine as R
 
         out["refine"] = R.bench_refine(n=args.n)
