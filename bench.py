@@ -13,6 +13,7 @@ refinement against the plain Float64x4 Cholesky solve on the same SPD matrix
 (A = Q diag(sigma) Q^T, kappa = 1e8), reporting both times and backward errors.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--no-refine] [--no-cpu-baseline]
+       python bench.py --suite blas3,factor,blas2,refine,batched,autodiff   (one JSON line per case)
 """
 from __future__ import annotations
 
@@ -114,6 +115,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--suite", default="",
+                    help="comma list of suite groups (blas3,factor,blas2,refine,batched,autodiff): print one JSON line "
+                         "per measured case instead of the headline line")
+    ap.add_argument("--suite-out", default="", help="also append the suite's JSON lines to this file")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the GEMM (scales HBM counters)")
     args = ap.parse_args()
@@ -128,6 +133,11 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU (the HIP library has no CPU fallback)")
     M.load()
+    if args.suite:
+        from mfla import suite
+
+        suite.run(tuple(args.suite.split(",")), out=args.suite_out or None)
+        return
     if args.copy_calibration:
         src = torch.ones(args.copy_calibration * 2**20 // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)

@@ -249,6 +249,66 @@ int mfla_batched_solve(mfla_handle h, int limbs, const mfla_batch_entry *entries
 int mfla_batched_max_order(void);
 int mfla_batched_max_rhs(void);
 
+/* ------------------------------------------ FP64 factor + Float64xN refinement */
+/* MFLA supplies the refinement pieces -- residual! (src/residual.jl:55-125),
+ * refinement_correction! (one ldiv!, src/residual.jl:302-332) and
+ * normwise_backward_error (src/residual.jl:215-269) -- and leaves the loop to
+ * the caller. This runs that loop with the factorization done in binary64:
+ *   F = cholesky / lu (partial pivoting) of the leading limb of A;
+ *   X = 0, R = B; repeat: D = F \ hi(R); X += D; R -= A*D
+ * until the normwise backward error estimate max_j |R_j| / (|A| |X_j| + |B_j|)
+ * (infinity norms, binary64) is <= tol or stops halving. The product A*D is
+ * summed with as many limbs as the size of D relative to X requires to keep
+ * the residual accurate to the Float64xN unit roundoff. limbs must be 2..4.
+ * `work` is a caller-owned device buffer of at least
+ * mfla_refine_workspace_size() doubles. history (host, max_iter entries) gets
+ * the estimate after each correction; stage_ms (host, 5 entries, may be NULL)
+ * milliseconds spent in factor, solve, update, residual and check.
+ * *info is the binary64 factorization's status (0, -1 non-finite input, k > 0
+ * failing pivot); a failed factorization returns MFLA_ERR_FAILED_FACTOR. */
+#define MFLA_REFINE_CHOLESKY 0
+#define MFLA_REFINE_LU 1
+int mfla_refine_workspace_size(int limbs, int kind, int64_t n, int64_t nrhs, int64_t *doubles);
+int mfla_refine_solve(mfla_handle h, int limbs, int kind, int uplo, mfla_matrix A, mfla_matrix B, mfla_matrix X, double *work,
+                      int64_t work_len, double tol, int max_iter, int64_t *info, int *iterations, double *history,
+                      double *stage_ms);
+
+/* Binary64 building blocks of the refinement, exposed for testing and for
+ * callers that run their own loop (no counterpart in MFLA).
+ * mfla_dgemm: C = alpha*op(A)*op(B) + beta*C on binary64 matrices (the
+ * leading limb plane of a descriptor), op = transpose for MFLA_TRANS.
+ * tri_a / tri_b: MFLA_LOWER / MFLA_UPPER declare op(A) / op(B) triangular
+ * (the other triangle is not read), MFLA_GENERAL a dense operand.
+ * mfla_dpotrf_inverse: binary64 Cholesky of the lower triangle of A in place
+ * and the inverse of the factor into the lower triangle of Inv (its strict
+ * upper triangle is used as scratch). */
+int mfla_dgemm(mfla_handle h, int trans_a, int trans_b, int tri_a, int tri_b, mfla_matrix C, mfla_matrix A, mfla_matrix B,
+               double alpha, double beta);
+int mfla_dpotrf_inverse(mfla_handle h, mfla_matrix A, mfla_matrix Inv, int64_t *info);
+
+/* ----------------------------------------------------- reverse-mode rules */
+/* MFLA has no differentiation rules (its dependencies are LinearAlgebra and
+ * MultiFloats only, Project.toml:6-8). These are the pullbacks a
+ * ChainRulesCore rrule (or a torch.autograd.Function) calls; every one reuses
+ * the Cholesky factor F of S = L L^T produced by mfla_potrf (info must be
+ * 0) instead of refactoring. Gradients with respect to the symmetric S are
+ * returned symmetrized: sym(G) = (G + G^T) / 2, exactly symmetric.
+ * mfla_logdet: logdet(S) = 2 sum_i log L_ii into out (host, limbs values).
+ * mfla_solve_adjoint: X = S^{-1} B with cotangent Xbar gives
+ *   Bbar = S^{-1} Xbar (the solve of src/factorizations/solve.jl:23-80 on
+ *   Xbar) and Sbar = -sym(Bbar X^T). Bbar may be the same matrix as Xbar.
+ * mfla_logdet_adjoint: Sbar = ybar * S^{-1} (ybar: host limbs).
+ * mfla_cholesky_adjoint: L = chol(S) with cotangent Lbar (lower triangle
+ *   used) gives Sbar = sym(L^{-T} Phi(L^T Lbar) L^{-1}), Phi = lower triangle
+ *   with the diagonal halved; work is an n x n scratch matrix. */
+int mfla_logdet(mfla_handle h, int limbs, mfla_matrix F, int64_t info, double *out);
+int mfla_solve_adjoint(mfla_handle h, int limbs, mfla_matrix F, int64_t info, mfla_matrix X, mfla_matrix Xbar, mfla_matrix Bbar,
+                       mfla_matrix Sbar, const mfla_config *config);
+int mfla_logdet_adjoint(mfla_handle h, int limbs, mfla_matrix F, int64_t info, const double *ybar, mfla_matrix Sbar,
+                        const mfla_config *config);
+int mfla_cholesky_adjoint(mfla_handle h, int limbs, mfla_matrix F, int64_t info, mfla_matrix Lbar, mfla_matrix Sbar,
+                          mfla_matrix work, const mfla_config *config);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@
 // kernels, and reads back status words where MFLA returns them.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -12,10 +13,13 @@
 #include <vector>
 
 #include "../../include/mfla_hip.h"
+#include "k_adjoint.hpp"
 #include "k_batched.hpp"
 #include "k_blas2.hpp"
 #include "k_blas3.hpp"
+#include "k_f64.hpp"
 #include "k_factor.hpp"
+#include "k_refine.hpp"
 #include "k_trsm.hpp"
 #include "mf_arith.hpp"
 
@@ -881,6 +885,518 @@ int mfla_backward_error(mfla_handle h, int N, int uplo, mfla_matrix A, mfla_matr
         }
         for (int l = 0; l < N; ++l) out[l * nrhs + j] = o[l];
     }
+    return MFLA_OK;
+}
+
+}  // extern "C"
+
+// ======================================================================
+// Binary64 factor + Float64xN iterative refinement
+namespace {
+
+DV dT(const DV &v) { return DV{v.p, v.cs, v.rs, v.ls}; }
+
+mfk::DGemmArgs dg(int64_t m, int64_t n, int64_t k, const DV &A, const DV &B, const DV &C, double alpha, double beta) {
+    mfk::DGemmArgs g{};
+    g.m = m; g.n = n; g.k = k; g.A = A; g.B = B; g.C = C;
+    g.alpha = alpha; g.beta = beta; g.beta_zero = beta == 0.0 ? 1 : 0;
+    g.ksplit = 1;
+    return g;
+}
+
+#define HIPRET(expr)                        \
+    do {                                    \
+        hipError_t e_ = (expr);             \
+        if (e_ != hipSuccess) return e_;    \
+    } while (0)
+
+constexpr int64_t kDiagBlock = 64;
+
+// Blocked right-looking binary64 Cholesky of F's lower triangle; the inverse
+// of every 64 x 64 diagonal factor block lands in the same block of Inv.
+hipError_t dpotrf_core(hipStream_t s, int64_t n, const DV &F, const DV &Inv, Status *st) {
+    for (int64_t j0 = 0; j0 < n; j0 += kDiagBlock) {
+        const int nbk = (int)std::min<int64_t>(kDiagBlock, n - j0);
+        HIPRET(mfk::dpotrf_diag(s, F, Inv, j0, nbk, st));
+        const int64_t rest = n - j0 - nbk;
+        if (rest == 0) break;
+        const DV A21 = sub(F, j0 + nbk, j0);
+        // L21 = A21 * inv(L11)^T in place: a 64-row output tile spans all nbk
+        // (<= 64) columns of its rows, reads them completely before it writes,
+        // and no other tile touches those rows.
+        mfk::DGemmArgs g = dg(rest, nbk, nbk, A21, dT(sub(Inv, j0, j0)), A21, 1.0, 0.0);
+        g.b_tri = mfk::TRI_UPPER;
+        HIPRET(mfk::dgemm(s, g, 1));
+        // A22 -= L21 * L21^T on the lower triangle
+        g = dg(rest, rest, nbk, A21, dT(A21), sub(F, j0 + nbk, j0 + nbk), -1.0, 1.0);
+        g.c_lower = 1;
+        HIPRET(mfk::dgemm(s, g, 1));
+    }
+    return hipSuccess;
+}
+
+// Recursive doubling from inverted diagonal blocks of size nb to the full
+// inverse of a triangular factor T:
+//   lower: inv([T11 0; T21 T22]) = [X11 0; -X22 T21 X11  X22]
+//   upper: inv([T11 T12; 0 T22]) = [X11  -X11 T12 X22; 0 X22]
+// All pairs of one level run as one batched GEMM; the intermediate product
+// is kept in the unused opposite triangle of Inv.
+hipError_t dtrtri_core(hipStream_t s, int64_t n, int64_t nb, const DV &T, const DV &Inv, bool lower) {
+    for (int64_t sz = nb; sz < n; sz *= 2) {
+        const int64_t full = n / (2 * sz);
+        auto level = [&](int64_t p, int64_t s2, int64_t batch) -> hipError_t {
+            const int64_t istep = 2 * sz * (Inv.rs + Inv.cs), tstep = 2 * sz * (T.rs + T.cs);
+            const DV X11 = sub(Inv, p, p), X22 = sub(Inv, p + sz, p + sz);
+            mfk::DGemmArgs g;
+            if (lower) {
+                const DV W = dT(sub(Inv, p, p + sz));                       // s2 x sz, strict upper storage
+                g = dg(s2, sz, sz, sub(T, p + sz, p), X11, W, 1.0, 0.0);     // W = T21 * X11
+                g.b_tri = mfk::TRI_LOWER; g.za = tstep; g.zb = istep; g.zc = istep;
+                HIPRET(mfk::dgemm(s, g, batch));
+                g = dg(s2, sz, s2, X22, W, sub(Inv, p + sz, p), -1.0, 0.0); // X21 = -X22 * W
+                g.a_tri = mfk::TRI_LOWER; g.za = istep; g.zb = istep; g.zc = istep;
+                return mfk::dgemm(s, g, batch);
+            }
+            const DV W = dT(sub(Inv, p + sz, p));                           // sz x s2, strict lower storage
+            g = dg(sz, s2, s2, sub(T, p, p + sz), X22, W, 1.0, 0.0);        // W = T12 * X22
+            g.b_tri = mfk::TRI_UPPER; g.za = tstep; g.zb = istep; g.zc = istep;
+            HIPRET(mfk::dgemm(s, g, batch));
+            g = dg(sz, s2, sz, X11, W, sub(Inv, p, p + sz), -1.0, 0.0);     // X12 = -X11 * W
+            g.a_tri = mfk::TRI_UPPER; g.za = istep; g.zb = istep; g.zc = istep;
+            return mfk::dgemm(s, g, batch);
+        };
+        if (full > 0) HIPRET(level(0, sz, full));
+        const int64_t p = full * 2 * sz;
+        if (p + sz < n) HIPRET(level(p, n - p - sz, 1));
+    }
+    return hipSuccess;
+}
+
+struct RefineWs {
+    double *F, *Inv, *InvU, *Y, *D, *R, *P, *rows, *stats;
+    int64_t *perm, *ipiv;
+    int64_t ksplit;
+};
+
+// Split-k factor for the n x nrhs correction products: enough workgroups to
+// cover the chip, at least 256 reduction steps per slice.
+int64_t refine_ksplit(int64_t n, int64_t nrhs) {
+    const int64_t tiles = mfk::cdiv(n, 64) * mfk::cdiv(nrhs, 64);
+    int64_t ks = 1;
+    while (tiles * ks * 2 <= 1024 && n / (ks * 2) >= 256) ks *= 2;
+    return ks;
+}
+
+int64_t refine_layout(int N, int kind, int64_t n, int64_t nrhs, double *base, RefineWs *out) {
+    const int64_t ks = refine_ksplit(n, nrhs);
+    int64_t off = 0;
+    auto take = [&](int64_t cnt) {
+        double *p = base ? base + off : nullptr;
+        off += (cnt + 31) / 32 * 32;   // 256-byte aligned regions
+        return p;
+    };
+    RefineWs w{};
+    w.F = take(n * n);
+    w.Inv = take(n * n);
+    w.InvU = kind == MFLA_REFINE_LU ? take(n * n) : nullptr;
+    w.Y = take(n * nrhs);
+    w.D = take(n * nrhs);
+    w.R = take((int64_t)N * n * nrhs);
+    w.P = ks > 1 ? take(ks * n * nrhs) : nullptr;
+    w.rows = take(n);
+    w.stats = take(4 * nrhs + 8);
+    w.perm = (int64_t *)take(n);
+    w.ipiv = (int64_t *)take(n);
+    w.ksplit = ks;
+    if (out) *out = w;
+    return off;
+}
+
+// Events between stages; the elapsed time up to each mark is charged to the
+// mark's stage.
+struct StageClock {
+    hipStream_t s;
+    bool on;
+    std::vector<std::pair<int, hipEvent_t>> marks;
+    void mark(int stage) {
+        if (!on) return;
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+        (void)hipEventRecord(e, s);
+        marks.emplace_back(stage, e);
+    }
+    ~StageClock() { finish(nullptr); }
+    void finish(double *out) {
+        if (out) for (int i = 0; i < 5; ++i) out[i] = 0.0;
+        if (!marks.empty()) (void)hipEventSynchronize(marks.back().second);
+        for (size_t i = 1; i < marks.size(); ++i) {
+            float ms = 0.f;
+            if (out && hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second) == hipSuccess) out[marks[i].first] += ms;
+        }
+        for (auto &m : marks) (void)hipEventDestroy(m.second);
+        marks.clear();
+    }
+};
+enum { ST_FACTOR = 0, ST_SOLVE = 1, ST_UPDATE = 2, ST_RESIDUAL = 3, ST_CHECK = 4 };
+
+}  // namespace
+
+extern "C" {
+
+int mfla_dgemm(mfla_handle h, int trans_a, int trans_b, int tri_a, int tri_b, mfla_matrix C, mfla_matrix A, mfla_matrix B,
+               double alpha, double beta) {
+    if (check_desc(C, "C") || check_desc(A, "A") || check_desc(B, "B")) return MFLA_ERR_ARGUMENT;
+    auto tri = [](int t, int &out) {
+        if (t == MFLA_GENERAL) out = mfk::TRI_NONE;
+        else if (t == MFLA_LOWER) out = mfk::TRI_LOWER;
+        else if (t == MFLA_UPPER) out = mfk::TRI_UPPER;
+        else return false;
+        return true;
+    };
+    int ta, tb;
+    if (!tri(tri_a, ta) || !tri(tri_b, tb)) return fail(MFLA_ERR_ARGUMENT, "triangle flags must be :general, :lower, or :upper");
+    if ((trans_a != MFLA_NOTRANS && trans_a != MFLA_TRANS) || (trans_b != MFLA_NOTRANS && trans_b != MFLA_TRANS))
+        return fail(MFLA_ERR_ARGUMENT, "trans must be 'N' or 'T'");
+    const int64_t m = trans_a == MFLA_TRANS ? A.cols : A.rows, k = trans_a == MFLA_TRANS ? A.rows : A.cols;
+    const int64_t kb = trans_b == MFLA_TRANS ? B.cols : B.rows, n = trans_b == MFLA_TRANS ? B.rows : B.cols;
+    if (k != kb || C.rows != m || C.cols != n) return fail(MFLA_ERR_DIMENSION, "matrix dimensions do not match");
+    if (extents_overlap(C, A, 1) || extents_overlap(C, B, 1)) return fail(MFLA_ERR_ALIAS, "output must not alias an input");
+    std::lock_guard<std::mutex> g(h->lock);
+    mfk::DGemmArgs a = dg(m, n, k, trans_a == MFLA_TRANS ? dvt(A) : dvn(A), trans_b == MFLA_TRANS ? dvt(B) : dvn(B), dvn(C), alpha, beta);
+    a.a_tri = ta; a.b_tri = tb;
+    if (k == 0) { a.k = 0; }
+    HIPCHK(mfk::dgemm(h->stream, a, 1));
+    return MFLA_OK;
+}
+
+int mfla_dpotrf_inverse(mfla_handle h, mfla_matrix A, mfla_matrix Inv, int64_t *info) {
+    if (check_desc(A, "A") || check_desc(Inv, "Inv")) return MFLA_ERR_ARGUMENT;
+    if (A.rows != A.cols || Inv.rows != A.rows || Inv.cols != A.cols) return fail(MFLA_ERR_DIMENSION, "square matrices of equal order required");
+    if (extents_overlap(A, Inv, 1)) return fail(MFLA_ERR_ALIAS, "the inverse must not alias the factor");
+    const int64_t n = A.rows;
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    HIPCHK(mfk::status_reset(s, h->status));
+    HIPCHK(mfk::finite_check(1, s, n, n, dvn(A), 1, h->status));
+    HIPCHK(dpotrf_core(s, n, dvn(A), dvn(Inv), h->status));
+    const int64_t r = read_info(h);
+    if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+    *info = r;
+    if (r == 0) {
+        HIPCHK(dtrtri_core(s, n, kDiagBlock, dvn(A), dvn(Inv), true));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    return MFLA_OK;
+}
+
+int mfla_refine_workspace_size(int limbs, int kind, int64_t n, int64_t nrhs, int64_t *doubles) {
+    if (limbs < 2 || limbs > 4) return fail(MFLA_ERR_ARGUMENT, "refinement needs a 2-4 limb working precision");
+    if (kind != MFLA_REFINE_CHOLESKY && kind != MFLA_REFINE_LU) return fail(MFLA_ERR_ARGUMENT, "kind must be cholesky or lu");
+    if (n < 0 || nrhs < 0) return fail(MFLA_ERR_DIMENSION, "negative dimensions");
+    *doubles = refine_layout(limbs, kind, n, nrhs, nullptr, nullptr);
+    return MFLA_OK;
+}
+
+int mfla_refine_solve(mfla_handle h, int N, int kind, int uplo, mfla_matrix A, mfla_matrix B, mfla_matrix X, double *work,
+                      int64_t work_len, double tol, int max_iter, int64_t *info, int *iterations, double *history,
+                      double *stage_ms) {
+    if (N < 2 || N > 4) return fail(MFLA_ERR_ARGUMENT, "refinement needs a 2-4 limb working precision");
+    if (kind != MFLA_REFINE_CHOLESKY && kind != MFLA_REFINE_LU) return fail(MFLA_ERR_ARGUMENT, "kind must be cholesky or lu");
+    if (uplo != MFLA_GENERAL && uplo != MFLA_LOWER && uplo != MFLA_UPPER)
+        return fail(MFLA_ERR_ARGUMENT, "uplo must be :general, :lower, or :upper");
+    if (kind == MFLA_REFINE_LU && uplo != MFLA_GENERAL) return fail(MFLA_ERR_ARGUMENT, "LU refinement needs a general matrix");
+    if (check_desc(A, "A") || check_desc(B, "B") || check_desc(X, "X")) return MFLA_ERR_ARGUMENT;
+    const int64_t n = A.rows, nrhs = B.cols;
+    if (A.cols != n || B.rows != n || X.rows != n || X.cols != nrhs) return fail(MFLA_ERR_DIMENSION, "refinement dimensions differ");
+    if (extents_overlap(X, A, N) || extents_overlap(X, B, N)) return fail(MFLA_ERR_ALIAS, "the solution must not alias A or B");
+    if (max_iter < 1) return fail(MFLA_ERR_ARGUMENT, "max_iter must be positive");
+    if (!info || !iterations || !history) return fail(MFLA_ERR_ARGUMENT, "null output pointer");
+    RefineWs w;
+    const int64_t need = refine_layout(N, kind, n, nrhs, work, &w);
+    if (work == nullptr || work_len < need) return fail(MFLA_ERR_WORKSPACE, "refinement workspace needs %lld doubles", (long long)need);
+    *iterations = 0;
+    *info = 0;
+    if (n == 0 || nrhs == 0) return MFLA_OK;
+
+    const DV Ahi{A.data, 1, A.ld, A.limb_stride};
+    const DV F{w.F, 1, n, 0}, Inv{w.Inv, 1, n, 0}, InvU{w.InvU, 1, n, 0};
+    const DV Y{w.Y, 1, n, 0}, D{w.D, 1, n, 0};
+    const DV R{w.R, 1, n, n * nrhs};
+    double *bmax = w.stats, *xmax = w.stats + nrhs, *rmax = w.stats + 2 * nrhs, *dmax = w.stats + 3 * nrhs;
+    hipStream_t s = h->stream;
+    StageClock clk{s, stage_ms != nullptr, {}};
+
+    // ---- binary64 factor and inverse factors
+    clk.mark(ST_FACTOR);
+    if (kind == MFLA_REFINE_LU) {
+        HIPCHK(mfk::dcopy(s, n, n, F, Ahi, 0));
+        mfla_matrix Fm{w.F, n, n, n, n * n};
+        double om[4];
+        int64_t finfo = 0;
+        mfla_config c;
+        mfla_config_default(&c);
+        c.lu_block = 64;
+        if (int rc = mfla_getrf(h, 1, Fm, w.ipiv, om, &finfo, &c, nullptr)) return rc;
+        *info = finfo;
+        if (finfo != 0) return fail(MFLA_ERR_FAILED_FACTOR, "SingularException(%lld)", (long long)finfo);
+    }
+    std::lock_guard<std::mutex> g(h->lock);
+    if (kind == MFLA_REFINE_CHOLESKY) {
+        HIPCHK(mfk::status_reset(s, h->status));
+        HIPCHK(mfk::dcopy(s, n, n, F, Ahi, uplo == MFLA_UPPER ? 2 : 0));
+        HIPCHK(mfk::finite_check(1, s, n, n, F, 1, h->status));
+        HIPCHK(dpotrf_core(s, n, F, Inv, h->status));
+        const int64_t r = read_info(h);
+        if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+        *info = r;
+        if (r != 0) return fail(MFLA_ERR_FAILED_FACTOR, "PosDefException(%lld)", (long long)r);
+        HIPCHK(dtrtri_core(s, n, kDiagBlock, F, Inv, true));
+    } else {
+        // P b gathers rows: perm = the ipiv row swaps applied to the identity
+        std::vector<int64_t> ip(n), perm(n);
+        HIPCHK(hipMemcpyAsync(ip.data(), w.ipiv, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) perm[i] = i;
+        for (int64_t k = 0; k < n; ++k) std::swap(perm[k], perm[ip[k] - 1]);
+        HIPCHK(hipMemcpyAsync(w.perm, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(mfk::trinv_blocks(1, s, n, (int)kDiagBlock, F, Inv, 1, 1));
+        HIPCHK(mfk::trinv_blocks(1, s, n, (int)kDiagBlock, F, InvU, 0, 0));
+        HIPCHK(dtrtri_core(s, n, kDiagBlock, F, Inv, true));
+        HIPCHK(dtrtri_core(s, n, kDiagBlock, F, InvU, false));
+    }
+    // ---- norms for the stopping test (binary64 estimates)
+    std::vector<double> hrows(n), hb(nrhs), hx(nrhs, 0.0), hr(nrhs), hd(nrhs);
+    HIPCHK(mfk::drow_abs_sum(s, n, n, Ahi, kernel_uplo(uplo), w.rows));
+    HIPCHK(mfk::col_absmax_hi(s, n, nrhs, DV{B.data, 1, B.ld, B.limb_stride}, bmax));
+    HIPCHK(hipMemcpyAsync(hrows.data(), w.rows, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hb.data(), bmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    HIPCHK(mfk::zero_fill(N, s, n, nrhs, dvn(X)));
+    HIPCHK(mfk::copy(N, s, n, nrhs, R, dvn(B)));
+    HIPCHK(hipStreamSynchronize(s));
+    double anorm = 0.0;
+    for (double v : hrows) anorm = std::max(anorm, v);
+    clk.mark(ST_FACTOR);
+
+    // ---- refinement loop
+    const DV Rhi{w.R, 1, n, 0};
+    double prev = 0.0;
+    int it = 0;
+    while (it < max_iter) {
+        // D = F64 \ hi(R)
+        mfk::DGemmArgs ga, gb;
+        if (kind == MFLA_REFINE_CHOLESKY) {
+            ga = dg(n, nrhs, n, Inv, Rhi, Y, 1.0, 0.0);
+            ga.a_tri = mfk::TRI_LOWER;
+            gb = dg(n, nrhs, n, dT(Inv), Y, D, 1.0, 0.0);
+            gb.a_tri = mfk::TRI_UPPER;
+        } else {
+            HIPCHK(mfk::dgather_rows(s, n, nrhs, D, Rhi, w.perm));
+            ga = dg(n, nrhs, n, Inv, D, Y, 1.0, 0.0);
+            ga.a_tri = mfk::TRI_LOWER;
+            gb = dg(n, nrhs, n, InvU, Y, D, 1.0, 0.0);
+            gb.a_tri = mfk::TRI_UPPER;
+        }
+        ga.ksplit = gb.ksplit = (int)w.ksplit;
+        ga.P = gb.P = w.P;
+        HIPCHK(mfk::dgemm(s, ga, 1));
+        HIPCHK(mfk::dgemm(s, gb, 1));
+        clk.mark(ST_SOLVE);
+        // precision of A*D: |D| / |X| = 2^-e needs about (53 N - e) / 52 limbs
+        HIPCHK(mfk::col_absmax_hi(s, n, nrhs, D, dmax));
+        HIPCHK(hipMemcpyAsync(hd.data(), dmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        clk.mark(ST_CHECK);
+        double rho = 0.0;
+        for (int64_t j = 0; j < nrhs; ++j) {
+            const double xn = std::max(hx[j], hd[j]);
+            if (xn > 0.0) rho = std::max(rho, hd[j] / xn);
+        }
+        int na = N;
+        if (rho > 0.0 && std::isfinite(rho)) {
+            const double limbs = std::ceil((52.0 * N + 8.0 + std::log2(rho)) / 52.0);
+            na = (int)std::min<double>(N, std::max(1.0, limbs));
+        }
+        HIPCHK(mfk::add_mixed(1, N, s, n, nrhs, dvn(X), D));
+        clk.mark(ST_UPDATE);
+        HIPCHK(mfk::resid_update(na, N, s, kernel_uplo(uplo), n, nrhs, n, Ahi, D, R));
+        clk.mark(ST_RESIDUAL);
+        HIPCHK(mfk::col_absmax_hi(s, n, nrhs, Rhi, rmax));
+        HIPCHK(mfk::col_absmax_hi(s, n, nrhs, DV{X.data, 1, X.ld, X.limb_stride}, xmax));
+        HIPCHK(hipMemcpyAsync(hr.data(), rmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hx.data(), xmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        clk.mark(ST_CHECK);
+        double be = 0.0;
+        for (int64_t j = 0; j < nrhs; ++j) {
+            const double den = anorm * hx[j] + hb[j];
+            const double e = den > 0.0 ? hr[j] / den : (hr[j] == 0.0 ? 0.0 : INFINITY);
+            be = std::max(be, e);
+            if (std::isnan(hr[j])) be = NAN;
+        }
+        history[it] = be;
+        ++it;
+        if (!std::isfinite(be) || be <= tol || (it > 1 && be > 0.5 * prev)) break;
+        prev = be;
+    }
+    *iterations = it;
+    clk.finish(stage_ms);
+    return MFLA_OK;
+}
+
+}  // extern "C"
+
+// ======================================================================
+// Adjoints of solve, Cholesky and logdet (reverse-mode rules)
+namespace {
+
+template <int N> mf::V<N> h_ldexp(mf::V<N> v, int k) {
+    for (int l = 0; l < N; ++l) v.x[l] = std::ldexp(v.x[l], k);
+    return v;
+}
+
+// exp(x) for |x| <= 1: Taylor series of expm1 at x / 2^12, then 12 steps of
+// t <- 2t + t^2 ((1 + t)^2 - 1), which keep t's relative accuracy.
+template <int N> mf::V<N> h_exp(const mf::V<N> &x) {
+    constexpr int kHalvings = 12;
+    const mf::V<N> r = h_ldexp(x, -kHalvings);
+    mf::V<N> term = r, t = r;
+    for (int k = 2; k <= 64; ++k) {
+        term = mf::div(mf::mul(term, r), mf::from<N>((double)k));
+        t = mf::add(t, term);
+        if (term.x[0] == 0.0 || std::fabs(term.x[0]) < std::ldexp(std::fabs(t.x[0]), -53 * N - 8)) break;
+    }
+    for (int i = 0; i < kHalvings; ++i) t = mf::add(mf::add(t, t), mf::mul(t, t));
+    return mf::add(mf::from<N>(1.0), t);
+}
+
+// log(m) for m in [1/2, 2]: Newton on exp, y <- y + m exp(-y) - 1, from the
+// binary64 logarithm (quadratic convergence: 53 -> 106 -> 212 bits).
+template <int N> mf::V<N> h_log(const mf::V<N> &m) {
+    mf::V<N> y = mf::from<N>(std::log(m.x[0]));
+    for (int it = 0; it < 4; ++it) y = mf::add(y, mf::sub(mf::mul(m, h_exp(mf::neg(y))), mf::from<N>(1.0)));
+    return y;
+}
+
+// logdet(S) = 2 log(prod L_ii) = 2 (e log 2 + log m)
+template <int N> void logdet_host(const double *me, double *out) {
+    mf::V<N> m;
+    for (int l = 0; l < N; ++l) m.x[l] = me[l];
+    const double e = me[N];
+    mf::V<N> v;
+    if (!(m.x[0] > 0.0) || !std::isfinite(m.x[0])) {
+        v = mf::from<N>(m.x[0] == 0.0 ? -INFINITY : NAN);
+    } else {
+        const mf::V<N> ln2 = h_log(mf::from<N>(2.0));
+        v = mf::add(mf::mul(mf::from<N>(e), ln2), h_log(m));
+        v = mf::add(v, v);
+    }
+    for (int l = 0; l < N; ++l) out[l] = v.x[l];
+}
+
+int check_adjoint_factor(int N, const mfla_matrix &F, int64_t info) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(F, "F")) return MFLA_ERR_ARGUMENT;
+    if (F.rows != F.cols) return fail(MFLA_ERR_DIMENSION, "the Cholesky factor must be square");
+    if (info != 0) return fail(MFLA_ERR_FAILED_FACTOR, "PosDefException(%lld)", (long long)info);
+    return MFLA_OK;
+}
+
+// S^{-1} B in place from the Cholesky factor (the potrs path)
+hipError_t potrs_core(mfla_handle h, int N, const mfla_matrix &F, const mfla_matrix &B, const mfla_config &c) {
+    hipError_t e = trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_NOTRANS, MFLA_NONUNIT, B, F, kOne, c, nullptr);
+    if (e != hipSuccess) return e;
+    return trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_TRANS, MFLA_NONUNIT, B, F, kOne, c, nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfla_logdet(mfla_handle h, int N, mfla_matrix F, int64_t info, double *out) {
+    if (int rc = check_adjoint_factor(N, F, info)) return rc;
+    const int64_t n = F.rows;
+    if (n == 0) { for (int l = 0; l < N; ++l) out[l] = 0.0; return MFLA_OK; }
+    std::lock_guard<std::mutex> g(h->lock);
+    double *buf = (double *)scratch(h, sizeof(double) * (N + 1));
+    if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
+    HIPCHK(mfk::diag_logprod(N, h->stream, n, dvn(F), buf));
+    double me[5];
+    HIPCHK(hipMemcpyAsync(me, buf, sizeof(double) * (N + 1), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    switch (N) {
+        case 1: logdet_host<1>(me, out); break;
+        case 2: logdet_host<2>(me, out); break;
+        case 3: logdet_host<3>(me, out); break;
+        default: logdet_host<4>(me, out); break;
+    }
+    return MFLA_OK;
+}
+
+int mfla_solve_adjoint(mfla_handle h, int N, mfla_matrix F, int64_t info, mfla_matrix X, mfla_matrix Xbar, mfla_matrix Bbar,
+                       mfla_matrix Sbar, const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (int rc = check_adjoint_factor(N, F, info)) return rc;
+    if (check_desc(X, "X") || check_desc(Xbar, "Xbar") || check_desc(Bbar, "Bbar") || check_desc(Sbar, "Sbar")) return MFLA_ERR_ARGUMENT;
+    const int64_t n = F.rows, nrhs = X.cols;
+    if (X.rows != n || Xbar.rows != n || Xbar.cols != nrhs || Bbar.rows != n || Bbar.cols != nrhs)
+        return fail(MFLA_ERR_DIMENSION, "solution, cotangent and right-hand-side gradient dimensions differ");
+    if (Sbar.rows != n || Sbar.cols != n) return fail(MFLA_ERR_DIMENSION, "matrix gradient dimensions differ");
+    const bool inplace = same_matrix(Bbar, Xbar);
+    if ((!inplace && extents_overlap(Bbar, Xbar, N)) || extents_overlap(Bbar, X, N) || extents_overlap(Bbar, F, N) ||
+        extents_overlap(Sbar, Bbar, N) || extents_overlap(Sbar, X, N) || extents_overlap(Sbar, F, N) || extents_overlap(Sbar, Xbar, N))
+        return fail(MFLA_ERR_ALIAS, "adjoint outputs must not alias the inputs");
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    if (!inplace) HIPCHK(mfk::copy(N, s, n, nrhs, dvn(Bbar), dvn(Xbar)));
+    HIPCHK(potrs_core(h, N, F, Bbar, c));                                    // Bbar = S^{-1} Xbar
+    HIPCHK(gemm_launch(h, N, c.gemm_strategy, c.gemm_tile, n, n, nrhs, dvn(Bbar), dvt(X), dvn(Sbar), kOne, kZero, nullptr));
+    const double mhalf[4] = {-0.5, 0.0, 0.0, 0.0};
+    HIPCHK(mfk::symmetrize(N, s, n, dvn(Sbar), dvn(Sbar), mhalf));          // Sbar = -sym(Bbar X^T)
+    return MFLA_OK;
+}
+
+int mfla_logdet_adjoint(mfla_handle h, int N, mfla_matrix F, int64_t info, const double *ybar, mfla_matrix Sbar,
+                        const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (int rc = check_adjoint_factor(N, F, info)) return rc;
+    if (check_desc(Sbar, "Sbar")) return MFLA_ERR_ARGUMENT;
+    const int64_t n = F.rows;
+    if (Sbar.rows != n || Sbar.cols != n) return fail(MFLA_ERR_DIMENSION, "matrix gradient dimensions differ");
+    if (extents_overlap(Sbar, F, N)) return fail(MFLA_ERR_ALIAS, "the gradient must not alias the factor");
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    HIPCHK(mfk::identity(N, s, n, dvn(Sbar)));
+    HIPCHK(potrs_core(h, N, F, Sbar, c));                                    // S^{-1}
+    double half_ybar[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int l = 0; l < N; ++l) half_ybar[l] = 0.5 * ybar[l];
+    HIPCHK(mfk::symmetrize(N, s, n, dvn(Sbar), dvn(Sbar), half_ybar));     // ybar * sym(S^{-1})
+    return MFLA_OK;
+}
+
+int mfla_cholesky_adjoint(mfla_handle h, int N, mfla_matrix F, int64_t info, mfla_matrix Lbar, mfla_matrix Sbar, mfla_matrix work,
+                          const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (int rc = check_adjoint_factor(N, F, info)) return rc;
+    if (check_desc(Lbar, "Lbar") || check_desc(Sbar, "Sbar") || check_desc(work, "work")) return MFLA_ERR_ARGUMENT;
+    const int64_t n = F.rows;
+    if (Lbar.rows != n || Lbar.cols != n || Sbar.rows != n || Sbar.cols != n || work.rows != n || work.cols != n)
+        return fail(MFLA_ERR_DIMENSION, "adjoint operands must be n x n");
+    if (extents_overlap(Sbar, F, N) || extents_overlap(Sbar, Lbar, N) || extents_overlap(work, F, N) || extents_overlap(work, Lbar, N) ||
+        extents_overlap(work, Sbar, N))
+        return fail(MFLA_ERR_ALIAS, "adjoint outputs must not alias the inputs");
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    HIPCHK(mfk::tril(N, s, n, dvn(F), dvn(work), 0));                        // L (upper storage ignored)
+    HIPCHK(gemm_launch(h, N, c.gemm_strategy, c.gemm_tile, n, n, n, dvt(work), dvn(Lbar), dvn(Sbar), kOne, kZero, nullptr));
+    HIPCHK(mfk::tril(N, s, n, dvn(Sbar), dvn(Sbar), 1));                     // Phi(L^T Lbar)
+    HIPCHK(trsm_core(h, N, MFLA_RIGHT, MFLA_LOWER, MFLA_NOTRANS, MFLA_NONUNIT, Sbar, F, kOne, c, nullptr));   // * L^{-1}
+    HIPCHK(trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_TRANS, MFLA_NONUNIT, Sbar, F, kOne, c, nullptr));     // L^{-T} *
+    const double half[4] = {0.5, 0.0, 0.0, 0.0};
+    HIPCHK(mfk::symmetrize(N, s, n, dvn(Sbar), dvn(Sbar), half));
     return MFLA_OK;
 }
 

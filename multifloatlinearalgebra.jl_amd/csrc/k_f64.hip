@@ -1,0 +1,270 @@
+// Binary64 kernels of the refinement's low-precision factor and correction
+// solve. These run in plain FP64 (no multi-double networks, no MFLA order
+// constraints), so the GEMM-shaped work goes to the CDNA4 FP64 matrix cores:
+//
+//  * k_dgemm: C = alpha * op(A) * op(B) + beta * C on strided views, 64x64
+//    tile per 256-thread workgroup (four waves, each 32x32 = 2x2 MFMA
+//    16x16x4 f64 blocks), A/B k-panels of 16 staged in LDS. Optional
+//    triangular structure of A / B (the k range of each tile is clipped and
+//    the zero triangle is never read), a lower-triangular output mask, a
+//    batch index (grid z) with element strides, and a split-k mode that
+//    writes partial sums for k_dgemm_reduce (fixed summation order, so the
+//    result is deterministic).
+//  * k_dpotrf_diag: Cholesky of one nb x nb (nb <= 64) diagonal block in LDS,
+//    plus the inverse of the resulting triangle (needed by the panel update
+//    L21 = A21 * inv(L11)^T and by the correction solve).
+//  * small helpers: copy of a binary64 plane with optional mirroring, row
+//    sums of |A|, row gathers by a permutation.
+#include "mf_common.hpp"
+#include "k_f64.hpp"
+
+namespace mfk {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int DG_T = 64, DG_BK = 16, DG_PAD = 1;
+
+// v_mfma_f64_16x16x4_f64 operand layout (lane l of the wave):
+//   A fragment: A[l % 16][l / 16]   (16 x 4)
+//   B fragment: B[l / 16][l % 16]   (4 x 16)
+//   D item v:   D[4 * (l / 16) + v][l % 16]
+__global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
+    __shared__ double As[DG_BK][DG_T + DG_PAD];
+    __shared__ double Bs[DG_BK][DG_T + DG_PAD];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wy = w & 1, wx = w >> 1;
+    const int64_t tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int64_t i0 = tm * DG_T, j0 = tn * DG_T;
+    const int64_t z = blockIdx.z;
+    const double *Ap = g.A.p + z * g.za, *Bp = g.B.p + z * g.zb;
+    double *Cp = g.C.p + z * g.zc;
+    if (g.c_lower && j0 > i0 + DG_T - 1) return;  // tile entirely above the diagonal
+
+    // k range of this tile (zero triangles of A / B contribute nothing)
+    int64_t klo = 0, khi = g.k;
+    if (g.a_tri == TRI_LOWER) khi = khi < i0 + DG_T ? khi : i0 + DG_T;
+    if (g.a_tri == TRI_UPPER) klo = klo > i0 ? klo : i0;
+    if (g.b_tri == TRI_LOWER) klo = klo > j0 ? klo : j0;
+    if (g.b_tri == TRI_UPPER) khi = khi < j0 + DG_T ? khi : j0 + DG_T;
+    // split-k: slice s of ksplit covers an equal share of [klo, khi)
+    const int ks = blockIdx.y;
+    if (g.ksplit > 1) {
+        const int64_t span = khi > klo ? khi - klo : 0;
+        const int64_t chunk = ((span + g.ksplit - 1) / g.ksplit + DG_BK - 1) / DG_BK * DG_BK;
+        const int64_t a = klo + ks * chunk, b = a + chunk;
+        klo = a;
+        khi = b < khi ? b : khi;
+    }
+
+    d4 acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+
+    const bool a_rows_fast = g.A.rs == 1, b_cols_fast = g.B.cs == 1;
+    for (int64_t k0 = klo; k0 < khi; k0 += DG_BK) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int ii, kk;
+            if (a_rows_fast) { ii = t & 63; kk = (t >> 6) + 4 * q; }
+            else { kk = t & 15; ii = (t >> 4) + 16 * q; }
+            const int64_t gi = i0 + ii, gk = k0 + kk;
+            bool ok = gi < g.m && gk < khi;
+            if (g.a_tri == TRI_LOWER) ok = ok && gk <= gi;
+            if (g.a_tri == TRI_UPPER) ok = ok && gk >= gi;
+            As[kk][ii] = ok ? Ap[gi * g.A.rs + gk * g.A.cs] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int jj, kk;
+            if (b_cols_fast) { jj = t & 63; kk = (t >> 6) + 4 * q; }
+            else { kk = t & 15; jj = (t >> 4) + 16 * q; }
+            const int64_t gj = j0 + jj, gk = k0 + kk;
+            bool ok = gj < g.n && gk < khi;
+            if (g.b_tri == TRI_LOWER) ok = ok && gk >= gj;
+            if (g.b_tri == TRI_UPPER) ok = ok && gk <= gj;
+            Bs[kk][jj] = ok ? Bp[gk * g.B.rs + gj * g.B.cs] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k4 = 0; k4 < DG_BK / 4; ++k4) {
+            const int kr = k4 * 4 + (lane >> 4);
+            double a[2], b[2];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi) a[bi] = As[kr][wy * 32 + bi * 16 + (lane & 15)];
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) b[bj] = Bs[kr][wx * 32 + bj * 16 + (lane & 15)];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t i = i0 + wy * 32 + bi * 16 + 4 * (lane >> 4) + v;
+                const int64_t j = j0 + wx * 32 + bj * 16 + (lane & 15);
+                if (i >= g.m || j >= g.n) continue;
+                if (g.c_lower && i < j) continue;
+                const double r = acc[bi][bj][v];
+                if (g.ksplit > 1) {
+                    g.P[((int64_t)ks * g.n + j) * g.m + i] = r;
+                } else {
+                    double *c = Cp + i * g.C.rs + j * g.C.cs;
+                    *c = g.beta_zero ? g.alpha * r : g.alpha * r + g.beta * *c;
+                }
+            }
+}
+
+// C = alpha * sum_s P[s] + beta * C, slices summed in ascending s.
+__global__ void k_dgemm_reduce(DGemmArgs g) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.m * g.n) return;
+    const int64_t i = e % g.m, j = e / g.m;
+    if (g.c_lower && i < j) return;
+    double s = 0.0;
+    for (int q = 0; q < g.ksplit; ++q) s += g.P[((int64_t)q * g.n + j) * g.m + i];
+    double *c = g.C.p + i * g.C.rs + j * g.C.cs;
+    *c = g.beta_zero ? g.alpha * s : g.alpha * s + g.beta * *c;
+}
+
+// ------------------------------------------------ diagonal block Cholesky
+// Factor the nb x nb block at (j0, j0) of F (lower triangle read), write L
+// back to F's lower triangle and inv(L) to Inv's lower triangle at (j0, j0).
+// A non-positive or non-finite pivot at block column c records info =
+// j0 + c + 1 and stops the factorization.
+__global__ __launch_bounds__(256) void k_dpotrf_diag(DV F, DV Inv, int64_t j0, int nb, Status *st) {
+    if (st->info != 0) return;
+    __shared__ double L[64 * 64];   // column-major: L[q * 64 + r]
+    __shared__ double X[64 * 64];   // row-major:    X[r * 64 + c]
+    __shared__ int fail_s;
+    const int t = threadIdx.x;
+    for (int e = t; e < 64 * 64; e += 256) {
+        const int r = e & 63, q = e >> 6;
+        L[e] = (r < nb && q < nb && r >= q) ? F.p[(j0 + r) * F.rs + (j0 + q) * F.cs] : 0.0;
+    }
+    if (t == 0) fail_s = 0;
+    __syncthreads();
+    for (int c = 0; c < nb; ++c) {
+        if (t == 0) {
+            const double d = L[c * 64 + c];
+            if (!(d > 0.0) || !__builtin_isfinite(d)) {
+                fail_s = 1;
+                st->info = j0 + c + 1;
+            } else {
+                L[c * 64 + c] = __builtin_sqrt(d);
+            }
+        }
+        __syncthreads();
+        if (fail_s) return;
+        const double p = L[c * 64 + c];
+        if (t > c && t < nb) L[c * 64 + t] /= p;
+        __syncthreads();
+        const int w = nb - c - 1;
+        for (int e = t; e < w * w; e += 256) {
+            const int r = c + 1 + e % w, q = c + 1 + e / w;
+            if (r >= q) L[q * 64 + r] -= L[c * 64 + r] * L[c * 64 + q];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < 64 * 64; e += 256) {
+        const int r = e & 63, q = e >> 6;
+        if (r < nb && q < nb && r >= q) F.p[(j0 + r) * F.rs + (j0 + q) * F.cs] = L[e];
+    }
+    // inv(L) by forward substitution against the identity, one column per thread
+    if (t < nb) {
+        const int c = t;
+        for (int r = c; r < nb; ++r) {
+            double s = (r == c) ? 1.0 : 0.0;
+            for (int k = c; k < r; ++k) s -= L[k * 64 + r] * X[k * 64 + c];
+            X[r * 64 + c] = s / L[r * 64 + r];
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * 64; e += 256) {
+        const int r = e & 63, q = e >> 6;
+        if (r < nb && q < nb && r >= q) Inv.p[(j0 + r) * Inv.rs + (j0 + q) * Inv.cs] = X[r * 64 + q];
+    }
+}
+
+// ------------------------------------------------------------ helpers
+// dst(i, j) = src(i, j) for the requested region; mirror = 1 fills the upper
+// triangle from the lower one (dst(i, j) = src(j, i) for i < j), mirror = 2
+// the lower from the upper.
+__global__ void k_dcopy(int64_t m, int64_t n, DV dst, DV src, int mirror) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const int64_t i = e % m, j = e / m;
+    int64_t si = i, sj = j;
+    if ((mirror == 1 && i < j) || (mirror == 2 && i > j)) { si = j; sj = i; }
+    dst.p[i * dst.rs + j * dst.cs] = src.p[si * src.rs + sj * src.cs];
+}
+
+// out[i] = sum_j |A(i, j)| (binary64 plane; uplo 1/2 reads a stored triangle
+// symmetrically).
+__global__ void k_drow_abs_sum(int64_t m, int64_t n, DV A, int uplo, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t a = i, b = j;
+        if ((uplo == 1 && a < b) || (uplo == 2 && a > b)) { a = j; b = i; }
+        s += fabs(A.p[a * A.rs + b * A.cs]);
+    }
+    out[i] = s;
+}
+
+// dst(i, c) = src(perm[i], c)
+__global__ void k_dgather_rows(int64_t m, int64_t ncols, DV dst, DV src, const int64_t *perm) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * ncols) return;
+    const int64_t i = e % m, c = e / m;
+    dst.p[i * dst.rs + c * dst.cs] = src.p[perm[i] * src.rs + c * src.cs];
+}
+
+// ---------------------------------------------------------- launchers
+hipError_t dgemm(hipStream_t s, DGemmArgs g, int64_t batch) {
+    if (g.m == 0 || g.n == 0 || batch == 0) return hipSuccess;
+    if (g.ksplit < 1) g.ksplit = 1;
+    if (g.ksplit > 1 && (g.P == nullptr || batch != 1)) return hipErrorInvalidValue;
+    const int64_t tiles_m = cdiv(g.m, DG_T), tiles_n = cdiv(g.n, DG_T);
+    const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)g.ksplit, (unsigned)batch);
+    hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, g, (int)tiles_m);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || g.ksplit == 1) return e;
+    hipLaunchKernelGGL(k_dgemm_reduce, dim3((unsigned)cdiv(g.m * g.n, 256)), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t dpotrf_diag(hipStream_t s, const DV &F, const DV &Inv, int64_t j0, int nb, Status *st) {
+    if (nb < 1 || nb > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_dpotrf_diag, dim3(1), dim3(256), 0, s, F, Inv, j0, nb, st);
+    return hipGetLastError();
+}
+
+hipError_t dcopy(hipStream_t s, int64_t m, int64_t n, const DV &dst, const DV &src, int mirror) {
+    if (m * n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dcopy, dim3((unsigned)cdiv(m * n, 256)), dim3(256), 0, s, m, n, dst, src, mirror);
+    return hipGetLastError();
+}
+
+hipError_t drow_abs_sum(hipStream_t s, int64_t m, int64_t n, const DV &A, int uplo, double *out) {
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_drow_abs_sum, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m, n, A, uplo, out);
+    return hipGetLastError();
+}
+
+hipError_t dgather_rows(hipStream_t s, int64_t m, int64_t ncols, const DV &dst, const DV &src, const int64_t *perm) {
+    if (m * ncols == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dgather_rows, dim3((unsigned)cdiv(m * ncols, 256)), dim3(256), 0, s, m, ncols, dst, src, perm);
+    return hipGetLastError();
+}
+
+}  // namespace mfk

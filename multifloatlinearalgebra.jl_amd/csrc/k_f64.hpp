@@ -1,0 +1,34 @@
+// Host-side interface of the binary64 refinement kernels (k_f64.hip).
+#pragma once
+#include "mf_common.hpp"
+
+namespace mfk {
+
+enum { TRI_NONE = 0, TRI_LOWER = 1, TRI_UPPER = 2 };
+
+// C = alpha * A * B + beta * C on strided binary64 views (ls unused).
+// a_tri: A(i, k) is zero for k > i (LOWER) / k < i (UPPER); b_tri: B(k, j) is
+// zero for k < j (LOWER) / k > j (UPPER); those entries are never read.
+// c_lower: only i >= j of C is written. Batch entry z offsets A, B, C by
+// z * za / zb / zc elements. ksplit > 1 (batch 1 only) writes partial sums
+// to P (ksplit * m * n doubles) and reduces them in ascending slice order.
+struct DGemmArgs {
+    int64_t m, n, k;
+    DV A, B, C;
+    double alpha, beta;
+    int beta_zero;
+    int a_tri, b_tri, c_lower;
+    int64_t za, zb, zc;
+    int ksplit;
+    double *P;
+};
+
+hipError_t dgemm(hipStream_t s, DGemmArgs g, int64_t batch);
+// Cholesky of the nb x nb (nb <= 64) diagonal block at (j0, j0) of F and the
+// inverse of its factor into Inv at (j0, j0).
+hipError_t dpotrf_diag(hipStream_t s, const DV &F, const DV &Inv, int64_t j0, int nb, Status *st);
+hipError_t dcopy(hipStream_t s, int64_t m, int64_t n, const DV &dst, const DV &src, int mirror);
+hipError_t drow_abs_sum(hipStream_t s, int64_t m, int64_t n, const DV &A, int uplo, double *out);
+hipError_t dgather_rows(hipStream_t s, int64_t m, int64_t ncols, const DV &dst, const DV &src, const int64_t *perm);
+
+}  // namespace mfk

@@ -12,22 +12,89 @@
 namespace mfk {
 
 // -------------------------------------------------------------- Cholesky
-// src/factorizations/cholesky.jl:10-34, one workgroup per panel.
+// src/factorizations/cholesky.jl:10-34, one workgroup per panel diagonal
+// block. Per column c: thread 0 forms d = A[c,c] - sum_k A[c,k]^2 while
+// thread t forms v = A[c+t,c] - sum_k A[c+t,k] A[c,k] for the rows below,
+// both over k ascending from the panel start (the same element formulas and
+// order as MFLA, evaluated concurrently); then sqrt(d) and v / sqrt(d).
+// Panels of up to 32 columns keep their lower triangle in LDS.
+constexpr int CHOL_LDS = 32;
+
+template <int N>
+__global__ __launch_bounds__(256) void k_chol_panel_lds(DV A, int64_t first, int64_t last, Status *st) {
+    if (st->info != 0) return;
+    __shared__ double P[CHOL_LDS * CHOL_LDS * N];     // P[(q * 32 + r) * N + l] = A(first + r, first + q)
+    __shared__ double piv_s[N];
+    __shared__ int fail_s;
+    const int nb = (int)(last - first + 1);
+    const int t = threadIdx.x;
+    for (int e = t; e < nb * nb; e += blockDim.x) {
+        const int r = e % nb, q = e / nb;
+        if (r >= q) {
+            const mf::V<N> v = ld<N>(A, first + r, first + q);
+#pragma unroll
+            for (int l = 0; l < N; ++l) P[(q * CHOL_LDS + r) * N + l] = v.x[l];
+        }
+    }
+    __syncthreads();
+    auto get = [&](int r, int q) {
+        mf::V<N> v;
+#pragma unroll
+        for (int l = 0; l < N; ++l) v.x[l] = P[(q * CHOL_LDS + r) * N + l];
+        return v;
+    };
+    for (int c = 0; c < nb; ++c) {
+        const int r = c + t;
+        mf::V<N> v = mf::zero<N>();
+        if (r < nb) {
+            v = get(r, c);
+            for (int k = 0; k < c; ++k) v = mf::msub(v, get(r, k), get(c, k));
+        }
+        if (t == 0) {
+            if (!mf::gt(v, mf::zero<N>())) {
+                fail_s = 1;
+                st->info = first + c + 1;
+            } else {
+                fail_s = 0;
+                const mf::V<N> p = mf::sqrt(v);
+                stv<N>(A, first + c, first + c, p);
+                lds_put<N>(piv_s, p);
+#pragma unroll
+                for (int l = 0; l < N; ++l) P[(c * CHOL_LDS + c) * N + l] = p.x[l];
+            }
+        }
+        __syncthreads();
+        if (fail_s) return;
+        if (t > 0 && r < nb) {
+            const mf::V<N> q = mf::div(v, lds_get<N>(piv_s));
+            stv<N>(A, first + r, first + c, q);
+#pragma unroll
+            for (int l = 0; l < N; ++l) P[(c * CHOL_LDS + r) * N + l] = q.x[l];
+        }
+        __syncthreads();
+    }
+}
+
+// Panels wider than 32 columns: the same computation from global memory.
 template <int N>
 __global__ __launch_bounds__(256) void k_chol_panel(DV A, int64_t first, int64_t last, Status *st) {
     if (st->info != 0) return;
     __shared__ double piv_s[N];
     __shared__ int fail_s;
     for (int64_t c = first; c <= last; ++c) {
+        const int64_t r = c + threadIdx.x;
+        mf::V<N> v = mf::zero<N>();
+        if (r <= last) {
+            v = ld<N>(A, r, c);
+            for (int64_t k = first; k < c; ++k) v = mf::msub(v, ld<N>(A, r, k), ld<N>(A, c, k));
+        }
         if (threadIdx.x == 0) {
-            mf::V<N> d = ld<N>(A, c, c);
-            for (int64_t k = first; k < c; ++k) { mf::V<N> v = ld<N>(A, c, k); d = mf::msub(d, v, v); }
-            if (!mf::gt(d, mf::zero<N>())) {
+            if (!mf::gt(v, mf::zero<N>())) {
                 fail_s = 1;
                 st->info = c + 1;
             } else {
                 fail_s = 0;
-                mf::V<N> p = mf::sqrt(d);
+                mf::V<N> p = mf::sqrt(v);
                 stv<N>(A, c, c, p);
                 lds_put<N>(piv_s, p);
             }
@@ -35,10 +102,12 @@ __global__ __launch_bounds__(256) void k_chol_panel(DV A, int64_t first, int64_t
         __syncthreads();
         if (fail_s) return;
         const mf::V<N> p = lds_get<N>(piv_s);
-        for (int64_t r = c + 1 + threadIdx.x; r <= last; r += blockDim.x) {
-            mf::V<N> v = ld<N>(A, r, c);
-            for (int64_t k = first; k < c; ++k) v = mf::msub(v, ld<N>(A, r, k), ld<N>(A, c, k));
-            stv<N>(A, r, c, mf::div(v, p));
+        for (int64_t rr = c + threadIdx.x; rr <= last; rr += blockDim.x) {
+            if (rr == c) continue;
+            mf::V<N> w = (rr == r) ? v : ld<N>(A, rr, c);
+            if (rr != r)
+                for (int64_t k = first; k < c; ++k) w = mf::msub(w, ld<N>(A, rr, k), ld<N>(A, c, k));
+            stv<N>(A, rr, c, mf::div(w, p));
         }
         __syncthreads();
     }
@@ -305,7 +374,11 @@ static inline unsigned nb(int64_t work, int b) { return (unsigned)cdiv(work > 0 
 template <int N> static S<N> mk(const double *v) { S<N> s; for (int l = 0; l < N; ++l) s.x[l] = v[l]; return s; }
 
 hipError_t chol_panel(int N, hipStream_t s, const DV &A, int64_t first, int64_t last, Status *st) {
-    MF_DISPATCH(N, hipLaunchKernelGGL(k_chol_panel<NN>, dim3(1), dim3(256), 0, s, A, first, last, st));
+    if (last - first + 1 <= CHOL_LDS) {
+        MF_DISPATCH(N, hipLaunchKernelGGL(k_chol_panel_lds<NN>, dim3(1), dim3(256), 0, s, A, first, last, st));
+    } else {
+        MF_DISPATCH(N, hipLaunchKernelGGL(k_chol_panel<NN>, dim3(1), dim3(256), 0, s, A, first, last, st));
+    }
     return hipGetLastError();
 }
 

@@ -96,6 +96,15 @@ PROTOTYPES = {
     "mfla_batched_solve": (_I, [_P, _I, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I, _CP]),
     "mfla_batched_max_order": (_I, []),
     "mfla_batched_max_rhs": (_I, []),
+    "mfla_refine_workspace_size": (_I, [_I, _I, _I64, _I64, ctypes.POINTER(_I64)]),
+    "mfla_refine_solve": (_I, [_P, _I, _I, _I, _M, _M, _M, _P, _I64, ctypes.c_double, _I, ctypes.POINTER(_I64),
+                               ctypes.POINTER(_I), _DP, _DP]),
+    "mfla_dgemm": (_I, [_P, _I, _I, _I, _I, _M, _M, _M, ctypes.c_double, ctypes.c_double]),
+    "mfla_dpotrf_inverse": (_I, [_P, _M, _M, ctypes.POINTER(_I64)]),
+    "mfla_logdet": (_I, [_P, _I, _M, _I64, _DP]),
+    "mfla_solve_adjoint": (_I, [_P, _I, _M, _I64, _M, _M, _M, _M, _CP]),
+    "mfla_logdet_adjoint": (_I, [_P, _I, _M, _I64, _DP, _M, _CP]),
+    "mfla_cholesky_adjoint": (_I, [_P, _I, _M, _I64, _M, _M, _M, _CP]),
 }
 
 _lib = None

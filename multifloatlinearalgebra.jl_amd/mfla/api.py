@@ -450,6 +450,30 @@ def normwise_backward_error(A, x, b, r, *, uplo="general"):
     return out[:, 0].copy() if x.dim() == 2 else out
 
 
+# ------------------------------------------------- binary64 building blocks
+_TRANS = {"N": 0, "T": 1}
+
+
+def dgemm_(C, A, B, alpha=1.0, beta=0.0, *, trans_a="N", trans_b="N", tri_a="general", tri_b="general"):
+    """Binary64 C = alpha*op(A)*op(B) + beta*C on the FP64 matrix cores, using
+    the leading limb plane of each operand. tri_a / tri_b declare op(A) / op(B)
+    lower or upper triangular (the other triangle is never read)."""
+    ta = _sym(trans_a, _TRANS, "trans must be 'N' or 'T'")
+    tb = _sym(trans_b, _TRANS, "trans must be 'N' or 'T'")
+    ra = _sym(tri_a, _UPLO_R, "triangle must be :general, :lower, or :upper")
+    rb = _sym(tri_b, _UPLO_R, "triangle must be :general, :lower, or :upper")
+    _raise(L.load().mfla_dgemm(_handle(), ta, tb, ra, rb, _mat(C), _mat(A), _mat(B), float(alpha), float(beta)))
+    return C
+
+
+def dpotrf_inverse_(A, Inv):
+    """Binary64 Cholesky of A's lower triangle in place and the inverse of the
+    factor into Inv's lower triangle. Returns the status (0, -1, k > 0)."""
+    info = ctypes.c_int64(0)
+    _raise(L.load().mfla_dpotrf_inverse(_handle(), _mat(A), _mat(Inv), ctypes.byref(info)))
+    return int(info.value)
+
+
 # --------------------------------------------------------------- interop
 def from_interleaved(dst, src):
     """Copy a Julia-layout (AoS) Float64xN matrix -- a CUDA float64 tensor of

@@ -137,6 +137,49 @@ class Batch:
         return out
 
 
+def _sym_low_limbs(hi, N, rng):
+    """Symmetric lower limbs hi * 2^-53k * U(-1, 1) (valid Float64xN values)."""
+    out = np.zeros((N,) + hi.shape)
+    out[0] = hi
+    for l in range(1, N):
+        u = rng.uniform(-1.0, 1.0, hi.shape)
+        u = (u + u.T) / 2
+        out[l] = out[l - 1] * 2.0**-53 * u
+    return out
+
+
+def synthetic_systems(count, nmin=16, nmax=256, N=3, nrhs=4, seed=3):
+    """The uneven batch of the benchmark configuration: n ~ U{nmin..nmax},
+    alternating SPD R R^T + n I (kind CHOLESKY) and KKT [H A^T; A -1e-8 I]
+    with H = R R^T + m I (m x m), A ~ N(0, 1) (p x m), p = round(m / 2),
+    m + p = n (kind LDLT); right-hand sides ~ N(0, 1). Planar (N, cols, rows)
+    arrays, symmetric in every limb."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        if i % 2 == 0:
+            R = rng.standard_normal((n, n))
+            A = _sym_low_limbs(R @ R.T + n * np.eye(n), N, rng)
+            kind = CHOLESKY
+        else:
+            m = int(round(2 * n / 3))
+            p = n - m
+            R = rng.standard_normal((m, m))
+            K = np.zeros((n, n))
+            K[:m, :m] = R @ R.T + m * np.eye(m)
+            Ab = rng.standard_normal((p, m))
+            K[m:, :m] = Ab
+            K[:m, m:] = Ab.T
+            K[m:, m:] = -1e-8 * np.eye(p)
+            A = _sym_low_limbs(K, N, rng)
+            kind = LDLT
+        B = np.zeros((N, nrhs, n))
+        B[0] = rng.standard_normal((nrhs, n))
+        out.append((kind, A, B))
+    return out
+
+
 def _time(fn, reps=3):
     ts = []
     for _ in range(reps):

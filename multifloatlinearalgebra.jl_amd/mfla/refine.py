@@ -2,18 +2,19 @@
 
 MFLA provides the pieces -- residual!, refinement_correction! (exactly one
 ldiv!), normwise_backward_error -- and leaves the loop to the caller
-(src/residual.jl:294-300, docs/SDPX_PROVIDER_READINESS.md). This module is
-that loop on the device, with the factorization done in FP64:
+(src/residual.jl:294-300, docs/SDPX_PROVIDER_READINESS.md). The library's
+native driver (mfla_refine_solve) runs that loop on the device:
 
-    F64 = factor(hi limb of A)                      # FP64 Cholesky / LU
+    F64 = factor(hi limb of A), inv(F64) by recursive doubling
     X = 0, R = B
-    repeat:  D = F64 \\ hi(R)      (FP64 solve of the residual's leading limb)
-             X = X + D             (Float64xN accumulation, add_mixed_)
-             R = B - A*X           (Float64xN residual, residual_)
-             until normwise_backward_error <= tol or it stops improving
+    repeat:  D = inv(F64) hi(R)    (two binary64 GEMMs on the matrix cores)
+             X = X + D             (Float64xN accumulation)
+             R = R - A*D           (summed with just enough limbs for |D|/|X|)
+             until the backward error estimate <= tol or it stops improving
 
 Convergence needs kappa(A) * u_64 < 1 (kappa <= 1e8 here); each step gains
-~ -log10(kappa * u_64) digits until the Float64xN residual floor.
+~ -log10(kappa * u_64) digits until the Float64xN residual floor. The same
+loop over MFLA's public pieces is refine_solve_pieces.
 """
 from __future__ import annotations
 
@@ -113,11 +114,57 @@ class _Timer:
         return {k: round(v, 3) for k, v in self.t.items()}
 
 
-def refine_solve(A, B, kind="cholesky", *, block=64, tol=None, max_iter=30, uplo="general", stages=False):
-    """Solve A X = B (A, B Float64xN on the device) with an FP64 factor and
-    Float64xN refinement. Returns (X, history) where history holds the
-    backward error after each correction; with stages=True also a dict of
-    milliseconds per stage (factor, solve, update, residual, check)."""
+_REFINE_KIND = {"cholesky": 0, "lu": 1}
+_STAGES = ("factor", "solve", "update", "residual", "check")
+_ws_cache: dict = {}
+
+
+def _refine_workspace(N, kind, n, nrhs):
+    import ctypes
+    from . import _lib as L
+    need = ctypes.c_int64(0)
+    M._raise(L.load().mfla_refine_workspace_size(N, _REFINE_KIND[kind], n, nrhs, ctypes.byref(need)))
+    key = (torch.cuda.current_device(), N, kind)
+    w = _ws_cache.get(key)
+    if w is None or w.numel() < need.value:
+        w = torch.empty(max(need.value, 1), dtype=torch.float64, device="cuda")
+        _ws_cache[key] = w
+    return w, need.value
+
+
+def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general", stages=False):
+    """Solve A X = B (A, B Float64xN on the device, N = 2..4) with a binary64
+    factor and Float64xN refinement, in the library's native driver
+    (mfla_refine_solve). Returns (X, history) -- history holds the binary64
+    backward-error estimate after each correction -- and with stages=True
+    also milliseconds per stage (factor, solve, update, residual, check)."""
+    import ctypes
+    from . import _lib as L
+    N = M._limbs(A, B)
+    tol = tol if tol is not None else 64.0 * 2.0 ** (-52 * N)
+    n, nrhs = A.shape[2], B.shape[1]
+    X = torch.empty_like(B)
+    work, length = _refine_workspace(N, kind, n, nrhs)
+    u = M._sym(uplo, M._UPLO_R, "uplo must be :general, :lower, or :upper")
+    hist = np.zeros(max_iter)
+    st = np.zeros(5)
+    info = ctypes.c_int64(0)
+    iters = ctypes.c_int(0)
+    rc = L.load().mfla_refine_solve(M._handle(), N, _REFINE_KIND[kind], u, M._mat(A), M._mat(B), M._mat(X),
+                                    ctypes.c_void_p(work.data_ptr()), length, float(tol), int(max_iter), ctypes.byref(info),
+                                    ctypes.byref(iters), M._sp(hist), M._sp(st) if stages else None)
+    M._raise(rc)
+    h = [float(v) for v in hist[: iters.value]]
+    if stages:
+        return X, h, {k: round(float(v), 3) for k, v in zip(_STAGES, st)}
+    return X, h
+
+
+def refine_solve_pieces(A, B, kind="cholesky", *, block=64, tol=None, max_iter=30, uplo="general", stages=False):
+    """The same refinement written as a caller would with MFLA's public
+    pieces: factor the leading limb with cholesky!/lu!, then ldiv! on the
+    residual's leading limb, add_mixed_, residual! and
+    normwise_backward_error per iteration."""
     N = A.shape[0]
     tol = tol if tol is not None else 64.0 * 2.0 ** (-52 * N)
     T = _Timer(stages)
@@ -171,6 +218,7 @@ def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812
     B = torch.from_numpy(Bh).cuda()
     out = {"n": n, "nrhs": nrhs, "kind": kind, "kappa": kappa, "limbs": N}
     for name, fn in (("refined", lambda: refine_solve(A, B, kind)), ("plain", lambda: (plain_solve(A, B, kind), None))):
+        fn()                                   # warm-up: code-object loading, workspace allocation
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         X, hist = fn()

@@ -1,0 +1,237 @@
+"""Measurement suite: every operation of the Float64xN path at the sizes of the
+benchmark configurations, on one GPU, with the work model each figure is
+priced by.
+
+Compute-bound operations report EFT FP64-TFLOP/s = MACs * F_N / t against the
+FP64 vector peak (78.6 TFLOP/s), with F_N the FLOPs of one multiply-accumulate
+network (2, 30, 111, 239 for N = 1..4), and the FP64 instruction issue rate
+(instructions per MAC: 2, 29, 108, 233; 99 on the x3 fused route) against
+39.3 T lane-instructions/s. Level-2 operations also report HBM GB/s against
+8 TB/s with compulsory bytes 8N(mn + m + n) per right-hand side.
+
+Inputs are seeded synthetic Float64xN matrices: hi limb N(0, 1), lower limbs
+hi * 2^-53 * U(-1, 1) (not renormalised; timing does not depend on it).
+SPD matrices are R R^T + n I formed with gemm!; the triangle of trsm! is the
+Cholesky factor of such a matrix.
+"""
+from __future__ import annotations
+
+import json
+import time
+
+import numpy as np
+import torch
+
+from . import api as M
+
+F_N = {1: 2, 2: 30, 3: 111, 4: 239}
+INSTR_N = {1: 2, 2: 29, 3: 108, 4: 233}
+INSTR_X3_FUSED = 99
+PEAK_TF = 78.6
+PEAK_INSTR = 39.3e12
+PEAK_HBM = 8.0e12
+
+
+def rand_dev(N, rows, cols, seed=0):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    x = torch.empty((N, cols, rows), dtype=torch.float64, device="cuda")
+    x[0] = torch.randn((cols, rows), generator=g, dtype=torch.float64, device="cuda")
+    for l in range(1, N):
+        u = torch.rand((cols, rows), generator=g, dtype=torch.float64, device="cuda") * 2 - 1
+        x[l] = x[l - 1] * 2.0**-53 * u
+    return x
+
+
+def spd_dev(N, n, seed=1):
+    R = rand_dev(N, n, n, seed)
+    S = torch.zeros_like(R)
+    M.gemm_(S, R, R.transpose(1, 2).contiguous())
+    S[0].diagonal().add_(float(n))
+    return S
+
+
+def _time(fn, reps=1, warm=1):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps / 1e3
+
+
+def _compute(name, N, macs, t, instr=None, **extra):
+    instr = instr or INSTR_N[N]
+    r = {"case": name, "limbs": N, "ms": round(t * 1e3, 3), "macs": macs,
+         "tflops": round(macs * F_N[N] / t / 1e12, 3), "frac": round(macs * F_N[N] / t / 1e12 / PEAK_TF, 4),
+         "issue_frac": round(macs * instr / t / PEAK_INSTR, 4)}
+    r.update(extra)
+    return r
+
+
+def _memory(name, N, bytes_, macs, t, **extra):
+    r = {"case": name, "limbs": N, "ms": round(t * 1e3, 3), "gbs": round(bytes_ / t / 1e9, 1),
+         "hbm_frac": round(bytes_ / t / PEAK_HBM, 4), "tflops": round(macs * F_N[N] / t / 1e12, 3),
+         "issue_frac": round(macs * INSTR_N[N] / t / PEAK_INSTR, 4)}
+    r.update(extra)
+    return r
+
+
+def case_gemm(N, m, k, n, reps=2):
+    A, B = rand_dev(N, m, k, 1), rand_dev(N, k, n, 2)
+    C = torch.zeros((N, n, m), dtype=torch.float64, device="cuda")
+    t = _time(lambda: M.gemm_(C, A, B), reps)
+    instr = INSTR_X3_FUSED if N == 3 else None
+    return _compute(f"gemm {m}x{k}x{n}", N, m * k * n, t, instr)
+
+
+def case_syrk(N, n, k, reps=2):
+    P = rand_dev(N, k, n, 3)
+    C = torch.zeros((N, n, n), dtype=torch.float64, device="cuda")
+    t = _time(lambda: M.syrk_(C, P), reps)
+    return _compute(f"syrk n={n} k={k}", N, k * n * (n + 1) // 2, t)
+
+
+def case_trsm(N, n, nrhs, order="blocked", reps=1):
+    F = M.cholesky_(spd_dev(N, n, 4), config=M.KernelConfig())
+    B0 = rand_dev(N, n, nrhs, 5)
+    B = B0.clone()
+    cfg = M.KernelConfig(trsm_order=order)
+
+    def run():
+        B.copy_(B0)
+        M.trsm_(B, F.factors, config=cfg)
+    t = _time(run, reps)
+    return _compute(f"trsm lower n={n} nrhs={nrhs} ({order})", N, nrhs * n * (n - 1) // 2, t)
+
+
+def case_factor(kind, N, n):
+    A0 = spd_dev(N, n, 6) if kind != "lu" else rand_dev(N, n, n, 7)
+    A = A0.clone()
+    small = A0[:, :64, :64].contiguous()
+    fn = {"cholesky": M.cholesky_, "lu": M.lu_, "ldlt": M.ldlt_}[kind]
+    fn(small.clone(), check=False)           # kernel loading
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn(A, check=False)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    macs = {"cholesky": n**3 // 6, "lu": n**3 // 3, "ldlt": n**3 // 6}[kind]
+    return _compute(f"{kind} n={n}", N, macs, t)
+
+
+def case_gemv(N, n, reps=5):
+    A = rand_dev(N, n, n, 8)
+    x = rand_dev(N, n, 1, 9).reshape(N, n).contiguous()
+    y = torch.zeros((N, n), dtype=torch.float64, device="cuda")
+    t = _time(lambda: M.gemv_(y, A, x), reps)
+    return _memory(f"gemv n={n}", N, 8 * N * (n * n + 2 * n), n * n, t)
+
+
+def case_trsv(N, n, reps=2):
+    F = M.cholesky_(spd_dev(N, n, 10))
+    x0 = rand_dev(N, n, 1, 11).reshape(N, n).contiguous()
+    x = x0.clone()
+
+    def run():
+        x.copy_(x0)
+        M.trsv_(x, F.factors)
+    t = _time(run, reps)
+    return _memory(f"trsv n={n}", N, 8 * N * (n * (n + 1) // 2 + 2 * n), n * (n - 1) // 2, t)
+
+
+def case_residual(N, n, nrhs, reps=2):
+    A = rand_dev(N, n, n, 12)
+    X, B = rand_dev(N, n, nrhs, 13), rand_dev(N, n, nrhs, 14)
+    R = torch.zeros_like(B)
+    t = _time(lambda: M.residual_(R, A, X, B), reps)
+    return _memory(f"residual n={n} nrhs={nrhs}", N, 8 * N * (n * n + 3 * n * nrhs), n * n * nrhs, t)
+
+
+def case_batched(count=10000, nmin=16, nmax=256, N=3, nrhs=4, seed=3):
+    """Config 3: uneven SPD / KKT x3 systems in one persistent launch."""
+    from .batched import Batch, synthetic_systems
+    systems = synthetic_systems(count, nmin, nmax, N, nrhs, seed)
+    b = Batch(systems, N)
+    b.solve()                                      # warm-up
+    b.restore()
+    g = b.graph()
+    torch.cuda.synchronize()
+    b.restore()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.replay()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    info = [r["info"] for r in b.results()]
+    macs = 0
+    for kind, A, B in systems:
+        n = A.shape[1]
+        macs += n**3 // 6 + 2 * n * n * B.shape[1] // 2
+    r = _compute(f"batched {count} x{N} systems n~U{{{nmin}..{nmax}}}", N, macs, t)
+    r["systems_per_s"] = round(count / t, 1)
+    r["failed"] = int(sum(1 for i in info if i != 0))
+    return r
+
+
+def case_autodiff(batch=32, n=1024, nrhs=32, N=4):
+    """Config 4: x = S^{-1} B and logdet(S) forward + pullbacks over a batch."""
+    from . import autodiff as AD
+    Ss = [spd_dev(N, n, 100 + i) for i in range(batch)]
+    Bs = [rand_dev(N, n, nrhs, 200 + i) for i in range(batch)]
+    W = rand_dev(N, n, nrhs, 300)
+    one = np.zeros(N)
+    one[0] = 1.0
+
+    def run():
+        for S, B in zip(Ss, Bs):
+            F = M.cholesky_(S.clone())
+            X = M.solve(F, B)
+            AD.logdet(F)
+            AD.solve_adjoint(F, X, W)
+            AD.logdet_adjoint(F, one)
+    t = _time(run, 1, warm=0)
+    return {"case": f"autodiff batch={batch} n={n} nrhs={nrhs}", "limbs": N, "ms": round(t * 1e3, 1),
+            "ms_per_problem": round(t * 1e3 / batch, 2)}
+
+
+def run(which=("blas3", "factor", "blas2", "refine", "batched", "autodiff"), out=None):
+    rows = []
+
+    def emit(r):
+        rows.append(r)
+        line = json.dumps(r)
+        print(line, flush=True)
+        if out:
+            with open(out, "a") as f:
+                f.write(line + "\n")
+
+    if "blas3" in which:
+        for N in (2, 3, 4):
+            emit(case_gemm(N, 2048, 2048, 2048))
+            emit(case_gemm(N, 4096, 4096, 4096, reps=1))
+            emit(case_gemm(N, 8192, 512, 8192, reps=1))
+            emit(case_syrk(N, 4096, 1024, reps=1))
+            emit(case_trsm(N, 4096, 256))
+    if "factor" in which:
+        for kind in ("cholesky", "lu", "ldlt"):
+            emit(case_factor(kind, 4, 4096 if kind != "ldlt" else 2048))
+    if "blas2" in which:
+        for N in (2, 3, 4):
+            emit(case_gemv(N, 4096))
+            emit(case_trsv(N, 4096))
+            emit(case_residual(N, 4096, 64))
+    if "refine" in which:
+        from .refine import bench_refine
+        for kind in ("cholesky", "lu"):
+            r = bench_refine(kind=kind)
+            r["case"] = f"refine {kind} n=4096 nrhs=64"
+            emit(r)
+    if "batched" in which:
+        emit(case_batched())
+    if "autodiff" in which:
+        emit(case_autodiff())
+    return rows

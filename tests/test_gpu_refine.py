@@ -42,3 +42,132 @@ def test_refined_solve_reaches_x4_backward_error(oracle, kind):
     Xp = R.plain_solve(A, B, kind).cpu().numpy()
     bp = exact_backward_error(oracle, Ah, Xp, Bh)
     assert bp.max() <= 1e-60, bp
+
+
+def _np(t):
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (64, 64, 64), (100, 37, 130), (257, 70, 65)])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_dgemm_matches_float64_reference(m, n, k, ta, tb):
+    from mfla import api as M
+
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = rng.standard_normal((k, m) if ta == "N" else (m, k))     # planar [col, row]
+    B = rng.standard_normal((n, k) if tb == "N" else (k, n))
+    C0 = rng.standard_normal((n, m))
+    opA = A.T if ta == "N" else A                                 # rows x cols views
+    opB = B.T if tb == "N" else B
+    for alpha, beta in ((1.0, 0.0), (-0.5, 2.0)):
+        C = torch.from_numpy(C0.copy()[None]).cuda()
+        M.dgemm_(C, torch.from_numpy(A[None]).cuda(), torch.from_numpy(B[None]).cuda(), alpha, beta, trans_a=ta, trans_b=tb)
+        ref = alpha * opA @ opB + beta * C0.T
+        err = np.abs(_np(C)[0].T - ref).max() / max(1.0, np.abs(ref).max())
+        assert err <= 4 * k * 2.0**-52, (m, n, k, ta, tb, err)
+
+
+@pytest.mark.parametrize("tri_a,tri_b", [("lower", "general"), ("upper", "general"), ("general", "lower"), ("general", "upper")])
+def test_dgemm_triangular_operands_ignore_the_other_triangle(tri_a, tri_b):
+    from mfla import api as M
+
+    rng = np.random.default_rng(11)
+    n, r = 150, 40
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((r, n)) if tri_b == "general" else rng.standard_normal((n, n))
+    Bm = B.T
+    Am = A.T
+    if tri_a != "general":
+        keep = np.tril(np.ones((n, n))) if tri_a == "lower" else np.triu(np.ones((n, n)))
+        Aref = Am * keep
+        Am_poison = np.where(keep > 0, Am, np.nan)
+        A = Am_poison.T.copy()
+        Bref = Bm
+        nout = r
+    else:
+        keep = np.tril(np.ones((n, n))) if tri_b == "lower" else np.triu(np.ones((n, n)))
+        Aref = Am
+        Bref = Bm * keep
+        B = np.where(keep > 0, Bm, np.nan).T.copy()
+        nout = n
+    C = torch.zeros((1, nout, n), dtype=torch.float64, device="cuda")
+    M.dgemm_(C, torch.from_numpy(A[None]).cuda(), torch.from_numpy(B[None]).cuda(), tri_a=tri_a, tri_b=tri_b)
+    ref = Aref @ Bref
+    assert np.isfinite(_np(C)).all()
+    assert np.abs(_np(C)[0].T - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 200, 300])
+def test_dpotrf_inverse(n):
+    from mfla import api as M
+
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n))
+    S = G @ G.T + n * np.eye(n)
+    Sp = np.where(np.tril(np.ones((n, n))) > 0, S, np.nan)     # the upper triangle is never read
+    A = torch.from_numpy(Sp.T.copy()[None]).cuda()
+    Inv = torch.full((1, n, n), np.nan, dtype=torch.float64, device="cuda")
+    assert M.dpotrf_inverse_(A, Inv) == 0
+    L = np.tril(_np(A)[0].T)
+    Li = np.tril(_np(Inv)[0].T)
+    Lref = np.linalg.cholesky(S)
+    assert np.abs(L - Lref).max() <= 1e-13 * np.abs(Lref).max() * max(1, n ** 0.5)
+    assert np.abs(Li @ L - np.eye(n)).max() <= 1e-13 * n
+
+
+def test_dpotrf_inverse_reports_failing_pivot():
+    from mfla import api as M
+
+    n = 100
+    S = np.eye(n) * 4.0
+    S[70, 70] = -1.0
+    A = torch.from_numpy(S.T.copy()[None]).cuda()
+    Inv = torch.zeros_like(A)
+    assert M.dpotrf_inverse_(A, Inv) == 71
+    S[3, 3] = np.inf
+    A = torch.from_numpy(S.T.copy()[None]).cuda()
+    assert M.dpotrf_inverse_(A, Inv) == -1
+
+
+@pytest.mark.parametrize("kind,uplo,n,nrhs", [("cholesky", "general", 256, 4), ("cholesky", "lower", 200, 3),
+                                              ("lu", "general", 192, 5)])
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_native_refinement_reaches_working_precision(oracle, kind, uplo, n, nrhs, N):
+    from mfla import refine as R
+
+    A = R.spd_kappa(n, 1e8, N=N) if kind == "cholesky" else R.general_kappa(n, 1e8, N=N)
+    if uplo == "lower":                     # poison the triangle that must not be read
+        iu = torch.triu_indices(n, n, 1, device="cuda")
+        A = A.clone()
+        A[:, iu[1], iu[0]] = float("nan")  # planar [col, row]: col > row is the upper triangle
+        A = A.contiguous()
+    rng = np.random.default_rng(5 + N)
+    Bh = np.zeros((N, nrhs, n))
+    Bh[0] = rng.standard_normal((nrhs, n))
+    for l in range(1, N):
+        Bh[l] = Bh[l - 1] * 2.0**-53 * rng.uniform(-1, 1, (nrhs, n))
+    B = torch.from_numpy(Bh).cuda()
+    X, hist = R.refine_solve(A, B, kind, uplo=uplo)
+    assert 2 <= len(hist) <= 12 and hist[-1] <= 64 * 2.0 ** (-52 * N), hist
+    Ah = A.cpu().numpy()
+    if uplo == "lower":
+        il = np.triu_indices(n, 1)
+        Ah[:, il[1], il[0]] = Ah[:, il[0], il[1]]      # symmetric from the lower triangle
+    be = exact_backward_error(oracle, Ah, X.cpu().numpy(), Bh)
+    assert be.max() <= 2.0 ** (-52 * N + 8), (be.max(), hist)
+
+
+def test_native_refinement_errors():
+    from mfla import api as M
+    from mfla import refine as R
+
+    n = 64
+    A = torch.zeros((4, n, n), dtype=torch.float64, device="cuda")
+    A[0] = -torch.eye(n, dtype=torch.float64, device="cuda")
+    B = torch.ones((4, 2, n), dtype=torch.float64, device="cuda")
+    with pytest.raises(M.PosDefException):
+        R.refine_solve(A, B, "cholesky")
+    with pytest.raises(M.DimensionMismatch):
+        R.refine_solve(A, B[:, :, :10].contiguous(), "cholesky")
+    with pytest.raises(M.ArgumentError):
+        R.refine_solve(A[:1].contiguous(), B[:1].contiguous(), "cholesky")
