@@ -47,6 +47,7 @@ extern "C" {
 #define MFLA_ERR_HIP (-4)
 #define MFLA_ERR_FAILED_FACTOR (-5)
 #define MFLA_ERR_WORKSPACE (-6)
+#define MFLA_ERR_BOUNDS (-7)      /* BoundsError (a reduction range outside the panel) */
 
 /* side / uplo / trans / diag symbols (MFLA's :left/:right, :lower/:upper,
  * :N/:T, :nonunit/:unit). MFLA_GENERAL is residual!'s uplo=:general. */
@@ -248,6 +249,22 @@ int mfla_batched_solve(mfla_handle h, int limbs, const mfla_batch_entry *entries
                        const mfla_config *config);
 int mfla_batched_max_order(void);
 int mfla_batched_max_rhs(void);
+
+/* ------------------------------------------- API completion (BLAS-3 extras) */
+/* trmm! (src/kernels/trmm.jl:75-131): B = alpha*op(A)*B (side LEFT) or
+ * B = alpha*B*op(A) (RIGHT) in place, MFLA's per-element order (ascending
+ * inner index, diagonal term first or last as in trmm.jl:1-73). Only the
+ * selected triangle of A is used. */
+int mfla_trmm(mfla_handle h, int limbs, int side, int uplo, int trans, int diag, mfla_matrix B, mfla_matrix A,
+              const double *alpha, const mfla_config *config);
+/* mfdot (src/kernels/dot.jl:1-22): sum of x[i]*y[i] over the column-major
+ * linear index, one ascending chain; out gets the limbs (host). */
+int mfla_dot(mfla_handle h, int limbs, mfla_matrix x, mfla_matrix y, double *out);
+/* syrk_packed! (src/kernels/syrk.jl:181-254): packed (a limbs x n(n+1)/2
+ * column vector holding the column-major lower triangle) = alpha *
+ * P[rf:rl, :]^T P[rf:rl, :] + beta * packed; rf, rl one-based inclusive. */
+int mfla_syrk_packed(mfla_handle h, int limbs, mfla_matrix packed, mfla_matrix P, const double *alpha, const double *beta,
+                     int64_t reduction_first, int64_t reduction_last, const mfla_config *config);
 
 /* ------------------------------------------ FP64 factor + Float64xN refinement */
 /* MFLA supplies the refinement pieces -- residual! (src/residual.jl:55-125),

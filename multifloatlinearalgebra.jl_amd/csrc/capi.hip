@@ -19,6 +19,7 @@
 #include "k_blas3.hpp"
 #include "k_f64.hpp"
 #include "k_factor.hpp"
+#include "k_level3x.hpp"
 #include "k_refine.hpp"
 #include "k_trsm.hpp"
 #include "mf_arith.hpp"
@@ -1397,6 +1398,78 @@ int mfla_cholesky_adjoint(mfla_handle h, int N, mfla_matrix F, int64_t info, mfl
     HIPCHK(trsm_core(h, N, MFLA_LEFT, MFLA_LOWER, MFLA_TRANS, MFLA_NONUNIT, Sbar, F, kOne, c, nullptr));     // L^{-T} *
     const double half[4] = {0.5, 0.0, 0.0, 0.0};
     HIPCHK(mfk::symmetrize(N, s, n, dvn(Sbar), dvn(Sbar), half));
+    return MFLA_OK;
+}
+
+}  // extern "C"
+
+// ======================================================================
+// trmm!, mfdot, syrk_packed!
+extern "C" {
+
+int mfla_trmm(mfla_handle h, int N, int side, int uplo, int trans, int diag, mfla_matrix B, mfla_matrix A, const double *alpha,
+              const mfla_config *) {
+    if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "trmm! requires a square triangular matrix");
+    if (check_tr_symbols(side, uplo, trans, diag)) return MFLA_ERR_ARGUMENT;
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A") || check_desc(B, "B")) return MFLA_ERR_ARGUMENT;
+    if (side == MFLA_LEFT && B.rows != A.rows) return fail(MFLA_ERR_DIMENSION, "trmm! left dimensions differ");
+    if (side == MFLA_RIGHT && B.cols != A.rows) return fail(MFLA_ERR_DIMENSION, "trmm! right dimensions differ");
+    if (extents_overlap(B, A, N)) return fail(MFLA_ERR_ALIAS, "trmm! does not support output/input aliasing");
+    const int64_t m = B.rows, n = B.cols, na = A.rows;
+    if (m * n == 0) return MFLA_OK;
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    // the products read B's original values: work from a copy
+    double *bc = (double *)scratch(h, sizeof(double) * (size_t)N * m * n);
+    if (!bc) return fail(MFLA_ERR_HIP, "scratch allocation failed");
+    const DV Bc{bc, 1, m, m * n};
+    HIPCHK(mfk::copy(N, s, m, n, Bc, dvn(B)));
+    const bool transposed = trans == MFLA_TRANS;
+    const bool eff_lower = ((uplo == MFLA_LOWER) == !transposed);
+    const DV opA = transposed ? dvt(A) : dvn(A);
+    const int left = side == MFLA_LEFT;
+    HIPCHK(mfk::trmm(N, s, left, eff_lower ? 1 : 0, diag == MFLA_UNIT, m, n, na, left ? opA : Bc, left ? Bc : opA, dvn(B),
+                     alpha ? alpha : kOne));
+    return MFLA_OK;
+}
+
+int mfla_dot(mfla_handle h, int N, mfla_matrix x, mfla_matrix y, double *out) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(x, "x") || check_desc(y, "y")) return MFLA_ERR_ARGUMENT;
+    if (x.rows != y.rows || x.cols != y.cols) return fail(MFLA_ERR_DIMENSION, "dot product shapes differ");
+    std::lock_guard<std::mutex> g(h->lock);
+    double *buf = (double *)scratch(h, sizeof(double) * N);
+    if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
+    HIPCHK(mfk::dot(N, h->stream, x.rows, x.cols, dvn(x), dvn(y), buf));
+    HIPCHK(hipMemcpyAsync(out, buf, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return MFLA_OK;
+}
+
+int mfla_syrk_packed(mfla_handle h, int N, mfla_matrix packed, mfla_matrix P, const double *alpha, const double *beta,
+                     int64_t reduction_first, int64_t reduction_last, const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(packed, "packed") || check_desc(P, "panel")) return MFLA_ERR_ARGUMENT;
+    const int64_t n = P.cols;
+    if (packed.cols != 1 || packed.rows != n * (n + 1) / 2) return fail(MFLA_ERR_DIMENSION, "syrk_packed! output length differs");
+    if (!(1 <= reduction_first && reduction_first <= reduction_last && reduction_last <= P.rows))
+        return fail(MFLA_ERR_BOUNDS, "reduction range %lld:%lld out of bounds for %lld rows", (long long)reduction_first,
+                    (long long)reduction_last, (long long)P.rows);
+    if (extents_overlap(packed, P, N)) return fail(MFLA_ERR_ALIAS, "syrk_packed! does not support output/input aliasing");
+    if (n == 0) return MFLA_OK;
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    double *dense = (double *)scratch(h, sizeof(double) * (size_t)N * n * n);
+    if (!dense) return fail(MFLA_ERR_HIP, "scratch allocation failed");
+    const DV D{dense, 1, n, n * n};
+    const double *be = beta ? beta : kZero;
+    if (!scalar_is_zero(be, N)) HIPCHK(mfk::pack_lower(N, s, n, D, packed.data, packed.limb_stride, 1));
+    const mfla_matrix Pr{P.data + (reduction_first - 1), reduction_last - reduction_first + 1, n, P.ld, P.limb_stride};
+    HIPCHK(lower_launch(h, N, c.gemm_tile, n, Pr.rows, dvt(Pr), dvn(Pr), D, alpha ? alpha : kOne, be, nullptr));
+    HIPCHK(mfk::pack_lower(N, s, n, D, packed.data, packed.limb_stride, 0));
     return MFLA_OK;
 }
 

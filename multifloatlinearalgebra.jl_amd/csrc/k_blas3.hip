@@ -15,8 +15,9 @@
 // the tiling. There is no split-K anywhere.
 //
 // Layout: a 256-thread workgroup computes a (16*TM) x (16*TN) tile; A and B
-// k-panels of depth BK are staged through LDS as [k][row][limb] so a thread's
-// TM (TN) consecutive elements are one contiguous LDS read. Operands are
+// k-panels of depth BK are staged through LDS. B is [k][col][limb] (a wave
+// reads 4 distinct columns: broadcasts); A is stored in 16-byte pieces so the
+// 16 row-threads of a wave read 256 contiguous bytes per piece. Operands are
 // strided views, so transposed operands (syrk's P^T P, gemmt's L R^T) need no
 // copies.
 #include "mf_common.hpp"
@@ -34,6 +35,19 @@ __global__ __launch_bounds__(256) void k_tile(int64_t n, int64_t K, DV A, DV B, 
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
+    // LDS slot of limb l of A(ii, kk). With TM * N even, the TM * N doubles a
+    // thread reads per k are stored as 16-byte pieces, piece-major over the
+    // 16 row-threads: the 16 distinct threads of a wave read one piece as 256
+    // contiguous bytes (no bank conflicts); otherwise [k][row][limb].
+    auto a_slot = [](int kk, int ii, int l) -> int {
+        if constexpr ((TM * N) % 2 == 0) {
+            constexpr int P = TM * N / 2;
+            const int owner = ii / TM, d = (ii % TM) * N + l;
+            return ((kk * P + (d >> 1)) * 16 + owner) * 2 + (d & 1);
+        } else {
+            return (kk * BM + ii) * N + l;
+        }
+    };
     const int64_t i0 = row_lo + (int64_t)blockIdx.x * BM;
     const int64_t j0 = (int64_t)blockIdx.y * BN;
     if (LOWER && i0 + BM - 1 < j0) return;  // tile strictly above the diagonal
@@ -59,15 +73,10 @@ __global__ __launch_bounds__(256) void k_tile(int64_t n, int64_t K, DV A, DV B, 
             int ii, kk;
             if (a_rowfast) { ii = e % BM; kk = e / BM; } else { kk = e % BK; ii = e / BK; }
             const int64_t gi = i0 + ii, gk = k0 + kk;
-            double *dst = &As[(kk * BM + ii) * N];
-            if (gi < row_hi && gk < K) {
-                const double *src = A.p + gi * A.rs + gk * A.cs;
+            const bool ok = gi < row_hi && gk < K;
+            const double *src = A.p + gi * A.rs + gk * A.cs;
 #pragma unroll
-                for (int l = 0; l < N; ++l) dst[l] = src[l * A.ls];
-            } else {
-#pragma unroll
-                for (int l = 0; l < N; ++l) dst[l] = 0.0;
-            }
+            for (int l = 0; l < N; ++l) As[a_slot(kk, ii, l)] = ok ? src[l * A.ls] : 0.0;
         }
         // stage B panel: BK x BN elements
         for (int e = tid; e < BN * BK; e += 256) {
@@ -91,7 +100,7 @@ __global__ __launch_bounds__(256) void k_tile(int64_t n, int64_t K, DV A, DV B, 
 #pragma unroll
             for (int a = 0; a < TM; ++a)
 #pragma unroll
-                for (int l = 0; l < N; ++l) av[a].x[l] = As[(kk * BM + tx * TM + a) * N + l];
+                for (int l = 0; l < N; ++l) av[a].x[l] = As[a_slot(kk, tx * TM + a, l)];
 #pragma unroll
             for (int b = 0; b < TN; ++b)
 #pragma unroll

@@ -19,6 +19,7 @@ MFLA_ERR_ALIAS = -3
 MFLA_ERR_HIP = -4
 MFLA_ERR_FAILED_FACTOR = -5
 MFLA_ERR_WORKSPACE = -6
+MFLA_ERR_BOUNDS = -7
 
 
 class MFMatrix(ctypes.Structure):
@@ -101,6 +102,9 @@ PROTOTYPES = {
                                ctypes.POINTER(_I), _DP, _DP]),
     "mfla_dgemm": (_I, [_P, _I, _I, _I, _I, _M, _M, _M, ctypes.c_double, ctypes.c_double]),
     "mfla_dpotrf_inverse": (_I, [_P, _M, _M, ctypes.POINTER(_I64)]),
+    "mfla_trmm": (_I, [_P, _I, _I, _I, _I, _I, _M, _M, _DP, _CP]),
+    "mfla_dot": (_I, [_P, _I, _M, _M, _DP]),
+    "mfla_syrk_packed": (_I, [_P, _I, _M, _M, _DP, _DP, _I64, _I64, _CP]),
     "mfla_logdet": (_I, [_P, _I, _M, _I64, _DP]),
     "mfla_solve_adjoint": (_I, [_P, _I, _M, _I64, _M, _M, _M, _M, _CP]),
     "mfla_logdet_adjoint": (_I, [_P, _I, _M, _I64, _DP, _M, _CP]),

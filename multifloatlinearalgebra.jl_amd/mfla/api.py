@@ -36,6 +36,10 @@ class DomainError(ValueError):
     pass
 
 
+class BoundsError(IndexError):
+    pass
+
+
 class PosDefException(Exception):
     def __init__(self, info):
         super().__init__(f"PosDefException({info})")
@@ -56,6 +60,8 @@ def _raise(rc: int):
         raise DimensionMismatch(msg)
     if rc in (L.MFLA_ERR_ARGUMENT, L.MFLA_ERR_ALIAS):
         raise ArgumentError(msg)
+    if rc == L.MFLA_ERR_BOUNDS:
+        raise BoundsError(msg)
     if rc == L.MFLA_ERR_FAILED_FACTOR:
         info = int(msg[msg.find("(") + 1: msg.find(")")]) if "(" in msg else 0
         if msg.startswith("PosDef"):
@@ -221,6 +227,63 @@ def trsm_(B, A, alpha=1.0, *, side="left", uplo="lower", trans="N", diag="nonuni
     d = _sym(diag, {"nonunit": 0, "unit": 1}, "diag must be :unit or :nonunit")
     _raise(L.load().mfla_trsm(_handle(), N, s, u, t, d, _mat(B), _mat(A), _sp(_scalar(alpha, N)), _cfg(config)))
     return B
+
+
+def trmm_(B, A, alpha=1.0, *, side="left", uplo="lower", trans="N", diag="nonunit", config=None):
+    """trmm! (src/kernels/trmm.jl:75-131): B = alpha*op(A)*B or alpha*B*op(A)."""
+    N = _limbs(B, A)
+    sd = _sym(side, {"left": 0, "right": 1}, "side must be :left or :right")
+    up = _sym(uplo, {"lower": 0, "upper": 1}, "uplo must be :lower or :upper")
+    tr = _sym(trans, {"N": 0, "T": 1}, "trans must be :N or :T")
+    dg = _sym(diag, {"nonunit": 0, "unit": 1}, "diag must be :unit or :nonunit")
+    _raise(L.load().mfla_trmm(_handle(), N, sd, up, tr, dg, _mat(B), _mat(A), _sp(_scalar(alpha, N)), _cfg(config)))
+    return B
+
+
+def mfdot(x, y):
+    """mfdot (src/kernels/dot.jl:1-22): one ascending chain over the
+    column-major linear index. Returns the N limbs (numpy)."""
+    N = _limbs(x, y)
+    if x.shape != y.shape:
+        raise DimensionMismatch("dot product shapes differ")
+    out = np.zeros(N)
+    _raise(L.load().mfla_dot(_handle(), N, _mat(x), _mat(y), _sp(out)))
+    return out
+
+
+def syrk_packed_(packed, P, alpha=1.0, beta=0.0, *, reduction_first=1, reduction_last=None, config=None):
+    """syrk_packed! (src/kernels/syrk.jl:181-254): packed is an (N, n(n+1)/2)
+    tensor holding the column-major lower triangle."""
+    N = _limbs(packed, P)
+    rl = P.shape[2] if reduction_last is None else reduction_last
+    _raise(L.load().mfla_syrk_packed(_handle(), N, _mat(packed), _mat(P), _sp(_scalar(alpha, N)), _sp(_scalar(beta, N)),
+                                     int(reduction_first), int(rl), _cfg(config)))
+    return packed
+
+
+def capabilities(N):
+    """capabilities(Float64xN) (src/capabilities.jl:15-74) for this backend:
+    the same facts with provider "mfla_hip", plus the device extensions."""
+    supported = 1 <= N <= 4
+    targets = {"x2": False, "x3": supported and N == 2, "x4": supported and N in (2, 3)}
+    ops = ("dot", "gemv", "transpose_gemv", "symv", "gemm", "gemmt", "syrk", "syrk_packed", "trsv", "trsm", "trmm",
+           "cholesky", "lu", "ldlt", "ldlt_lightweight_metadata", "factor_diagnostics", "vector_rhs", "multi_rhs",
+           "residual", "refinement_correction", "reusable_workspace")
+    caps = {"provider": "mfla_hip", "scalar_type": f"Float64x{N}", "base_type": "Float64", "supported": supported,
+            "limb_count": N}
+    caps.update({op: supported for op in ops})
+    caps.update({
+        "rrqr": False, "apply_q": False, "solve_r": False,
+        "mixed_precision_residual": any(targets.values()), "mixed_residual_targets": targets,
+        "mixed_residual_target_types": tuple(f"Float64x{k}" for k in (3, 4) if targets[f"x{k}"]),
+        "factor_metadata_ownership": "factor_owned", "factor_matrix_ownership": "borrowed_input",
+        "factorization_destructive": True, "factor_solve_mutates_factor": False,
+        "shared_gemm_workspace_concurrency": "serialized_safe", "concurrent_factor_workspace": False,
+        "syrk_authoritative_triangle": "lower", "syrk_inactive_triangle": "preserved",
+        "threading": False, "stream_ordered": supported,
+        "batched_solve": supported, "binary64_refinement": supported and N >= 2, "reverse_mode_rules": supported,
+    })
+    return caps
 
 
 # ----------------------------------------------------------------- BLAS 2

@@ -156,6 +156,82 @@ void trsm_t(int side_right, int upper, int trans, int unit, int64_t m, int64_t n
 // ----------------------------------------------------------------- GEMV
 // src/kernels/gemv.jl:1-93: y = alpha*op(A)*x + beta*y, ascending reduction,
 // beta == 0 => y unread.
+// ------------------------------------------------------------------ TRMM
+// src/kernels/trmm.jl:1-73. Per element acc = 0 and, with "+=" meaning
+// acc = acc + x*y: left lower k = 0..i-1 then the diagonal term; left upper
+// the diagonal term then k = i+1..m-1; right lower the diagonal then
+// k = j+1..n-1; right upper k = 0..j-1 then the diagonal. Terms are coeff*B
+// (left) or B*coeff (right); a unit diagonal adds B(i, j). Result alpha*acc.
+// The in-place orders (rows descending for left lower, columns descending for
+// right upper) keep every consumed input unmodified.
+template <int N>
+void trmm_t(int side_right, int upper, int trans, int unit, int64_t m, int64_t n, View<N> A, View<N> B, MF<N> alpha) {
+    const bool transposed = trans != 0;
+    const bool eff_lower = ((!upper) == (!transposed));
+    auto coeff = [&](int64_t row, int64_t col) { return transposed ? A.get(col, row) : A.get(row, col); };
+    if (!side_right) {
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t t = 0; t < m; ++t) {
+                const int64_t i = eff_lower ? m - 1 - t : t;
+                MF<N> acc = mf_zero<N>();
+                auto diag = [&]() { acc = unit ? add(acc, B.get(i, j)) : add(acc, mul(A.get(i, i), B.get(i, j))); };
+                if (eff_lower) {
+                    for (int64_t k = 0; k < i; ++k) acc = add(acc, mul(coeff(i, k), B.get(k, j)));
+                    diag();
+                } else {
+                    diag();
+                    for (int64_t k = i + 1; k < m; ++k) acc = add(acc, mul(coeff(i, k), B.get(k, j)));
+                }
+                B.set(i, j, mul(alpha, acc));
+            }
+    } else {
+        for (int64_t i = 0; i < m; ++i)
+            for (int64_t t = 0; t < n; ++t) {
+                const int64_t j = eff_lower ? t : n - 1 - t;
+                MF<N> acc = mf_zero<N>();
+                auto diag = [&]() { acc = unit ? add(acc, B.get(i, j)) : add(acc, mul(B.get(i, j), A.get(j, j))); };
+                if (eff_lower) {
+                    diag();
+                    for (int64_t k = j + 1; k < n; ++k) acc = add(acc, mul(B.get(i, k), coeff(k, j)));
+                } else {
+                    for (int64_t k = 0; k < j; ++k) acc = add(acc, mul(B.get(i, k), coeff(k, j)));
+                    diag();
+                }
+                B.set(i, j, mul(alpha, acc));
+            }
+    }
+}
+
+// ------------------------------------------------------------------ dot
+// src/kernels/dot.jl:8-22: acc = acc + x[i]*y[i] over the column-major
+// linear index, ascending.
+template <int N> MF<N> dot_t(int64_t m, int64_t n, View<N> x, View<N> y) {
+    MF<N> acc = mf_zero<N>();
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < m; ++i) acc = add(acc, mul(x.get(i, j), y.get(i, j)));
+    return acc;
+}
+
+// ----------------------------------------------------------- packed SYRK
+// src/kernels/syrk.jl:110-254: packed column-major lower triangle,
+// (r, c) -> c(2n - c + 1)/2 + (r - c) (0-based); per element acc = 0,
+// acc = acc + P[k, r]*P[k, c] over k = rf..rl ascending, then alpha*acc
+// (beta == 0, output unread) or alpha*acc + beta*out.
+template <int N>
+void syrk_packed_t(int64_t n, int64_t rf, int64_t rl, View<N> P, double *out, int64_t ls, MF<N> alpha, MF<N> beta) {
+    const bool bz = iszero(beta);
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t r = c; r < n; ++r) {
+            MF<N> acc = mf_zero<N>();
+            for (int64_t k = rf; k <= rl; ++k) acc = add(acc, mul(P.get(k, r), P.get(k, c)));
+            const int64_t idx = c * (2 * n - c + 1) / 2 + (r - c);
+            MF<N> o;
+            for (int l = 0; l < N; ++l) o.x[l] = out[l * ls + idx];
+            const MF<N> v = bz ? mul(alpha, acc) : add(mul(alpha, acc), mul(beta, o));
+            for (int l = 0; l < N; ++l) out[l * ls + idx] = v.x[l];
+        }
+}
+
 template <int N>
 void gemv_t(int trans, int64_t m, int64_t n, View<N> A, View<N> x, View<N> y, MF<N> alpha, MF<N> beta) {
     const bool overwrite = iszero(beta);
@@ -729,6 +805,23 @@ int orc_lower_update(int N, int64_t n, int64_t k, double *L, int64_t ldl, int64_
 int orc_trsm(int N, int side_right, int upper, int trans, int unit, int64_t m, int64_t n, double *A, int64_t lda, int64_t lsa,
              double *B, int64_t ldb, int64_t lsb, const double *alpha, int threads) {
     DISPATCH(N, trsm_t<NN>(side_right, upper, trans, unit, m, n, View<NN>{A, lda, lsa}, View<NN>{B, ldb, lsb}, scal<NN>(alpha), threads));
+    return 0;
+}
+
+int orc_trmm(int N, int side_right, int upper, int trans, int unit, int64_t m, int64_t n, double *A, int64_t lda, int64_t lsa,
+             double *B, int64_t ldb, int64_t lsb, const double *alpha) {
+    DISPATCH(N, trmm_t<NN>(side_right, upper, trans, unit, m, n, View<NN>{A, lda, lsa}, View<NN>{B, ldb, lsb}, scal<NN>(alpha)));
+    return 0;
+}
+
+int orc_dot(int N, int64_t m, int64_t n, double *x, int64_t ldx, int64_t lsx, double *y, int64_t ldy, int64_t lsy, double *out) {
+    DISPATCH(N, { const MF<NN> r = dot_t<NN>(m, n, View<NN>{x, ldx, lsx}, View<NN>{y, ldy, lsy}); for (int l = 0; l < NN; ++l) out[l] = r.x[l]; });
+    return 0;
+}
+
+int orc_syrk_packed(int N, int64_t n, int64_t rf, int64_t rl, double *P, int64_t ldp, int64_t lsp, double *out, int64_t ls,
+                    const double *alpha, const double *beta) {
+    DISPATCH(N, syrk_packed_t<NN>(n, rf, rl, View<NN>{P, ldp, lsp}, out, ls, scal<NN>(alpha), scal<NN>(beta)));
     return 0;
 }
 

@@ -163,6 +163,33 @@ def trsm(B, A, alpha=1.0, side="left", uplo="lower", trans="N", diag="nonunit", 
     return B
 
 
+def trmm(B, A, alpha=1.0, side="left", uplo="lower", trans="N", diag="nonunit"):
+    N, n_b, m_b = B.shape
+    _check(lib().orc_trmm(c_int(N), c_int(side == "right"), c_int(uplo == "upper"), c_int(trans == "T"), c_int(diag == "unit"),
+                          c_i64(m_b), c_i64(n_b), *_mat(A), *_mat(B), _ptr(_scal(alpha, N))))
+    return B
+
+
+def dot(x, y):
+    """mfdot over planar (N, cols, rows) or (N, n) arrays; returns N limbs."""
+    x3 = x.reshape(x.shape[0], 1, -1) if x.ndim == 2 else x
+    y3 = y.reshape(y.shape[0], 1, -1) if y.ndim == 2 else y
+    N, cols, rows = x3.shape
+    out = np.zeros(N)
+    _check(lib().orc_dot(c_int(N), c_i64(rows), c_i64(cols), *_mat(np.ascontiguousarray(x3)), *_mat(np.ascontiguousarray(y3)),
+                         _ptr(out)))
+    return out
+
+
+def syrk_packed(out, P, alpha=1.0, beta=0.0, reduction_first=1, reduction_last=None):
+    """out: planar (N, n(n+1)/2) packed lower triangle; P planar (N, n, k)."""
+    N, n, k = P.shape
+    rl = k if reduction_last is None else reduction_last
+    _check(lib().orc_syrk_packed(c_int(N), c_i64(n), c_i64(reduction_first - 1), c_i64(rl - 1), *_mat(P), _ptr(out),
+                                 c_i64(out.shape[1]), _ptr(_scal(alpha, N)), _ptr(_scal(beta, N))))
+    return out
+
+
 def trsv(x, A, alpha=1.0, uplo="lower", trans="N", diag="nonunit"):
     N, n = x.shape
     x3 = x.reshape(N, 1, n)
