@@ -83,8 +83,9 @@ def bench_gemm(args, M, torch):
     Ad = torch.from_numpy(A).cuda()
     Bd = torch.from_numpy(B).cuda()
     Cd = torch.zeros((N, n, n), dtype=torch.float64, device="cuda")
+    cfg = M.KernelConfig(gemm_tile=args.tile)
     for _ in range(args.warmup):
-        M.gemm_(Cd, Ad, Bd)
+        M.gemm_(Cd, Ad, Bd, config=cfg)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -92,7 +93,7 @@ def bench_gemm(args, M, torch):
     e0.record(stream)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        M.gemm_(Cd, Ad, Bd)
+        M.gemm_(Cd, Ad, Bd, config=cfg)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -111,6 +112,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--limbs", type=int, default=4)
+    ap.add_argument("--tile", type=int, default=-1, help="GEMM output tile: -1 auto, 0 16x16, 1 32x32, 2 32x64")
     ap.add_argument("--cpu-cols", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -974,7 +974,7 @@ hipError_t dtrtri_core(hipStream_t s, int64_t n, int64_t nb, const DV &T, const 
 }
 
 struct RefineWs {
-    double *F, *Inv, *InvU, *Y, *D, *R, *P, *rows, *stats;
+    double *F, *Inv, *InvU, *Y, *D, *R, *P, *rows, *part, *stats;
     int64_t *perm, *ipiv;
     int64_t ksplit;
 };
@@ -1005,6 +1005,7 @@ int64_t refine_layout(int N, int kind, int64_t n, int64_t nrhs, double *base, Re
     w.R = take((int64_t)N * n * nrhs);
     w.P = ks > 1 ? take(ks * n * nrhs) : nullptr;
     w.rows = take(n);
+    w.part = take((int64_t)mfk::drow_abs_chunks() * n);
     w.stats = take(4 * nrhs + 8);
     w.perm = (int64_t *)take(n);
     w.ipiv = (int64_t *)take(n);
@@ -1168,7 +1169,7 @@ int mfla_refine_solve(mfla_handle h, int N, int kind, int uplo, mfla_matrix A, m
     }
     // ---- norms for the stopping test (binary64 estimates)
     std::vector<double> hrows(n), hb(nrhs), hx(nrhs, 0.0), hr(nrhs), hd(nrhs);
-    HIPCHK(mfk::drow_abs_sum(s, n, n, Ahi, kernel_uplo(uplo), w.rows));
+    HIPCHK(mfk::drow_abs_sum(s, n, n, Ahi, kernel_uplo(uplo), w.rows, w.part));
     HIPCHK(mfk::col_absmax_hi(s, n, nrhs, DV{B.data, 1, B.ld, B.limb_stride}, bmax));
     HIPCHK(hipMemcpyAsync(hrows.data(), w.rows, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(hb.data(), bmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));

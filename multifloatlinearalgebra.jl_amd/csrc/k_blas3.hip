@@ -26,7 +26,7 @@
 namespace mfk {
 
 template <int N, int NET, int TM, int TN, int BK, int MODE, bool LOWER>
-__global__ __launch_bounds__(256) void k_tile(int64_t n, int64_t K, DV A, DV B, DV C, S<N> alpha, S<N> beta,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_tile(int64_t n, int64_t K, DV A, DV B, DV C, S<N> alpha, S<N> beta,
                                               int beta_zero, int64_t row_lo, int64_t row_hi, const Status *status) {
     if (status && status->info != 0) return;
     constexpr int BM = 16 * TM, BN = 16 * TN;
@@ -146,8 +146,11 @@ static hipError_t go(hipStream_t s, int64_t rows, int64_t n, int64_t K, const DV
 template <int N, int NET, int MODE, bool LOWER>
 static hipError_t pick(hipStream_t s, int tile, int64_t rows, int64_t n, int64_t K, const DV &A, const DV &B, const DV &C,
                        const S<N> &al, const S<N> &be, int bz, int64_t row_lo, const Status *st) {
-    // tile 1: 32x32 tiles (2x2 per thread); tile 0: 16x16 tiles (1x1 per thread)
-    // for small outputs where 32x32 tiles would leave most CUs idle.
+    // tile 1: 32x32 tiles (2x2 per thread); tile 2: 32x64 tiles (2x4 per
+    // thread, eight independent MAC chains per thread); tile 0: 16x16 tiles
+    // (1x1 per thread) for small outputs where larger tiles would leave most
+    // CUs idle.
+    if (tile == 2) return go<N, NET, 2, 4, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st);
     if (tile == 1) return go<N, NET, 2, 2, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st);
     return go<N, NET, 1, 1, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st);
 }

@@ -24,10 +24,14 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int DG_T = 64, DG_BK = 16, DG_PAD = 1;
 
-// v_mfma_f64_16x16x4_f64 operand layout (lane l of the wave):
-//   A fragment: A[l % 16][l / 16]   (16 x 4)
-//   B fragment: B[l / 16][l % 16]   (4 x 16)
-//   D item v:   D[4 * (l / 16) + v][l % 16]
+// v_mfma_f64_16x16x4_f64 operand layout (lane l of the wave), as measured
+// on gfx950 with integer-valued operands:
+//   first operand:  P[l % 16][l / 16]   (16 x 4)
+//   second operand: Q[l / 16][l % 16]   (4 x 16)
+//   D item v:       D[l / 16 + 4 * v][l % 16]   (D = P Q)
+// The kernel forms the transposed tile D = B^T A^T (P = B^T, Q = A^T), so
+// lane l holds C rows i = l % 16: the 16 lanes of a row group store 16
+// consecutive elements of a column-major C column.
 __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
     __shared__ double As[DG_BK][DG_T + DG_PAD];
     __shared__ double Bs[DG_BK][DG_T + DG_PAD];
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
 #pragma unroll
             for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+                for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[bj], a[bi], acc[bi][bj], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -109,8 +113,8 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t i = i0 + wy * 32 + bi * 16 + 4 * (lane >> 4) + v;
-                const int64_t j = j0 + wx * 32 + bj * 16 + (lane & 15);
+                const int64_t i = i0 + wy * 32 + bi * 16 + (lane & 15);
+                const int64_t j = j0 + wx * 32 + bj * 16 + (lane >> 4) + 4 * v;
                 if (i >= g.m || j >= g.n) continue;
                 if (g.c_lower && i < j) continue;
                 const double r = acc[bi][bj][v];
@@ -138,59 +142,78 @@ __global__ void k_dgemm_reduce(DGemmArgs g) {
 // ------------------------------------------------ diagonal block Cholesky
 // Factor the nb x nb block at (j0, j0) of F (lower triangle read), write L
 // back to F's lower triangle and inv(L) to Inv's lower triangle at (j0, j0).
-// A non-positive or non-finite pivot at block column c records info =
+// The block is padded to 64 x 64 with an identity. Factorization: right-
+// looking with deferred scaling (a_rq -= a_rc a_qc / d_c, one barrier per
+// column), then L = column c / sqrt(d_c). Inverse: recursive doubling over
+// block sizes 1, 2, 4, .., 32, each level two small triangular products
+// (W = L21 X11 kept in the strict upper triangle, X21 = -X22 W). A
+// non-positive or non-finite pivot at block column c records info =
 // j0 + c + 1 and stops the factorization.
-__global__ __launch_bounds__(256) void k_dpotrf_diag(DV F, DV Inv, int64_t j0, int nb, Status *st) {
+constexpr int DP_LD = 65, DP_THREADS = 1024;
+
+__global__ __launch_bounds__(DP_THREADS) void k_dpotrf_diag(DV F, DV Inv, int64_t j0, int nb, Status *st) {
     if (st->info != 0) return;
-    __shared__ double L[64 * 64];   // column-major: L[q * 64 + r]
-    __shared__ double X[64 * 64];   // row-major:    X[r * 64 + c]
-    __shared__ int fail_s;
+    __shared__ double L[64 * DP_LD];   // column-major: L[q * DP_LD + r]
+    __shared__ double X[64 * DP_LD];   // row-major:    X[r * DP_LD + c]
+    __shared__ double sd[64];
     const int t = threadIdx.x;
-    for (int e = t; e < 64 * 64; e += 256) {
+    for (int e = t; e < 64 * 64; e += DP_THREADS) {
         const int r = e & 63, q = e >> 6;
-        L[e] = (r < nb && q < nb && r >= q) ? F.p[(j0 + r) * F.rs + (j0 + q) * F.cs] : 0.0;
+        double v = (r == q) ? 1.0 : 0.0;
+        if (r < nb && q < nb && r >= q) v = F.p[(j0 + r) * F.rs + (j0 + q) * F.cs];
+        L[q * DP_LD + r] = v;
     }
-    if (t == 0) fail_s = 0;
     __syncthreads();
     for (int c = 0; c < nb; ++c) {
-        if (t == 0) {
-            const double d = L[c * 64 + c];
-            if (!(d > 0.0) || !__builtin_isfinite(d)) {
-                fail_s = 1;
-                st->info = j0 + c + 1;
-            } else {
-                L[c * 64 + c] = __builtin_sqrt(d);
-            }
+        const double d = L[c * DP_LD + c];
+        if (!(d > 0.0) || !__builtin_isfinite(d)) {      // uniform: every thread reads the same pivot
+            if (t == 0) st->info = j0 + c + 1;
+            return;
         }
-        __syncthreads();
-        if (fail_s) return;
-        const double p = L[c * 64 + c];
-        if (t > c && t < nb) L[c * 64 + t] /= p;
-        __syncthreads();
+        const double inv_d = 1.0 / d;
         const int w = nb - c - 1;
-        for (int e = t; e < w * w; e += 256) {
+        for (int e = t; e < w * w; e += DP_THREADS) {
             const int r = c + 1 + e % w, q = c + 1 + e / w;
-            if (r >= q) L[q * 64 + r] -= L[c * 64 + r] * L[c * 64 + q];
+            if (r >= q) L[q * DP_LD + r] -= L[c * DP_LD + r] * L[c * DP_LD + q] * inv_d;
         }
         __syncthreads();
     }
-    for (int e = t; e < 64 * 64; e += 256) {
+    if (t < 64) sd[t] = __builtin_sqrt(L[t * DP_LD + t]);
+    __syncthreads();
+    for (int e = t; e < 64 * 64; e += DP_THREADS) {
         const int r = e & 63, q = e >> 6;
-        if (r < nb && q < nb && r >= q) F.p[(j0 + r) * F.rs + (j0 + q) * F.cs] = L[e];
-    }
-    // inv(L) by forward substitution against the identity, one column per thread
-    if (t < nb) {
-        const int c = t;
-        for (int r = c; r < nb; ++r) {
-            double s = (r == c) ? 1.0 : 0.0;
-            for (int k = c; k < r; ++k) s -= L[k * 64 + r] * X[k * 64 + c];
-            X[r * 64 + c] = s / L[r * 64 + r];
-        }
+        if (r > q) L[q * DP_LD + r] /= sd[q];
     }
     __syncthreads();
-    for (int e = t; e < 64 * 64; e += 256) {
+    if (t < 64) {
+        L[t * DP_LD + t] = sd[t];
+        X[t * DP_LD + t] = 1.0 / sd[t];
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * 64; e += DP_THREADS) {
         const int r = e & 63, q = e >> 6;
-        if (r < nb && q < nb && r >= q) Inv.p[(j0 + r) * Inv.rs + (j0 + q) * Inv.cs] = X[r * 64 + q];
+        if (r < nb && q < nb && r >= q) F.p[(j0 + r) * F.rs + (j0 + q) * F.cs] = L[q * DP_LD + r];
+    }
+    for (int s = 1; s < 64; s *= 2) {
+        const int pairs = 64 / (2 * s);
+        for (int e = t; e < pairs * s * s; e += DP_THREADS) {       // W = L21 X11 (X11 lower: k >= j)
+            const int p = e / (s * s), i = (e / s) % s, j = e % s, b = p * 2 * s;
+            double acc = 0.0;
+            for (int k = j; k < s; ++k) acc += L[(b + k) * DP_LD + b + s + i] * X[(b + k) * DP_LD + b + j];
+            X[(b + j) * DP_LD + b + s + i] = acc;
+        }
+        __syncthreads();
+        for (int e = t; e < pairs * s * s; e += DP_THREADS) {       // X21 = -X22 W (X22 lower: k <= i)
+            const int p = e / (s * s), i = (e / s) % s, j = e % s, b = p * 2 * s;
+            double acc = 0.0;
+            for (int k = 0; k <= i; ++k) acc += X[(b + s + i) * DP_LD + b + s + k] * X[(b + j) * DP_LD + b + s + k];
+            X[(b + s + i) * DP_LD + b + j] = -acc;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < 64 * 64; e += DP_THREADS) {
+        const int r = e & 63, q = e >> 6;
+        if (r < nb && q < nb && r >= q) Inv.p[(j0 + r) * Inv.rs + (j0 + q) * Inv.cs] = X[r * DP_LD + q];
     }
 }
 
@@ -207,17 +230,30 @@ __global__ void k_dcopy(int64_t m, int64_t n, DV dst, DV src, int mirror) {
     dst.p[i * dst.rs + j * dst.cs] = src.p[si * src.rs + sj * src.cs];
 }
 
-// out[i] = sum_j |A(i, j)| (binary64 plane; uplo 1/2 reads a stored triangle
-// symmetrically).
-__global__ void k_drow_abs_sum(int64_t m, int64_t n, DV A, int uplo, double *out) {
+// part[c * m + i] = sum over column chunk c of |A(i, j)| (binary64 plane;
+// uplo 1 / 2 reads a stored triangle symmetrically); k_drow_abs_finish adds
+// the chunks of each row in ascending order.
+constexpr int ROW_CHUNKS = 32;
+
+__global__ void k_drow_abs_sum(int64_t m, int64_t n, DV A, int uplo, double *part) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
+    const int64_t c = blockIdx.y, per = (n + ROW_CHUNKS - 1) / ROW_CHUNKS;
+    const int64_t jlo = c * per, jhi = (jlo + per < n) ? jlo + per : n;
     double s = 0.0;
-    for (int64_t j = 0; j < n; ++j) {
+    for (int64_t j = jlo; j < jhi; ++j) {
         int64_t a = i, b = j;
         if ((uplo == 1 && a < b) || (uplo == 2 && a > b)) { a = j; b = i; }
         s += fabs(A.p[a * A.rs + b * A.cs]);
     }
+    part[c * m + i] = s;
+}
+
+__global__ void k_drow_abs_finish(int64_t m, const double *part, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int c = 0; c < ROW_CHUNKS; ++c) s += part[c * m + i];
     out[i] = s;
 }
 
@@ -245,7 +281,7 @@ hipError_t dgemm(hipStream_t s, DGemmArgs g, int64_t batch) {
 
 hipError_t dpotrf_diag(hipStream_t s, const DV &F, const DV &Inv, int64_t j0, int nb, Status *st) {
     if (nb < 1 || nb > 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_dpotrf_diag, dim3(1), dim3(256), 0, s, F, Inv, j0, nb, st);
+    hipLaunchKernelGGL(k_dpotrf_diag, dim3(1), dim3(DP_THREADS), 0, s, F, Inv, j0, nb, st);
     return hipGetLastError();
 }
 
@@ -255,11 +291,16 @@ hipError_t dcopy(hipStream_t s, int64_t m, int64_t n, const DV &dst, const DV &s
     return hipGetLastError();
 }
 
-hipError_t drow_abs_sum(hipStream_t s, int64_t m, int64_t n, const DV &A, int uplo, double *out) {
+hipError_t drow_abs_sum(hipStream_t s, int64_t m, int64_t n, const DV &A, int uplo, double *out, double *part) {
     if (m == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_drow_abs_sum, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m, n, A, uplo, out);
+    hipLaunchKernelGGL(k_drow_abs_sum, dim3((unsigned)cdiv(m, 256), ROW_CHUNKS), dim3(256), 0, s, m, n, A, uplo, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_drow_abs_finish, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m, part, out);
     return hipGetLastError();
 }
+
+int drow_abs_chunks() { return ROW_CHUNKS; }
 
 hipError_t dgather_rows(hipStream_t s, int64_t m, int64_t ncols, const DV &dst, const DV &src, const int64_t *perm) {
     if (m * ncols == 0) return hipSuccess;

@@ -28,7 +28,9 @@ hipError_t dgemm(hipStream_t s, DGemmArgs g, int64_t batch);
 // inverse of its factor into Inv at (j0, j0).
 hipError_t dpotrf_diag(hipStream_t s, const DV &F, const DV &Inv, int64_t j0, int nb, Status *st);
 hipError_t dcopy(hipStream_t s, int64_t m, int64_t n, const DV &dst, const DV &src, int mirror);
-hipError_t drow_abs_sum(hipStream_t s, int64_t m, int64_t n, const DV &A, int uplo, double *out);
+// out[i] = sum_j |A(i, j)|; part: drow_abs_chunks() * m doubles of scratch.
+hipError_t drow_abs_sum(hipStream_t s, int64_t m, int64_t n, const DV &A, int uplo, double *out, double *part);
+int drow_abs_chunks();
 hipError_t dgather_rows(hipStream_t s, int64_t m, int64_t ncols, const DV &dst, const DV &src, const int64_t *perm);
 
 }  // namespace mfk

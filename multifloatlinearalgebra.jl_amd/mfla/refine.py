@@ -135,13 +135,15 @@ def _refine_workspace(N, kind, n, nrhs):
 def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general", stages=False):
     """Solve A X = B (A, B Float64xN on the device, N = 2..4) with a binary64
     factor and Float64xN refinement, in the library's native driver
-    (mfla_refine_solve). Returns (X, history) -- history holds the binary64
-    backward-error estimate after each correction -- and with stages=True
+    (mfla_refine_solve). The default tolerance 2^-(52N+8) stops at the
+    Float64xN residual floor (the plain Float64xN solve's backward error).
+    Returns (X, history) -- history holds the binary64 backward-error
+    estimate after each correction -- and with stages=True
     also milliseconds per stage (factor, solve, update, residual, check)."""
     import ctypes
     from . import _lib as L
     N = M._limbs(A, B)
-    tol = tol if tol is not None else 64.0 * 2.0 ** (-52 * N)
+    tol = tol if tol is not None else 2.0 ** (-52 * N - 8)
     n, nrhs = A.shape[2], B.shape[1]
     X = torch.empty_like(B)
     work, length = _refine_workspace(N, kind, n, nrhs)
