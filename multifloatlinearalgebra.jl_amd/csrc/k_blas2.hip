@@ -109,19 +109,38 @@ __global__ void k_arith(int op, int64_t count, const double *x, const double *y,
 }
 
 // --------------------------------------------------------------- GEMV
-// src/kernels/gemv.jl:1-93: one thread per output, ascending reduction.
+// src/kernels/gemv.jl:1-93: one thread per output, ascending reduction. The
+// reduction is one serial chain per output (MFLA's order), so the kernel is
+// bound by the chain's latency, not by HBM: 64-thread workgroups spread the
+// outputs over as many CUs as possible, and a ring of GV_PF prefetched
+// (A, x) pairs keeps the loads GV_PF steps ahead of the chain.
+constexpr int GV_PF = 8;
+
 template <int N>
-__global__ __launch_bounds__(256) void k_gemv(int trans, int64_t outs, int64_t red, DV A, DV x, DV y, S<N> alpha, S<N> beta,
-                                              int beta_zero, const Status *st) {
+__global__ __launch_bounds__(64) void k_gemv(int trans, int64_t outs, int64_t red, DV A, DV x, DV y, S<N> alpha, S<N> beta,
+                                             int beta_zero, const Status *st) {
     if (st && st->info != 0) return;
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= outs) return;
+    const DV Ao = trans ? DV{A.p + o * A.cs, A.cs, A.rs, A.ls} : DV{A.p + o * A.rs, A.rs, A.cs, A.ls};   // Ao(0, k)
     mf::V<N> acc = mf::zero<N>();
-    if (!trans) {
-        for (int64_t k = 0; k < red; ++k) acc = mf::mac(acc, ld<N>(A, o, k), ld<N>(x, k, 0));
-    } else {
-        for (int64_t k = 0; k < red; ++k) acc = mf::mac(acc, ld<N>(A, k, o), ld<N>(x, k, 0));
+    mf::V<N> ra[GV_PF], rx[GV_PF];
+#pragma unroll
+    for (int q = 0; q < GV_PF; ++q) {
+        if (q < red) { ra[q] = ld<N>(Ao, 0, q); rx[q] = ld<N>(x, q, 0); }
     }
+    int64_t k = 0;
+    for (; k + GV_PF <= red; k += GV_PF) {
+#pragma unroll
+        for (int q = 0; q < GV_PF; ++q) {
+            acc = mf::mac(acc, ra[q], rx[q]);
+            const int64_t kn = k + GV_PF + q;
+            if (kn < red) { ra[q] = ld<N>(Ao, 0, kn); rx[q] = ld<N>(x, kn, 0); }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < GV_PF; ++q)
+        if (k + q < red) acc = mf::mac(acc, ra[q], rx[q]);
     mf::V<N> r = beta_zero ? mf::mul(sv(alpha), acc) : mf::add(mf::mul(sv(alpha), acc), mf::mul(sv(beta), ld<N>(y, o, 0)));
     stv<N>(y, o, 0, r);
 }
@@ -309,7 +328,7 @@ hipError_t gemv(int N, hipStream_t s, int trans, int64_t m, int64_t n, const DV 
                 const double *beta, int beta_zero, const Status *st) {
     const int64_t outs = trans ? n : m, red = trans ? m : n;
     if (outs == 0) return hipSuccess;
-    MF_DISPATCH(N, hipLaunchKernelGGL(k_gemv<NN>, dim3(nb256(outs)), dim3(256), 0, s, trans, outs, red, A, x, y, mk<NN>(alpha),
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_gemv<NN>, dim3((unsigned)((outs + 63) / 64)), dim3(64), 0, s, trans, outs, red, A, x, y, mk<NN>(alpha),
                                       mk<NN>(beta), beta_zero, st));
     return hipGetLastError();
 }

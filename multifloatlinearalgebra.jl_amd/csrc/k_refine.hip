@@ -48,50 +48,8 @@ __global__ __launch_bounds__(256) void k_trinv_blocks(int64_t n, int nb, DV T, D
 }
 
 // --------------------------------------------- mixed-precision products
-// a (NA limbs) * d (binary64) to NA-limb accuracy.
-template <int NA> __device__ __forceinline__ mf::V<NA> mul_d(const mf::V<NA> &a, double d);
-template <> __device__ __forceinline__ mf::V<1> mul_d<1>(const mf::V<1> &a, double d) { mf::V<1> r; r.x[0] = a.x[0] * d; return r; }
-template <> __device__ __forceinline__ mf::V<2> mul_d<2>(const mf::V<2> &a, double d) {
-    double p0, e0;
-    mf::two_prod(a.x[0], d, p0, e0);
-    double s1 = a.x[1] * d;
-    s1 += e0;
-    mf::fast_two_sum(p0, s1, p0, s1);
-    mf::V<2> r; r.x[0] = p0; r.x[1] = s1; return r;
-}
-template <> __device__ __forceinline__ mf::V<3> mul_d<3>(const mf::V<3> &a, double d) {
-    double p0, e0, p1, e1, s1, t1;
-    mf::two_prod(a.x[0], d, p0, e0);
-    mf::two_prod(a.x[1], d, p1, e1);
-    double s2 = a.x[2] * d;
-    mf::two_sum(p1, e0, s1, t1);
-    s2 += e1;
-    s2 += t1;
-    mf::fast_two_sum(p0, s1, p0, s1);
-    mf::fast_two_sum(s1, s2, s1, s2);
-    mf::fast_two_sum(p0, s1, p0, s1);
-    mf::V<3> r; r.x[0] = p0; r.x[1] = s1; r.x[2] = s2; return r;
-}
-template <> __device__ __forceinline__ mf::V<4> mul_d<4>(const mf::V<4> &a, double d) {
-    double p0, e0, p1, e1, p2, e2, s1, t1, s2, t2, t3;
-    mf::two_prod(a.x[0], d, p0, e0);
-    mf::two_prod(a.x[1], d, p1, e1);
-    mf::two_prod(a.x[2], d, p2, e2);
-    double s3 = a.x[3] * d;
-    mf::two_sum(p1, e0, s1, t1);
-    mf::two_sum(p2, e1, s2, t2);
-    mf::two_sum(s2, t1, s2, t3);
-    s3 += e2;
-    t2 += t3;
-    s3 += t2;
-    mf::fast_two_sum(p0, s1, p0, s1);
-    mf::fast_two_sum(s1, s2, s1, s2);
-    mf::fast_two_sum(s2, s3, s2, s3);
-    mf::fast_two_sum(p0, s1, p0, s1);
-    mf::fast_two_sum(s1, s2, s1, s2);
-    mf::fast_two_sum(p0, s1, p0, s1);
-    mf::V<4> r; r.x[0] = p0; r.x[1] = s1; r.x[2] = s2; r.x[3] = s3; return r;
-}
+// a (NA limbs) * d (binary64) to NA-limb accuracy (mf::mul_dbl).
+template <int NA> __device__ __forceinline__ mf::V<NA> mul_d(const mf::V<NA> &a, double d) { return mf::mul_dbl<NA>(a, d); }
 
 // R(NR) -= A(top NA limbs) * D(FP64); 16x16 output tile per 256 threads, A and
 // D k-panels staged in LDS, ascending k.

@@ -262,6 +262,56 @@ template <int N> MF_HD V<N> renorm_np1(double *t) {
     if (N > 1) r.x[N - 1] += t[N];
     return r;
 }
+// a (N limbs) * d (binary64) to N-limb accuracy: TwoProd of every limb but
+// the last, the partial products gathered by magnitude level (TwoSum where a
+// level has two terms), then FastTwoSum sweeps.
+template <int N> MF_HD V<N> mul_dbl(const V<N> &a, double d);
+template <> MF_HD V<1> mul_dbl<1>(const V<1> &a, double d) { V<1> r; r.x[0] = a.x[0] * d; return r; }
+template <> MF_HD V<2> mul_dbl<2>(const V<2> &a, double d) {
+    double p0, e0;
+    two_prod(a.x[0], d, p0, e0);
+    double s1 = a.x[1] * d;
+    s1 += e0;
+    fast_two_sum(p0, s1, p0, s1);
+    V<2> r; r.x[0] = p0; r.x[1] = s1; return r;
+}
+template <> MF_HD V<3> mul_dbl<3>(const V<3> &a, double d) {
+    double p0, e0, p1, e1, s1, t1;
+    two_prod(a.x[0], d, p0, e0);
+    two_prod(a.x[1], d, p1, e1);
+    double s2 = a.x[2] * d;
+    two_sum(p1, e0, s1, t1);
+    s2 += e1;
+    s2 += t1;
+    fast_two_sum(p0, s1, p0, s1);
+    fast_two_sum(s1, s2, s1, s2);
+    fast_two_sum(p0, s1, p0, s1);
+    V<3> r; r.x[0] = p0; r.x[1] = s1; r.x[2] = s2; return r;
+}
+template <> MF_HD V<4> mul_dbl<4>(const V<4> &a, double d) {
+    double p0, e0, p1, e1, p2, e2, s1, t1, s2, t2, t3;
+    two_prod(a.x[0], d, p0, e0);
+    two_prod(a.x[1], d, p1, e1);
+    two_prod(a.x[2], d, p2, e2);
+    double s3 = a.x[3] * d;
+    two_sum(p1, e0, s1, t1);
+    two_sum(p2, e1, s2, t2);
+    two_sum(s2, t1, s2, t3);
+    s3 += e2;
+    t2 += t3;
+    s3 += t2;
+    fast_two_sum(p0, s1, p0, s1);
+    fast_two_sum(s1, s2, s1, s2);
+    fast_two_sum(s2, s3, s2, s3);
+    fast_two_sum(p0, s1, p0, s1);
+    fast_two_sum(s1, s2, s1, s2);
+    fast_two_sum(p0, s1, p0, s1);
+    V<4> r; r.x[0] = p0; r.x[1] = s1; r.x[2] = s2; r.x[3] = s3; return r;
+}
+
+// Long division: N+1 partial quotients q_i = r0 / y0, the remainder updated
+// as r - y*q_i with the N-limb x binary64 product, then an error-free
+// renormalisation of the quotient digits.
 template <int N> MF_HD V<N> div(const V<N> &x, const V<N> &y) {
     if constexpr (N == 1) {
         V<1> r; r.x[0] = x.x[0] / y.x[0]; return r;
@@ -270,7 +320,7 @@ template <int N> MF_HD V<N> div(const V<N> &x, const V<N> &y) {
         V<N> r = x;
         for (int i = 0; i <= N; ++i) {
             q[i] = r.x[0] / y.x[0];
-            if (i < N) r = sub(r, mul(y, from<N>(q[i])));
+            if (i < N) r = sub(r, mul_dbl<N>(y, q[i]));
         }
         return renorm_np1<N>(q);
     }

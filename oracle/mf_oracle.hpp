@@ -277,7 +277,8 @@ template <int N> static inline MF<N> mf_min(const MF<N> &x, const MF<N> &y) { re
 
 // ---------------------------------------------------------- division / sqrt
 // Long division: N+1 partial quotients q_i = r0 / y0 with the remainder
-// updated in N-limb arithmetic, then an error-free renormalisation.
+// r - y*q_i updated in N-limb arithmetic (y*q_i by the N-limb x binary64
+// product below), then an error-free renormalisation.
 template <int N> static inline MF<N> renorm_from(const double *q) {
     // q has N+1 terms of decreasing magnitude: bottom-up two_sum, then two
     // top-down fast_two_sum sweeps; the last term is folded into limb N-1.
@@ -291,13 +292,62 @@ template <int N> static inline MF<N> renorm_from(const double *q) {
     if (N > 1) r.x[N - 1] += t[N];
     return r;
 }
+// y (N limbs) times a binary64 q to N-limb accuracy: the exact products of
+// the leading N-1 limbs (TwoProd) and the plain product of the last, summed
+// by magnitude level -- level 1: p1 + e0 (TwoSum), level 2: p2 + e1 + the
+// level-1 error (TwoSum, TwoSum), level 3: p3 + e2 + the level-2 errors
+// (plain) -- then FastTwoSum sweeps (p0,s1),(s1,s2),(s2,s3),(p0,s1),(s1,s2),
+// (p0,s1).
+template <int N> static inline MF<N> mul_by_double(const MF<N> &y, double q) {
+    MF<N> r;
+    if (N == 1) { r.x[0] = y.x[0] * q; return r; }
+    double p0, e0;
+    two_prod(y.x[0], q, p0, e0);
+    if (N == 2) {
+        double s1 = y.x[1] * q;
+        s1 = s1 + e0;
+        fast_two_sum(p0, s1, p0, s1);
+        r.x[0] = p0; r.x[1] = s1;
+        return r;
+    }
+    double p1, e1, s1, t1;
+    two_prod(y.x[1], q, p1, e1);
+    if (N == 3) {
+        double s2 = y.x[2] * q;
+        two_sum(p1, e0, s1, t1);
+        s2 = s2 + e1;
+        s2 = s2 + t1;
+        fast_two_sum(p0, s1, p0, s1);
+        fast_two_sum(s1, s2, s1, s2);
+        fast_two_sum(p0, s1, p0, s1);
+        r.x[0] = p0; r.x[1] = s1; r.x[2] = s2;
+        return r;
+    }
+    double p2, e2, s2, t2, t3;
+    two_prod(y.x[2], q, p2, e2);
+    double s3 = y.x[3] * q;
+    two_sum(p1, e0, s1, t1);
+    two_sum(p2, e1, s2, t2);
+    two_sum(s2, t1, s2, t3);
+    s3 = s3 + e2;
+    t2 = t2 + t3;
+    s3 = s3 + t2;
+    fast_two_sum(p0, s1, p0, s1);
+    fast_two_sum(s1, s2, s1, s2);
+    fast_two_sum(s2, s3, s2, s3);
+    fast_two_sum(p0, s1, p0, s1);
+    fast_two_sum(s1, s2, s1, s2);
+    fast_two_sum(p0, s1, p0, s1);
+    r.x[0] = p0; r.x[1] = s1; r.x[2] = s2; r.x[3] = s3;
+    return r;
+}
 template <int N> static inline MF<N> mf_div(const MF<N> &x, const MF<N> &y) {
     if (N == 1) { MF<N> r; r.x[0] = x.x[0] / y.x[0]; return r; }
     double q[N + 1];
     MF<N> r = x;
     for (int i = 0; i <= N; ++i) {
         q[i] = r.x[0] / y.x[0];
-        if (i < N) r = sub(r, mul(y, mf_from<N>(q[i])));
+        if (i < N) r = sub(r, mul_by_double(y, q[i]));
     }
     return renorm_from<N>(q);
 }
