@@ -35,6 +35,8 @@ struct mfla_handle_s {
     void *scratch = nullptr;    // grows on demand (non-hot paths only)
     size_t scratch_bytes = 0;
     std::mutex lock;            // calls on one handle are serialised
+    hipStream_t side = nullptr; // second stream of the look-ahead factorization
+    hipEvent_t ev_join = nullptr, ev_panel = nullptr, ev_update = nullptr;
 };
 
 struct mfla_workspace_s {
@@ -153,6 +155,16 @@ hipError_t lower_launch(mfla_handle h, int N, int tile, int64_t n, int64_t k, co
     t.mode = mfk::MODE_AB; t.net = mf::NET_STD; t.lower = 1; t.beta_zero = scalar_is_zero(beta, N) ? 1 : 0; t.tile = tile;
     t.row_lo = 0; t.rows = n; t.n = n; t.K = k;
     return mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
+}
+
+// Second stream (default priority) and events for look-ahead factorizations.
+hipError_t ensure_side(mfla_handle h) {
+    if (h->side) return hipSuccess;
+    hipError_t e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
+    if (e != hipSuccess) { h->side = nullptr; return e; }
+    if ((e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
+    return hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming);
 }
 
 // Kernel-side symmetric-access codes: 0 general, 1 lower, 2 upper.
@@ -295,6 +307,13 @@ int mfla_handle_destroy(mfla_handle h) {
     (void)hipFree(h->status);
     (void)hipFree(h->flags);
     if (h->scratch) (void)hipFree(h->scratch);
+    if (h->side) {
+        (void)hipStreamSynchronize(h->side);
+        (void)hipStreamDestroy(h->side);
+        (void)hipEventDestroy(h->ev_join);
+        (void)hipEventDestroy(h->ev_panel);
+        (void)hipEventDestroy(h->ev_update);
+    }
     delete h;
     return MFLA_OK;
 }
@@ -548,20 +567,63 @@ int mfla_potrf(mfla_handle h, int N, mfla_matrix A, int64_t *info, const mfla_co
     HIPCHK(mfk::finite_check(N, s, n, n, dvn(A), 1, st));
     const int64_t block = std::max<int64_t>(c.cholesky_block, 1);
     const DV a = dvn(A);
+    // Right-looking blocked Cholesky (cholesky.jl:68-107) with look-ahead:
+    // after panel j's trsm, the next block column is updated on the main
+    // stream (so panel j+1 can start) while the rest of the trailing SYRK
+    // runs on a second stream. Every element still receives the panel
+    // updates in ascending panel order with the same per-update arithmetic,
+    // so the factor is bitwise identical to the sequential order.
+    const bool lookahead = n > 4 * block;
+    hipStream_t s1 = s;
+    if (lookahead) {
+        HIPCHK(ensure_side(h));
+        s1 = h->side;
+        HIPCHK(hipEventRecord(h->ev_join, s));
+        HIPCHK(hipStreamWaitEvent(s1, h->ev_join, 0));
+    }
     for (int64_t first = 0; first < n; first += block) {
         const int64_t last = std::min(first + block, n) - 1, nb = last - first + 1;
         HIPCHK(mfk::chol_panel(N, s, a, first, last, st));
-        if (last + 1 < n) {
-            const int64_t rest = n - last - 1;
-            // trsm!(A21, L11; side=:right, uplo=:lower, trans=:T) -> forward over panel columns
-            mfk::TrsmArgs ta{};
-            ta.n = nb; ta.nrhs = rest; ta.forward = 1; ta.unit = 0; ta.exact_order = 1;
-            const DV A21 = sub(a, last + 1, first), L11 = sub(a, first, first);
-            HIPCHK(mfk::trsm_solve(N, s, ta, DV{A21.p, A21.cs, A21.rs, A21.ls}, L11, L11, st));
+        if (last + 1 >= n) break;
+        const int64_t rest = n - last - 1;
+        // trsm!(A21, L11; side=:right, uplo=:lower, trans=:T) -> forward over panel columns
+        mfk::TrsmArgs ta{};
+        ta.n = nb; ta.nrhs = rest; ta.forward = 1; ta.unit = 0; ta.exact_order = 1;
+        const DV A21 = sub(a, last + 1, first), L11 = sub(a, first, first);
+        const DV A21t{A21.p, A21.cs, A21.rs, A21.ls};
+        HIPCHK(mfk::trsm_solve(N, s, ta, A21t, L11, L11, st));
+        if (!lookahead) {
             // syrk!(A22, transpose(A21), -1, 1)
-            HIPCHK(lower_launch(h, N, c.gemm_tile, rest, nb, A21, DV{A21.p, A21.cs, A21.rs, A21.ls}, sub(a, last + 1, last + 1),
-                                kMinusOne, kOne, st));
+            HIPCHK(lower_launch(h, N, c.gemm_tile, rest, nb, A21, A21t, sub(a, last + 1, last + 1), kMinusOne, kOne, st));
+            continue;
         }
+        HIPCHK(hipEventRecord(h->ev_panel, s));
+        // next block column (all rows below its diagonal), after the side
+        // stream's update from the previous panel reached it
+        const int64_t nb2 = std::min(block, rest);
+        if (first > 0) HIPCHK(hipStreamWaitEvent(s, h->ev_update, 0));
+        mfk::TileArgs t{};
+        t.mode = mfk::MODE_AB; t.net = mf::NET_STD; t.lower = 1; t.beta_zero = 0; t.tile = c.gemm_tile;
+        t.row_lo = 0; t.rows = rest; t.n = nb2; t.K = nb;
+        HIPCHK(mfk::tile_launch_n(N, s, t, A21, A21t, sub(a, last + 1, last + 1), kMinusOne, kOne, st));
+        if (rest > nb2) {
+            // the remaining trailing matrix on the side stream; it skips its
+            // work only for failures in panels up to this one (a failure in
+            // a later panel happens after this update in MFLA's order)
+            HIPCHK(hipStreamWaitEvent(s1, h->ev_panel, 0));
+            const int64_t rest2 = rest - nb2;
+            const DV B21 = sub(A21, nb2, 0);
+            mfk::TileArgs u{};
+            u.mode = mfk::MODE_AB; u.net = mf::NET_STD; u.lower = 1; u.beta_zero = 0; u.tile = c.gemm_tile;
+            u.row_lo = 0; u.rows = rest2; u.n = rest2; u.K = nb; u.status_limit = last + 1;
+            HIPCHK(mfk::tile_launch_n(N, s1, u, B21, DV{B21.p, B21.cs, B21.rs, B21.ls}, sub(a, last + 1 + nb2, last + 1 + nb2),
+                                      kMinusOne, kOne, st));
+            HIPCHK(hipEventRecord(h->ev_update, s1));
+        }
+    }
+    if (lookahead) {
+        HIPCHK(hipEventRecord(h->ev_join, s1));
+        HIPCHK(hipStreamWaitEvent(s, h->ev_join, 0));
     }
     const int64_t r = read_info(h);
     if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");

@@ -16,6 +16,8 @@ struct TileArgs {
     int64_t rows;      // number of output rows handled (from row_lo)
     int64_t n;         // output columns
     int64_t K;         // reduction length
+    int64_t status_limit;  // 0: return on any recorded failure; > 0: only on
+                           // failures at (one-based) columns <= status_limit
 };
 
 // C(i, j) for i in [row_lo, row_lo + rows), j in [0, n) from a(i, k) = A(i, k)

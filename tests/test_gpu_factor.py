@@ -62,6 +62,25 @@ def kkt(rng, N, m):
     return K
 
 
+@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("bad", [5, 150, 290])
+def test_cholesky_failure_in_a_late_panel_bitwise(M, oracle, N, bad):
+    """A breakdown in panel j leaves the matrix exactly as MFLA's sequential
+    order does: every trailing update of panels < j applied, nothing after
+    (the look-ahead's overlapped updates must neither skip nor add work)."""
+    rng = np.random.default_rng(bad + N)
+    n, block = 300, 16
+    S = spd(oracle, rng, N, n)
+    S[:, bad, bad] = 0.0
+    S[0, bad, bad] = -1.0                       # not positive definite at column `bad`
+    ref = S.copy()
+    info_ref = oracle.potrf(ref, block)
+    Sd = dev(S)
+    F = M.cholesky_(Sd, check=False, config=M.KernelConfig(cholesky_block=block))
+    assert F.info == info_ref and 0 < info_ref <= bad + 1
+    assert_bitwise(host(Sd), ref, f"failed cholesky x{N} at {info_ref}")
+
+
 @pytest.mark.parametrize("N", [2, 3, 4])
 def test_reference_kats_on_gpu(M, N):
     res = run_all(GpuBackend(), N)
@@ -70,7 +89,7 @@ def test_reference_kats_on_gpu(M, N):
 
 
 @pytest.mark.parametrize("N", [1, 2, 3, 4])
-@pytest.mark.parametrize("n,block", [(37, 16), (64, 16), (100, 7), (130, 32)])
+@pytest.mark.parametrize("n,block", [(37, 16), (64, 16), (100, 7), (130, 32), (300, 16)])
 def test_cholesky_bitwise(M, oracle, N, n, block):
     rng = np.random.default_rng(n * 10 + N + block)
     S = spd(oracle, rng, N, n)
