@@ -66,6 +66,12 @@ extern "C" {
 #define MFLA_GEMM_DIRECT 1
 #define MFLA_GEMM_PACKED 2
 #define MFLA_GEMM_FUSED 3
+/* Ozaki-split product on the INT8 matrix cores (no MFLA counterpart): both
+ * operands sliced into 7-bit digits, digit products summed exactly in int32,
+ * recombined in Float64xN. Normwise accuracy ~ K * 2^-(52N) * max|A(i,:)|
+ * * max|B(:,j)| per element (not MFLA's rounding order: parity is within
+ * that tolerance, not bitwise). Reduction length <= 8192; gemm! only. */
+#define MFLA_GEMM_OZAKI 4
 /* ldlt_strategy (src/factorizations/ldlt.jl:329-350) */
 #define MFLA_LDLT_AUTO 0
 #define MFLA_LDLT_UNBLOCKED 1
@@ -285,6 +291,9 @@ int mfla_syrk_packed(mfla_handle h, int limbs, mfla_matrix packed, mfla_matrix P
  * failing pivot); a failed factorization returns MFLA_ERR_FAILED_FACTOR. */
 #define MFLA_REFINE_CHOLESKY 0
 #define MFLA_REFINE_LU 1
+/* OR into kind: residual updates R -= A*D as Ozaki-split INT8 MFMA products
+ * (A split once, n <= 8192) instead of the limb-adaptive multi-double sums. */
+#define MFLA_REFINE_OZAKI 16
 int mfla_refine_workspace_size(int limbs, int kind, int64_t n, int64_t nrhs, int64_t *doubles);
 int mfla_refine_solve(mfla_handle h, int limbs, int kind, int uplo, mfla_matrix A, mfla_matrix B, mfla_matrix X, double *work,
                       int64_t work_len, double tol, int max_iter, int64_t *info, int *iterations, double *history,

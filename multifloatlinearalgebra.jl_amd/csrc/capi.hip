@@ -20,6 +20,7 @@
 #include "k_f64.hpp"
 #include "k_factor.hpp"
 #include "k_level3x.hpp"
+#include "k_ozaki.hpp"
 #include "k_refine.hpp"
 #include "k_trsm.hpp"
 #include "mf_arith.hpp"
@@ -165,6 +166,35 @@ hipError_t ensure_side(mfla_handle h) {
     if ((e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
     return hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming);
+}
+
+// Ozaki-split gemm: C = alpha*A*B + beta*C with A (m x k) and B (k x n)
+// sliced into digits on the INT8 matrix cores (scratch from the handle).
+int ozaki_gemm(mfla_handle h, int N, int64_t m, int64_t n, int64_t k, const DV &A, const DV &B, const DV &C, const double *alpha,
+               const double *beta) {
+    const int64_t ld = (k + 63) / 64 * 64;
+    if (ld > 8192) return fail(MFLA_ERR_ARGUMENT, "gemm_strategy=:ozaki supports reduction lengths up to 8192");
+    const int S = mfk::oz_digits(N), U = S;
+    const int64_t mx = std::max(m, n);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    const size_t o_ea = take(sizeof(int) * m), o_eb = take(sizeof(int) * n), o_part = take(sizeof(double) * mfk::oz_chunks() * mx);
+    const size_t o_da = take((size_t)S * m * ld), o_db = take((size_t)S * n * ld), o_pl = take(sizeof(int32_t) * (size_t)(U + 1) * m * n);
+    char *base = (char *)scratch(h, off);
+    if (!base) return fail(MFLA_ERR_HIP, "scratch allocation failed (%zu bytes)", off);
+    int *ea = (int *)(base + o_ea), *eb = (int *)(base + o_eb);
+    double *part = (double *)(base + o_part);
+    int8_t *da = (int8_t *)(base + o_da), *db = (int8_t *)(base + o_db);
+    int32_t *pl = (int32_t *)(base + o_pl);
+    hipStream_t s = h->stream;
+    const DV Bt{B.p, B.cs, B.rs, B.ls};                 // Bt(j, kk) = B(kk, j)
+    HIPCHK(mfk::oz_exponents(s, m, k, A, 0, ea, part));
+    HIPCHK(mfk::oz_exponents(s, n, k, Bt, 0, eb, part));
+    HIPCHK(mfk::oz_split(N, s, m, k, A, 0, ea, S, da, ld, (int64_t)m * ld));
+    HIPCHK(mfk::oz_split(N, s, n, k, Bt, 0, eb, S, db, ld, (int64_t)n * ld));
+    HIPCHK(mfk::oz_gemm(s, m, n, ld, da, ld, (int64_t)m * ld, S, db, ld, (int64_t)n * ld, S, U, pl));
+    HIPCHK(mfk::oz_combine(N, s, m, n, U, pl, ea, eb, C, alpha, beta, scalar_is_zero(beta, N) ? 1 : 0, 0));
+    return MFLA_OK;
 }
 
 // Kernel-side symmetric-access codes: 0 general, 1 lower, 2 upper.
@@ -443,9 +473,12 @@ int mfla_gemm(mfla_handle h, int N, mfla_matrix C, mfla_matrix A, mfla_matrix B,
     if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
     if (check_desc(A, "A") || check_desc(B, "B") || check_desc(C, "C")) return MFLA_ERR_ARGUMENT;
     if (extents_overlap(C, A, N) || extents_overlap(C, B, N)) return fail(MFLA_ERR_ALIAS, "gemm! does not support output/input aliasing");
-    if (c.gemm_strategy < 0 || c.gemm_strategy > 3) return fail(MFLA_ERR_ARGUMENT, "gemm_strategy must be :auto, :direct, :packed, or :fused");
+    if (c.gemm_strategy < 0 || c.gemm_strategy > 4)
+        return fail(MFLA_ERR_ARGUMENT, "gemm_strategy must be :auto, :direct, :packed, :fused, or :ozaki");
     if (c.gemm_strategy == MFLA_GEMM_FUSED && N != 3) return fail(MFLA_ERR_ARGUMENT, "gemm_strategy=:fused is only available for Float64x3");
     std::lock_guard<std::mutex> g(h->lock);
+    if (c.gemm_strategy == MFLA_GEMM_OZAKI && m * n > 0 && k > 0)
+        return ozaki_gemm(h, N, m, n, k, dvn(A), dvn(B), dvn(C), alpha ? alpha : kOne, beta ? beta : kZero);
     HIPCHK(gemm_launch(h, N, c.gemm_strategy, c.gemm_tile, m, n, k, dvn(A), dvn(B), dvn(C), alpha ? alpha : kOne, beta ? beta : kZero,
                        nullptr));
     return MFLA_OK;
@@ -1039,7 +1072,15 @@ struct RefineWs {
     double *F, *Inv, *InvU, *Y, *D, *R, *P, *rows, *part, *stats;
     int64_t *perm, *ipiv;
     int64_t ksplit;
+    // Ozaki residual: digit planes of A (S x n x ld int8), of the correction
+    // (T x nrhs x ld int8), exponents, int32 digit-pair sums
+    int8_t *dA, *dD;
+    int *eA, *eD;
+    int32_t *planes;
+    int64_t ld;
+    int S, T;
 };
+constexpr int kOzCorrectionDigits = 6;
 
 // Split-k factor for the n x nrhs correction products: enough workgroups to
 // cover the chip, at least 256 reduction steps per slice.
@@ -1061,7 +1102,7 @@ int64_t refine_layout(int N, int kind, int64_t n, int64_t nrhs, double *base, Re
     RefineWs w{};
     w.F = take(n * n);
     w.Inv = take(n * n);
-    w.InvU = kind == MFLA_REFINE_LU ? take(n * n) : nullptr;
+    w.InvU = (kind & 15) == MFLA_REFINE_LU ? take(n * n) : nullptr;
     w.Y = take(n * nrhs);
     w.D = take(n * nrhs);
     w.R = take((int64_t)N * n * nrhs);
@@ -1072,6 +1113,16 @@ int64_t refine_layout(int N, int kind, int64_t n, int64_t nrhs, double *base, Re
     w.perm = (int64_t *)take(n);
     w.ipiv = (int64_t *)take(n);
     w.ksplit = ks;
+    if (kind & MFLA_REFINE_OZAKI) {
+        w.ld = (n + 63) / 64 * 64;
+        w.S = mfk::oz_digits(N);
+        w.T = kOzCorrectionDigits;
+        w.dA = (int8_t *)take((w.S * n * w.ld + 7) / 8);
+        w.dD = (int8_t *)take((w.T * nrhs * w.ld + 7) / 8);
+        w.eA = (int *)take((n + 1) / 2);
+        w.eD = (int *)take((nrhs + 1) / 2);
+        w.planes = (int32_t *)take(((int64_t)(w.S + w.T - 1) * n * nrhs + 1) / 2);
+    }
     if (out) *out = w;
     return off;
 }
@@ -1155,17 +1206,21 @@ int mfla_dpotrf_inverse(mfla_handle h, mfla_matrix A, mfla_matrix Inv, int64_t *
 
 int mfla_refine_workspace_size(int limbs, int kind, int64_t n, int64_t nrhs, int64_t *doubles) {
     if (limbs < 2 || limbs > 4) return fail(MFLA_ERR_ARGUMENT, "refinement needs a 2-4 limb working precision");
-    if (kind != MFLA_REFINE_CHOLESKY && kind != MFLA_REFINE_LU) return fail(MFLA_ERR_ARGUMENT, "kind must be cholesky or lu");
+    if ((kind & ~MFLA_REFINE_OZAKI) != MFLA_REFINE_CHOLESKY && (kind & ~MFLA_REFINE_OZAKI) != MFLA_REFINE_LU)
+        return fail(MFLA_ERR_ARGUMENT, "kind must be cholesky or lu");
     if (n < 0 || nrhs < 0) return fail(MFLA_ERR_DIMENSION, "negative dimensions");
     *doubles = refine_layout(limbs, kind, n, nrhs, nullptr, nullptr);
     return MFLA_OK;
 }
 
-int mfla_refine_solve(mfla_handle h, int N, int kind, int uplo, mfla_matrix A, mfla_matrix B, mfla_matrix X, double *work,
+int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matrix A, mfla_matrix B, mfla_matrix X, double *work,
                       int64_t work_len, double tol, int max_iter, int64_t *info, int *iterations, double *history,
                       double *stage_ms) {
+    const bool ozaki = (kind_flags & MFLA_REFINE_OZAKI) != 0;
+    const int kind = kind_flags & ~MFLA_REFINE_OZAKI;
     if (N < 2 || N > 4) return fail(MFLA_ERR_ARGUMENT, "refinement needs a 2-4 limb working precision");
     if (kind != MFLA_REFINE_CHOLESKY && kind != MFLA_REFINE_LU) return fail(MFLA_ERR_ARGUMENT, "kind must be cholesky or lu");
+    if (ozaki && A.rows > 8192) return fail(MFLA_ERR_ARGUMENT, "Ozaki residual updates support n up to 8192");
     if (uplo != MFLA_GENERAL && uplo != MFLA_LOWER && uplo != MFLA_UPPER)
         return fail(MFLA_ERR_ARGUMENT, "uplo must be :general, :lower, or :upper");
     if (kind == MFLA_REFINE_LU && uplo != MFLA_GENERAL) return fail(MFLA_ERR_ARGUMENT, "LU refinement needs a general matrix");
@@ -1176,7 +1231,7 @@ int mfla_refine_solve(mfla_handle h, int N, int kind, int uplo, mfla_matrix A, m
     if (max_iter < 1) return fail(MFLA_ERR_ARGUMENT, "max_iter must be positive");
     if (!info || !iterations || !history) return fail(MFLA_ERR_ARGUMENT, "null output pointer");
     RefineWs w;
-    const int64_t need = refine_layout(N, kind, n, nrhs, work, &w);
+    const int64_t need = refine_layout(N, kind_flags, n, nrhs, work, &w);
     if (work == nullptr || work_len < need) return fail(MFLA_ERR_WORKSPACE, "refinement workspace needs %lld doubles", (long long)need);
     *iterations = 0;
     *info = 0;
@@ -1240,6 +1295,10 @@ int mfla_refine_solve(mfla_handle h, int N, int kind, int uplo, mfla_matrix A, m
     HIPCHK(hipStreamSynchronize(s));
     double anorm = 0.0;
     for (double v : hrows) anorm = std::max(anorm, v);
+    if (ozaki) {   // A's digit planes, once per solve
+        HIPCHK(mfk::oz_exponents(s, n, n, Ahi, kernel_uplo(uplo), w.eA, w.part));
+        HIPCHK(mfk::oz_split(N, s, n, n, Ahi, kernel_uplo(uplo), w.eA, w.S, w.dA, w.ld, n * w.ld));
+    }
     clk.mark(ST_FACTOR);
 
     // ---- refinement loop
@@ -1266,6 +1325,20 @@ int mfla_refine_solve(mfla_handle h, int N, int kind, int uplo, mfla_matrix A, m
         HIPCHK(mfk::dgemm(s, ga, 1));
         HIPCHK(mfk::dgemm(s, gb, 1));
         clk.mark(ST_SOLVE);
+        if (ozaki) {
+            // D rounded to 7T-bit fixed point per column (the value added to X
+            // is the value whose digits are multiplied), R -= A*D exactly up
+            // to A's digit truncation, then summed in N limbs
+            const DV Dt{w.D, n, 1, 0};
+            HIPCHK(mfk::oz_exponents(s, nrhs, n, Dt, 0, w.eD, w.part));
+            HIPCHK(mfk::oz_round_fp64(s, n, nrhs, D, w.eD, w.T, w.dD, w.ld, nrhs * w.ld));
+            HIPCHK(mfk::add_mixed(1, N, s, n, nrhs, dvn(X), D));
+            clk.mark(ST_UPDATE);
+            const int U = w.S + w.T - 2;
+            HIPCHK(mfk::oz_gemm(s, n, nrhs, w.ld, w.dA, w.ld, n * w.ld, w.S, w.dD, w.ld, nrhs * w.ld, w.T, U, w.planes));
+            HIPCHK(mfk::oz_combine(N, s, n, nrhs, U, w.planes, w.eA, w.eD, R, nullptr, nullptr, 0, 1));
+            clk.mark(ST_RESIDUAL);
+        } else {
         // precision of A*D: |D| / |X| = 2^-e needs about (53 N - e) / 52 limbs
         HIPCHK(mfk::col_absmax_hi(s, n, nrhs, D, dmax));
         HIPCHK(hipMemcpyAsync(hd.data(), dmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
@@ -1285,6 +1358,7 @@ int mfla_refine_solve(mfla_handle h, int N, int kind, int uplo, mfla_matrix A, m
         clk.mark(ST_UPDATE);
         HIPCHK(mfk::resid_update(na, N, s, kernel_uplo(uplo), n, nrhs, n, Ahi, D, R));
         clk.mark(ST_RESIDUAL);
+        }
         HIPCHK(mfk::col_absmax_hi(s, n, nrhs, Rhi, rmax));
         HIPCHK(mfk::col_absmax_hi(s, n, nrhs, DV{X.data, 1, X.ld, X.limb_stride}, xmax));
         HIPCHK(hipMemcpyAsync(hr.data(), rmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));

@@ -92,7 +92,7 @@ def synchronize():
 
 
 # ------------------------------------------------------------------ config
-_GEMM = {"auto": 0, "direct": 1, "packed": 2, "fused": 3}
+_GEMM = {"auto": 0, "direct": 1, "packed": 2, "fused": 3, "ozaki": 4}
 _LDLT = {"auto": 0, "unblocked": 1, "blocked": 2}
 _ORDER = {"auto": 0, "exact": 1, "blocked": 2}
 
@@ -119,7 +119,7 @@ class KernelConfig:
 
     def _c(self):
         if self.gemm_strategy not in _GEMM:
-            raise ArgumentError("gemm_strategy must be :auto, :direct, :packed, or :fused")
+            raise ArgumentError("gemm_strategy must be :auto, :direct, :packed, :fused, or :ozaki")
         if self.ldlt_strategy not in _LDLT:
             raise ArgumentError("ldlt_strategy must be :auto, :unblocked, or :blocked")
         c = L.MFConfig()

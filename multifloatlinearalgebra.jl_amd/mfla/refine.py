@@ -119,12 +119,12 @@ _STAGES = ("factor", "solve", "update", "residual", "check")
 _ws_cache: dict = {}
 
 
-def _refine_workspace(N, kind, n, nrhs):
+def _refine_workspace(N, kind, n, nrhs, flag=0):
     import ctypes
     from . import _lib as L
     need = ctypes.c_int64(0)
-    M._raise(L.load().mfla_refine_workspace_size(N, _REFINE_KIND[kind], n, nrhs, ctypes.byref(need)))
-    key = (torch.cuda.current_device(), N, kind)
+    M._raise(L.load().mfla_refine_workspace_size(N, _REFINE_KIND[kind] | flag, n, nrhs, ctypes.byref(need)))
+    key = (torch.cuda.current_device(), N, kind, flag)
     w = _ws_cache.get(key)
     if w is None or w.numel() < need.value:
         w = torch.empty(max(need.value, 1), dtype=torch.float64, device="cuda")
@@ -132,27 +132,32 @@ def _refine_workspace(N, kind, n, nrhs):
     return w, need.value
 
 
-def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general", stages=False):
+def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general", stages=False, residual="limbs"):
     """Solve A X = B (A, B Float64xN on the device, N = 2..4) with a binary64
     factor and Float64xN refinement, in the library's native driver
     (mfla_refine_solve). The default tolerance 2^-(52N+8) stops at the
     Float64xN residual floor (the plain Float64xN solve's backward error).
     Returns (X, history) -- history holds the binary64 backward-error
     estimate after each correction -- and with stages=True
-    also milliseconds per stage (factor, solve, update, residual, check)."""
+    also milliseconds per stage (factor, solve, update, residual, check).
+    residual="limbs" sums R -= A*D in as many limbs as |D|/|X| requires;
+    residual="ozaki" uses the Ozaki-split INT8 matrix-core product."""
     import ctypes
     from . import _lib as L
     N = M._limbs(A, B)
+    if residual not in ("limbs", "ozaki"):
+        raise M.ArgumentError("residual must be :limbs or :ozaki")
+    flag = 16 if residual == "ozaki" else 0
     tol = tol if tol is not None else 2.0 ** (-52 * N - 8)
     n, nrhs = A.shape[2], B.shape[1]
     X = torch.empty_like(B)
-    work, length = _refine_workspace(N, kind, n, nrhs)
+    work, length = _refine_workspace(N, kind, n, nrhs, flag)
     u = M._sym(uplo, M._UPLO_R, "uplo must be :general, :lower, or :upper")
     hist = np.zeros(max_iter)
     st = np.zeros(5)
     info = ctypes.c_int64(0)
     iters = ctypes.c_int(0)
-    rc = L.load().mfla_refine_solve(M._handle(), N, _REFINE_KIND[kind], u, M._mat(A), M._mat(B), M._mat(X),
+    rc = L.load().mfla_refine_solve(M._handle(), N, _REFINE_KIND[kind] | flag, u, M._mat(A), M._mat(B), M._mat(X),
                                     ctypes.c_void_p(work.data_ptr()), length, float(tol), int(max_iter), ctypes.byref(info),
                                     ctypes.byref(iters), M._sp(hist), M._sp(st) if stages else None)
     M._raise(rc)
@@ -208,7 +213,7 @@ def plain_solve(A, B, kind="cholesky", *, block=16):
     return M.solve(F, B, config=M.KernelConfig(trsm_order="blocked"))
 
 
-def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812):
+def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812, residual="limbs"):
     """Times the refined and the plain Float64xN solve on the same matrix and
     right-hand sides; reports times and Float64xN backward errors."""
     A = spd_kappa(n, kappa, N=N, seed=seed) if kind == "cholesky" else general_kappa(n, kappa, N=N, seed=seed)
@@ -218,8 +223,8 @@ def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812
     for l in range(1, N):
         Bh[l] = Bh[l - 1] * 2.0**-53 * rng.uniform(-1, 1, (nrhs, n))
     B = torch.from_numpy(Bh).cuda()
-    out = {"n": n, "nrhs": nrhs, "kind": kind, "kappa": kappa, "limbs": N}
-    for name, fn in (("refined", lambda: refine_solve(A, B, kind)), ("plain", lambda: (plain_solve(A, B, kind), None))):
+    out = {"n": n, "nrhs": nrhs, "kind": kind, "kappa": kappa, "limbs": N, "residual": residual}
+    for name, fn in (("refined", lambda: refine_solve(A, B, kind, residual=residual)), ("plain", lambda: (plain_solve(A, B, kind), None))):
         fn()                                   # warm-up: code-object loading, workspace allocation
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -234,7 +239,7 @@ def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812
             out[name]["iterations"] = len(hist)
             out[name]["history"] = [float(f"{h:.3e}") for h in hist]
     out["speedup"] = round(out["plain"]["seconds"] / out["refined"]["seconds"], 3)
-    _, _, st = refine_solve(A, B, kind, stages=True)
+    _, _, st = refine_solve(A, B, kind, stages=True, residual=residual)
     out["refined"]["stage_ms"] = st
     P = _Timer(True)
     P.mark("start")

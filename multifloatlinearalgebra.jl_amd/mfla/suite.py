@@ -80,10 +80,19 @@ def _memory(name, N, bytes_, macs, t, **extra):
     return r
 
 
-def case_gemm(N, m, k, n, reps=2):
+def case_gemm(N, m, k, n, reps=2, strategy="auto"):
     A, B = rand_dev(N, m, k, 1), rand_dev(N, k, n, 2)
     C = torch.zeros((N, n, m), dtype=torch.float64, device="cuda")
-    t = _time(lambda: M.gemm_(C, A, B), reps)
+    cfg = M.KernelConfig(gemm_strategy=strategy)
+    t = _time(lambda: M.gemm_(C, A, B, config=cfg), reps)
+    if strategy == "ozaki":
+        # INT8 matrix-core work: digit-pair products over K per output
+        S = {1: 11, 2: 19, 3: 27, 4: 34}[N]
+        pairs = (S + 1) * (S + 2) // 2
+        ops = 2.0 * pairs * m * n * ((k + 63) // 64 * 64)
+        return {"case": f"gemm {m}x{k}x{n} (ozaki)", "limbs": N, "ms": round(t * 1e3, 3), "macs": m * k * n,
+                "x_equiv_tflops": round(m * k * n * F_N[N] / t / 1e12, 3), "int8_tops": round(ops / t / 1e12, 1),
+                "int8_frac": round(ops / t / 5.0e15, 4)}
     instr = INSTR_X3_FUSED if N == 3 else None
     return _compute(f"gemm {m}x{k}x{n}", N, m * k * n, t, instr)
 
@@ -229,6 +238,14 @@ def run(which=("blas3", "factor", "blas2", "refine", "batched", "autodiff"), out
         for kind in ("cholesky", "lu"):
             r = bench_refine(kind=kind)
             r["case"] = f"refine {kind} n=4096 nrhs=64"
+            emit(r)
+    if "ozaki" in which:
+        for N in (2, 3, 4):
+            emit(case_gemm(N, 4096, 4096, 4096, reps=2, strategy="ozaki"))
+        from .refine import bench_refine
+        for kind in ("cholesky", "lu"):
+            r = bench_refine(kind=kind, residual="ozaki")
+            r["case"] = f"refine {kind} n=4096 nrhs=64 (ozaki residual)"
             emit(r)
     if "batched" in which:
         emit(case_batched())
