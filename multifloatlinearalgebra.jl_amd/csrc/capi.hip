@@ -172,7 +172,7 @@ hipError_t ensure_side(mfla_handle h) {
 // sliced into digits on the INT8 matrix cores (scratch from the handle).
 int ozaki_gemm(mfla_handle h, int N, int64_t m, int64_t n, int64_t k, const DV &A, const DV &B, const DV &C, const double *alpha,
                const double *beta) {
-    const int64_t ld = (k + 63) / 64 * 64;
+    const int64_t al = mfk::oz_row_align(), ld = (k + al - 1) / al * al;
     if (ld > 8192) return fail(MFLA_ERR_ARGUMENT, "gemm_strategy=:ozaki supports reduction lengths up to 8192");
     const int S = mfk::oz_digits(N), U = S;
     const int64_t mx = std::max(m, n);
@@ -1114,7 +1114,7 @@ int64_t refine_layout(int N, int kind, int64_t n, int64_t nrhs, double *base, Re
     w.ipiv = (int64_t *)take(n);
     w.ksplit = ks;
     if (kind & MFLA_REFINE_OZAKI) {
-        w.ld = (n + 63) / 64 * 64;
+        w.ld = (n + mfk::oz_row_align() - 1) / mfk::oz_row_align() * mfk::oz_row_align();
         w.S = mfk::oz_digits(N);
         w.T = kOzCorrectionDigits;
         w.dA = (int8_t *)take((w.S * n * w.ld + 7) / 8);

@@ -176,57 +176,89 @@ __global__ void k_oz_round_fp64(int64_t K, int64_t ncols, DV D, const int *e, in
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 // out[u * m * n + j * m + i] = sum_{s + t = u, s < SA, t < SB} sum_k dA_s(i,k) dB_t(j,k)
-// K is the (64-multiple) padded length of the digit rows; padding digits are
-// zero.
+// K is the padded length of the digit rows (a multiple of OZ_KS); padding
+// digits are zero. Workgroup tile (32 WM) x (32 WN) from 2 x 2 waves of
+// WM x WN MFMA 16x16x64 blocks; every (plane s, 128-byte k-stage) is one
+// stage of the loop, with the next stage prefetched into registers while
+// the current one is multiplied from LDS. LDS rows are padded to 144 bytes
+// so the 16 rows a lane group reads fall on distinct banks.
+constexpr int OZ_KS = 128, OZ_LDW = 144;
+
+template <int WM, int WN>
 __global__ __launch_bounds__(256) void k_oz_gemm(int64_t m, int64_t n, int64_t K, const int8_t *dA, int64_t lda, int64_t planeA,
-                                                 int SA, const int8_t *dB, int64_t ldb, int64_t planeB, int SB, int u0,
-                                                 int32_t *out, int tiles_m) {
-    __shared__ __align__(16) int8_t As[64 * 64];
-    __shared__ __align__(16) int8_t Bs[64 * 64];
+                                                 int SA, const int8_t *dB, int64_t ldb, int64_t planeB, int SB, int32_t *out,
+                                                 int tiles_m) {
+    constexpr int TM = 32 * WM, TN = 32 * WN;
+    constexpr int CA = TM * (OZ_KS / 16) / 256, CB = TN * (OZ_KS / 16) / 256;   // 16-byte chunks per thread
+    __shared__ __align__(16) int8_t As[TM * OZ_LDW];
+    __shared__ __align__(16) int8_t Bs[TN * OZ_LDW];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wy = w & 1, wx = w >> 1;
-    const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * 64, j0 = (int64_t)(blockIdx.x / tiles_m) * 64;
-    const int u = u0 + (int)blockIdx.z;
+    const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * TM, j0 = (int64_t)(blockIdx.x / tiles_m) * TN;
+    const int u = (int)blockIdx.z;
     const int slo = u - (SB - 1) > 0 ? u - (SB - 1) : 0, shi = u < SA - 1 ? u : SA - 1;
-    v4i acc[2][2];
+    const int64_t kst = K / OZ_KS;
+    const int64_t stages = (int64_t)(shi - slo + 1) * kst;
+    v4i acc[WM][WN];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < WM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = v4i{0, 0, 0, 0};
-    // staging: 256 threads x 16 bytes = one 64 x 64-byte tile of each operand
-    const int sr = t >> 2, sc = (t & 3) * 16;
-    for (int s = slo; s <= shi; ++s) {
-        const int8_t *pa = dA + (int64_t)s * planeA, *pb = dB + (int64_t)(u - s) * planeB;
-        for (int64_t k0 = 0; k0 < K; k0 += 64) {
-            v4i va = v4i{0, 0, 0, 0}, vb = v4i{0, 0, 0, 0};
-            const int64_t gi = i0 + sr, gj = j0 + sr, gk = k0 + sc;
-            if (gi < m) va = *(const v4i *)(pa + gi * lda + gk);
-            if (gj < n) vb = *(const v4i *)(pb + gj * ldb + gk);
-            *(v4i *)(As + sr * 64 + sc) = va;
-            *(v4i *)(Bs + sr * 64 + sc) = vb;
-            __syncthreads();
-            v4i fa[2], fb[2];
+        for (int b = 0; b < WN; ++b) acc[a][b] = v4i{0, 0, 0, 0};
+    v4i ra[CA], rb[CB];
+    auto fetch = [&](int64_t g) {
+        const int sp = slo + (int)(g / kst);
+        const int64_t k0 = (g % kst) * OZ_KS;
+        const int8_t *pa = dA + (int64_t)sp * planeA + k0, *pb = dB + (int64_t)(u - sp) * planeB + k0;
 #pragma unroll
-            for (int a = 0; a < 2; ++a) fa[a] = *(const v4i *)(As + (wy * 32 + a * 16 + (lane & 15)) * 64 + (lane >> 4) * 16);
-#pragma unroll
-            for (int b = 0; b < 2; ++b) fb[b] = *(const v4i *)(Bs + (wx * 32 + b * 16 + (lane & 15)) * 64 + (lane >> 4) * 16);
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-            __syncthreads();
+        for (int q = 0; q < CA; ++q) {
+            const int c = t + 256 * q, r = c / (OZ_KS / 16), col = (c % (OZ_KS / 16)) * 16;
+            ra[q] = (i0 + r < m) ? *(const v4i *)(pa + (i0 + r) * lda + col) : v4i{0, 0, 0, 0};
         }
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+            const int c = t + 256 * q, r = c / (OZ_KS / 16), col = (c % (OZ_KS / 16)) * 16;
+            rb[q] = (j0 + r < n) ? *(const v4i *)(pb + (j0 + r) * ldb + col) : v4i{0, 0, 0, 0};
+        }
+    };
+    if (stages > 0) fetch(0);
+    for (int64_t g = 0; g < stages; ++g) {
+#pragma unroll
+        for (int q = 0; q < CA; ++q) {
+            const int c = t + 256 * q, r = c / (OZ_KS / 16), col = (c % (OZ_KS / 16)) * 16;
+            *(v4i *)(As + r * OZ_LDW + col) = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+            const int c = t + 256 * q, r = c / (OZ_KS / 16), col = (c % (OZ_KS / 16)) * 16;
+            *(v4i *)(Bs + r * OZ_LDW + col) = rb[q];
+        }
+        __syncthreads();
+        if (g + 1 < stages) fetch(g + 1);
+#pragma unroll
+        for (int ks = 0; ks < OZ_KS / 64; ++ks) {
+            v4i fa[WM], fb[WN];
+#pragma unroll
+            for (int a = 0; a < WM; ++a) fa[a] = *(const v4i *)(As + (wy * 16 * WM + a * 16 + (lane & 15)) * OZ_LDW + ks * 64 + (lane >> 4) * 16);
+#pragma unroll
+            for (int b = 0; b < WN; ++b) fb[b] = *(const v4i *)(Bs + (wx * 16 * WN + b * 16 + (lane & 15)) * OZ_LDW + ks * 64 + (lane >> 4) * 16);
+#pragma unroll
+            for (int a = 0; a < WM; ++a)
+#pragma unroll
+                for (int b = 0; b < WN; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fb[b], fa[a], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
     }
-    // D item v of lane l: row 4 (l / 16) + v, column l % 16 of the 16 x 16 block
-    int32_t *o = out + (int64_t)blockIdx.z * m * n;
+    // with the B fragment first, item v of lane l holds row (lane % 16) and
+    // column 4 (lane / 16) + v of the 16 x 16 block: coalesced column stores
+    int32_t *o = out + (int64_t)u * m * n;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < WM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < WN; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t i = i0 + wy * 32 + a * 16 + 4 * (lane >> 4) + v;
-                const int64_t j = j0 + wx * 32 + b * 16 + (lane & 15);
+                const int64_t i = i0 + wy * 16 * WM + a * 16 + (lane & 15);
+                const int64_t j = j0 + wx * 16 * WN + b * 16 + 4 * (lane >> 4) + v;
                 if (i < m && j < n) o[j * m + i] = acc[a][b][v];
             }
 }
@@ -299,12 +331,21 @@ hipError_t oz_gemm(hipStream_t s, int64_t m, int64_t n, int64_t K, const int8_t 
                    const int8_t *dB, int64_t ldb, int64_t planeB, int SB, int U, int32_t *out) {
     if (m == 0 || n == 0) return hipSuccess;
     // int32 headroom: pairs * K * 2^12 < 2^31
-    if (K > 8192 || K % 64 != 0 || lda % 16 != 0 || ldb % 16 != 0 || (SA < SB ? SA : SB) > 34) return hipErrorInvalidValue;
-    const int tiles_m = (int)cdiv(m, 64);
-    const dim3 grid((unsigned)(tiles_m * cdiv(n, 64)), 1, (unsigned)(U + 1));
-    hipLaunchKernelGGL(k_oz_gemm, grid, dim3(256), 0, s, m, n, K, dA, lda, planeA, SA, dB, ldb, planeB, SB, 0, out, tiles_m);
+    if (K > 8192 || K % OZ_KS != 0 || lda % 16 != 0 || ldb % 16 != 0 || (SA < SB ? SA : SB) > 34 || U > SA + SB - 2)
+        return hipErrorInvalidValue;
+    if (n <= 64) {
+        const int tiles_m = (int)cdiv(m, 128);
+        const dim3 grid((unsigned)(tiles_m * cdiv(n, 64)), 1, (unsigned)(U + 1));
+        hipLaunchKernelGGL((k_oz_gemm<4, 2>), grid, dim3(256), 0, s, m, n, K, dA, lda, planeA, SA, dB, ldb, planeB, SB, out, tiles_m);
+    } else {
+        const int tiles_m = (int)cdiv(m, 128);
+        const dim3 grid((unsigned)(tiles_m * cdiv(n, 128)), 1, (unsigned)(U + 1));
+        hipLaunchKernelGGL((k_oz_gemm<4, 4>), grid, dim3(256), 0, s, m, n, K, dA, lda, planeA, SA, dB, ldb, planeB, SB, out, tiles_m);
+    }
     return hipGetLastError();
 }
+
+int oz_row_align() { return OZ_KS; }
 
 hipError_t oz_combine(int N, hipStream_t s, int64_t m, int64_t n, int U, const int32_t *planes, const int *ea, const int *eb,
                       const DV &C, const double *alpha, const double *beta, int beta_zero, int mode) {

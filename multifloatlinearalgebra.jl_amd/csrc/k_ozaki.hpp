@@ -7,6 +7,8 @@ namespace mfk {
 // Digits per operand for an N-limb result (7 bits each, 34 for Float64x4).
 int oz_digits(int N);
 int oz_chunks();
+// digit rows are padded to a multiple of this many bytes
+int oz_row_align();
 // e[i] = ilogb(max_k |hi(X(i, k))|) + 3 (or a zero-row sentinel); part:
 // oz_chunks() * m doubles of scratch. uplo 0 general, 1 / 2: X symmetric
 // stored in its lower / upper triangle.
@@ -21,7 +23,7 @@ hipError_t oz_round_fp64(hipStream_t s, int64_t K, int64_t ncols, const DV &D, c
                          int64_t plane);
 // out[u][j][i] (int32, m x n column-major per u) = sum over digit pairs
 // s + t = u of sum_k dA_s(i, k) dB_t(j, k), for u = 0..U. K = padded digit
-// row length (<= 8192, multiple of 64).
+// row length (<= 8192, multiple of oz_row_align()).
 hipError_t oz_gemm(hipStream_t s, int64_t m, int64_t n, int64_t K, const int8_t *dA, int64_t lda, int64_t planeA, int SA,
                    const int8_t *dB, int64_t ldb, int64_t planeB, int SB, int U, int32_t *out);
 // acc = sum_u planes[u] 2^(ea_i + eb_j - 7(u + 2)) in N limbs (u ascending);
