@@ -3,7 +3,7 @@
 Headline (the JSON `value`): Float64x4 gemm! C = A*B at n = 4096, inputs
 resident in HBM, reported in FP64-TFLOP/s with EFT FLOPs counted:
     FLOPs = m*k*n * F_4, F_4 = 239 (x4 add network 124 adds + x4 product
-    network 93 adds + 10 muls + 6 FMAs counted as 2; 233 FP64 instructions).
+    network 89 adds + 6 muls + 10 FMAs counted as 2; 229 FP64 instructions).
 One step = one gemm! call = one kernel launch. The roofline entry prices the
 kernel against MI355X's FP64 peak (78.6 TFLOP/s); `issue_frac` gives FP64
 instruction issue against 39.3 T lane-instructions/s.
@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "multifloatlinearalgebra.jl_amd"))
 
 # FP64 operations per multiply-accumulate of each network: (FLOPs, instructions)
-NETWORK_COST = {1: (2, 2), 2: (30, 29), 3: (111, 108), 4: (239, 233)}
+NETWORK_COST = {1: (2, 2), 2: (30, 27), 3: (111, 108), 4: (239, 229)}
 NETWORK_COST_X3_FUSED = (102, 99)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP64_INSTR = 39.3e12
@@ -105,6 +105,33 @@ def bench_gemm(args, M, torch):
     return {"A": A, "B": B, "ms": ms, "wall": wall, "tflops": tflops, "issue_frac": issue, "macs": macs}
 
 
+def bench_ozaki(args, M, torch, g):
+    """The same product with gemm_strategy=:ozaki (INT8 matrix cores, exact
+    digit products recombined in Float64xN; normwise accuracy, not MFLA's
+    rounding order). Reported beside the headline, never as `value`."""
+    N, n = args.limbs, args.n
+    Ad = torch.from_numpy(g["A"]).cuda()
+    Bd = torch.from_numpy(g["B"]).cuda()
+    Cd = torch.zeros((N, n, n), dtype=torch.float64, device="cuda")
+    cfg = M.KernelConfig(gemm_strategy="ozaki")
+    M.gemm_(Cd, Ad, Bd, config=cfg)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    steps = max(1, args.steps)
+    e0.record()
+    for _ in range(steps):
+        M.gemm_(Cd, Ad, Bd, config=cfg)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    S = {1: 11, 2: 19, 3: 27, 4: 34}[N]
+    ops = 2.0 * (S + 1) * (S + 2) / 2 * n * n * ((n + 127) // 128 * 128)
+    return {"ms": round(ms, 3), "speedup_vs_eft_gemm": round(g["ms"] / ms, 2),
+            "int8_tops": round(ops / (ms * 1e-3) / 1e12, 1), "int8_frac_of_5pops": round(ops / (ms * 1e-3) / 5.0e15, 4),
+            "x_equiv_tflops": round(float(n) ** 3 * NETWORK_COST[N][0] / (ms * 1e-3) / 1e12, 2),
+            "note": "digit-pair INT8 MFMA products; not EFT FLOPs, not the headline"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +144,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--no-ozaki", action="store_true")
     ap.add_argument("--suite", default="",
                     help="comma list of suite groups (blas3,factor,blas2,refine,batched,autodiff): print one JSON line "
                          "per measured case instead of the headline line")
@@ -177,6 +205,8 @@ def main():
         from mfla import refine as R
 
         out["refine"] = R.bench_refine(n=args.n)
+    if not args.no_ozaki:
+        out["ozaki_gemm"] = bench_ozaki(args, M, torch, g)
     print(json.dumps(out))
 
 

@@ -220,6 +220,7 @@ hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int di
     a.n = n;
     a.unit = diag == MFLA_UNIT;
     a.exact_order = trsm_exact(c, n) ? 1 : 0;
+    a.swap_product = side == MFLA_RIGHT ? 1 : 0;   // B[i,k]*coeff(k,j) (trsm.jl:136-172)
     DV X, CA;
     if (side == MFLA_LEFT) {
         X = dvn(B); a.nrhs = B.cols;
@@ -621,7 +622,7 @@ int mfla_potrf(mfla_handle h, int N, mfla_matrix A, int64_t *info, const mfla_co
         const int64_t rest = n - last - 1;
         // trsm!(A21, L11; side=:right, uplo=:lower, trans=:T) -> forward over panel columns
         mfk::TrsmArgs ta{};
-        ta.n = nb; ta.nrhs = rest; ta.forward = 1; ta.unit = 0; ta.exact_order = 1;
+        ta.n = nb; ta.nrhs = rest; ta.forward = 1; ta.unit = 0; ta.exact_order = 1; ta.swap_product = 1;
         const DV A21 = sub(a, last + 1, first), L11 = sub(a, first, first);
         const DV A21t{A21.p, A21.cs, A21.rs, A21.ls};
         HIPCHK(mfk::trsm_solve(N, s, ta, A21t, L11, L11, st));

@@ -28,7 +28,7 @@ namespace mfk {
 template <int N, int NET, int TM, int TN, int BK, int MODE, bool LOWER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_tile(int64_t n, int64_t K, DV A, DV B, DV C, S<N> alpha, S<N> beta,
                                               int beta_zero, int64_t row_lo, int64_t row_hi, const Status *status,
-                                              int64_t status_limit) {
+                                              int64_t status_limit, int swap) {
     if (status) {
         const int64_t info = status->info;
         if (info != 0 && (status_limit == 0 || info < 0 || info <= status_limit)) return;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             for (int a = 0; a < TM; ++a)
 #pragma unroll
                 for (int b = 0; b < TN; ++b) {
-                    if (MODE == MODE_SUB) acc[a][b] = mf::msub(acc[a][b], av[a], bv[b]);
+                    if (MODE == MODE_SUB) acc[a][b] = swap ? mf::msub(acc[a][b], bv[b], av[a]) : mf::msub(acc[a][b], av[a], bv[b]);
                     else acc[a][b] = mf::Mac<N, NET>::apply(acc[a][b], av[a], bv[b]);
                 }
         }
@@ -139,24 +139,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 // ---------------------------------------------------------------- launcher
 template <int N, int NET, int TM, int TN, int BK, int MODE, bool LOWER>
 static hipError_t go(hipStream_t s, int64_t rows, int64_t n, int64_t K, const DV &A, const DV &B, const DV &C,
-                     const S<N> &al, const S<N> &be, int bz, int64_t row_lo, const Status *st, int64_t lim) {
+                     const S<N> &al, const S<N> &be, int bz, int64_t row_lo, const Status *st, int64_t lim, int swap) {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     dim3 grid((unsigned)cdiv(rows, BM), (unsigned)cdiv(n, BN));
     hipLaunchKernelGGL((k_tile<N, NET, TM, TN, BK, MODE, LOWER>), grid, dim3(256), 0, s, n, K, A, B, C, al, be, bz, row_lo,
-                       row_lo + rows, st, lim);
+                       row_lo + rows, st, lim, swap);
     return hipGetLastError();
 }
 
 template <int N, int NET, int MODE, bool LOWER>
 static hipError_t pick(hipStream_t s, int tile, int64_t rows, int64_t n, int64_t K, const DV &A, const DV &B, const DV &C,
-                       const S<N> &al, const S<N> &be, int bz, int64_t row_lo, const Status *st, int64_t lim) {
+                       const S<N> &al, const S<N> &be, int bz, int64_t row_lo, const Status *st, int64_t lim, int swap) {
     // tile 1: 32x32 tiles (2x2 per thread); tile 2: 32x64 tiles (2x4 per
     // thread, eight independent MAC chains per thread); tile 0: 16x16 tiles
     // (1x1 per thread) for small outputs where larger tiles would leave most
     // CUs idle.
-    if (tile == 2) return go<N, NET, 2, 4, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim);
-    if (tile == 1) return go<N, NET, 2, 2, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim);
-    return go<N, NET, 1, 1, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim);
+    if (tile == 2) return go<N, NET, 2, 4, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
+    if (tile == 1) return go<N, NET, 2, 2, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
+    return go<N, NET, 1, 1, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
 }
 
 static int auto_tile(int64_t rows, int64_t n) {
@@ -173,17 +173,17 @@ hipError_t tile_launch(hipStream_t s, const TileArgs &t, const DV &A, const DV &
     const int tile = t.tile >= 0 ? t.tile : auto_tile(t.rows, t.n);
     const int bz = t.beta_zero;
     if (t.mode == MODE_SUB) {
-        if (t.lower) return pick<N, mf::NET_STD, MODE_SUB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit);
-        return pick<N, mf::NET_STD, MODE_SUB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit);
+        if (t.lower) return pick<N, mf::NET_STD, MODE_SUB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);
+        return pick<N, mf::NET_STD, MODE_SUB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);
     }
     if constexpr (N == 3) {
         if (t.net == mf::NET_FUSED3) {
-            if (t.lower) return pick<3, mf::NET_FUSED3, MODE_AB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit);
-            return pick<3, mf::NET_FUSED3, MODE_AB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit);
+            if (t.lower) return pick<3, mf::NET_FUSED3, MODE_AB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);
+            return pick<3, mf::NET_FUSED3, MODE_AB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);
         }
     }
-    if (t.lower) return pick<N, mf::NET_STD, MODE_AB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit);
-    return pick<N, mf::NET_STD, MODE_AB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit);
+    if (t.lower) return pick<N, mf::NET_STD, MODE_AB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);
+    return pick<N, mf::NET_STD, MODE_AB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);
 }
 
 hipError_t tile_launch_n(int N, hipStream_t s, const TileArgs &t, const DV &A, const DV &B, const DV &C, const double *alpha,

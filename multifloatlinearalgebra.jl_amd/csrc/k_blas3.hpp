@@ -18,6 +18,7 @@ struct TileArgs {
     int64_t K;         // reduction length
     int64_t status_limit;  // 0: return on any recorded failure; > 0: only on
                            // failures at (one-based) columns <= status_limit
+    int swap_product;      // MODE_SUB: terms b(k,j)*a(i,k) instead of a*b
 };
 
 // C(i, j) for i in [row_lo, row_lo + rows), j in [0, n) from a(i, k) = A(i, k)

@@ -32,7 +32,7 @@ constexpr int TC = 4;    // right-hand sides per workgroup (threads in y)
 
 template <int N, bool FWD>
 __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit,
-                                                      const Status *st) {
+                                                      int swap, const Status *st) {
     if (st && st->info != 0) return;
     __shared__ double xs[TC * N];
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
             mf::V<N> xq;
 #pragma unroll
             for (int l = 0; l < N; ++l) xq.x[l] = xs[ty * N + l];
-            v = mf::msub(v, ld<N>(CA, p, q), xq);
+            v = swap ? mf::msub(v, xq, ld<N>(CA, p, q)) : mf::msub(v, ld<N>(CA, p, q), xq);
         }
         __syncthreads();
     }
@@ -62,13 +62,14 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
 
 // Exact MFLA order for backward substitution: one thread per right-hand side.
 template <int N>
-__global__ void k_trsm_seq_bwd(int64_t n, int64_t nrhs, DV X, DV CA, DV D, int unit, const Status *st) {
+__global__ void k_trsm_seq_bwd(int64_t n, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap, const Status *st) {
     if (st && st->info != 0) return;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nrhs) return;
     for (int64_t p = n - 1; p >= 0; --p) {
         mf::V<N> v = ld<N>(X, p, c);
-        for (int64_t q = p + 1; q < n; ++q) v = mf::msub(v, ld<N>(CA, p, q), ld<N>(X, q, c));
+        for (int64_t q = p + 1; q < n; ++q)
+            v = swap ? mf::msub(v, ld<N>(X, q, c), ld<N>(CA, p, q)) : mf::msub(v, ld<N>(CA, p, q), ld<N>(X, q, c));
         if (!unit) v = mf::div(v, ld<N>(D, p, p));
         stv<N>(X, p, c, v);
     }
@@ -78,10 +79,10 @@ static DV sub(const DV &v, int64_t i, int64_t j) { return DV{v.p + i * v.rs + j 
 
 template <int N>
 static hipError_t diag(hipStream_t s, bool fwd, int64_t p0, int nbk, int64_t nrhs, const DV &X, const DV &CA, const DV &D, int unit,
-                       const Status *st) {
+                       int swap, const Status *st) {
     dim3 grid((unsigned)cdiv(nrhs, TC)), block(TB, TC);
-    if (fwd) hipLaunchKernelGGL((k_trsm_diag<N, true>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, st);
-    else hipLaunchKernelGGL((k_trsm_diag<N, false>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, st);
+    if (fwd) hipLaunchKernelGGL((k_trsm_diag<N, true>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st);
+    else hipLaunchKernelGGL((k_trsm_diag<N, false>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st);
     return hipGetLastError();
 }
 
@@ -92,10 +93,11 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
     hipError_t e = hipSuccess;
     TileArgs t{};
     t.mode = MODE_SUB; t.net = mf::NET_STD; t.lower = 0; t.beta_zero = 0; t.tile = -1; t.row_lo = 0; t.n = nrhs;
+    t.swap_product = a.swap_product;
     if (a.forward) {
         for (int64_t p0 = 0; p0 < n; p0 += TB) {
             const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
-            if ((e = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, st)) != hipSuccess) return e;
+            if ((e = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st)) != hipSuccess) return e;
             const int64_t p1 = p0 + nbk;
             if (p1 < n) {
                 t.rows = n - p1; t.K = nbk;
@@ -105,13 +107,13 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
         return hipSuccess;
     }
     if (a.exact_order) {
-        hipLaunchKernelGGL(k_trsm_seq_bwd<N>, dim3((unsigned)cdiv(nrhs, 64)), dim3(64), 0, s, n, nrhs, X, CA, D, a.unit, st);
+        hipLaunchKernelGGL(k_trsm_seq_bwd<N>, dim3((unsigned)cdiv(nrhs, 64)), dim3(64), 0, s, n, nrhs, X, CA, D, a.unit, a.swap_product, st);
         return hipGetLastError();
     }
     for (int64_t p1 = n; p1 > 0; p1 -= TB) {
         const int64_t p0 = p1 > TB ? p1 - TB : 0;
         const int nbk = (int)(p1 - p0);
-        if ((e = diag<N>(s, false, p0, nbk, nrhs, X, CA, D, a.unit, st)) != hipSuccess) return e;
+        if ((e = diag<N>(s, false, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st)) != hipSuccess) return e;
         if (p0 > 0) {
             t.rows = p0; t.K = nbk;
             if ((e = tile_launch_n(N, s, t, sub(CA, 0, p0), sub(X, p0, 0), X, nullptr, nullptr, st)) != hipSuccess) return e;

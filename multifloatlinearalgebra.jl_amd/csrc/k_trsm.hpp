@@ -10,6 +10,8 @@ struct TrsmArgs {
     int forward;      // 1: dependencies on q < p; 0: on q > p
     int unit;         // unit diagonal (no division)
     int exact_order;  // backward only: 1 = MFLA's ascending-q chain, 0 = blocked
+    int swap_product; // 1: each term is X(q)*CA(p,q) (MFLA's right-side solves,
+                      // trsm.jl:136-149, 159-172); 0: CA(p,q)*X(q) (left side)
 };
 
 // Solves in place on the solution view X(p, c) with coefficient view CA(p, q)

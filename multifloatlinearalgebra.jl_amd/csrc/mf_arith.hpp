@@ -126,10 +126,10 @@ MF_HD V<1> mul(const V<1> &x, const V<1> &y) { V<1> r; r.x[0] = x.x[0] * y.x[0];
 MF_HD V<2> mul(const V<2> &x, const V<2> &y) {
     double p, e;
     two_prod(x.x[0], y.x[0], p, e);
-    double t = x.x[0] * y.x[1];
-    double u = x.x[1] * y.x[0];
-    t += u;
-    e += t;
+    // first-order cross terms fused into the error term (two FMAs = the same
+    // two multiplies and two additions, one rounding each)
+    e = __builtin_fma(x.x[0], y.x[1], e);
+    e = __builtin_fma(x.x[1], y.x[0], e);
     fast_two_sum(p, e, p, e);
     V<2> r; r.x[0] = p; r.x[1] = e; return r;
 }
@@ -172,10 +172,6 @@ MF_HD V<4> mul(const V<4> &x, const V<4> &y) {
     two_prod(x.x[0], y.x[2], p02, e02);
     two_prod(x.x[1], y.x[1], p11, e11);
     two_prod(x.x[2], y.x[0], p20, e20);
-    double q03 = x.x[0] * y.x[3];
-    double q12 = x.x[1] * y.x[2];
-    double q21 = x.x[2] * y.x[1];
-    double q30 = x.x[3] * y.x[0];
     double s1, t1, t2, p0;
     two_sum(p01, p10, s1, t1);
     two_sum(e00, s1, s1, t2);
@@ -187,18 +183,22 @@ MF_HD V<4> mul(const V<4> &x, const V<4> &y) {
     two_sum(a, d, a, g);
     two_sum(t1, t2, h, i);
     two_sum(a, h, a, j);
-    double z = q03 + q30;
-    double w = q12 + q21;
-    z += w;
-    w = e02 + e20;
-    w += e11;
-    z += w;
-    w = b + c;
+    // third-order terms: the rounded errors of the lower orders summed, then
+    // the four third-order products fused in (FMA: one multiply and one
+    // addition each)
+    double z = e02 + e20;
+    z += e11;
+    double w = b + c;
     z += w;
     w = f + g;
     z += w;
     w = i + j;
     z += w;
+    z = __builtin_fma(x.x[0], y.x[3], z);
+    z = __builtin_fma(x.x[3], y.x[0], z);
+    z = __builtin_fma(x.x[1], y.x[2], z);
+    z = __builtin_fma(x.x[2], y.x[1], z);
+
     two_sum(s1, a, s1, a);
     two_sum(a, z, a, z);
     fast_two_sum(p0, s1, p0, s1);

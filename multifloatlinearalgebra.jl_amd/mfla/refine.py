@@ -132,7 +132,7 @@ def _refine_workspace(N, kind, n, nrhs, flag=0):
     return w, need.value
 
 
-def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general", stages=False, residual="limbs"):
+def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general", stages=False, residual="auto"):
     """Solve A X = B (A, B Float64xN on the device, N = 2..4) with a binary64
     factor and Float64xN refinement, in the library's native driver
     (mfla_refine_solve). The default tolerance 2^-(52N+8) stops at the
@@ -141,12 +141,16 @@ def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general"
     estimate after each correction -- and with stages=True
     also milliseconds per stage (factor, solve, update, residual, check).
     residual="limbs" sums R -= A*D in as many limbs as |D|/|X| requires;
-    residual="ozaki" uses the Ozaki-split INT8 matrix-core product."""
+    residual="ozaki" uses the Ozaki-split INT8 matrix-core product (exact
+    digit products recombined in Float64xN; n <= 8192); "auto" picks ozaki
+    when n <= 8192."""
     import ctypes
     from . import _lib as L
     N = M._limbs(A, B)
-    if residual not in ("limbs", "ozaki"):
-        raise M.ArgumentError("residual must be :limbs or :ozaki")
+    if residual not in ("limbs", "ozaki", "auto"):
+        raise M.ArgumentError("residual must be :auto, :limbs or :ozaki")
+    if residual == "auto":
+        residual = "ozaki" if A.shape[2] <= 8192 else "limbs"
     flag = 16 if residual == "ozaki" else 0
     tol = tol if tol is not None else 2.0 ** (-52 * N - 8)
     n, nrhs = A.shape[2], B.shape[1]
@@ -213,7 +217,7 @@ def plain_solve(A, B, kind="cholesky", *, block=16):
     return M.solve(F, B, config=M.KernelConfig(trsm_order="blocked"))
 
 
-def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812, residual="limbs"):
+def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812, residual="auto"):
     """Times the refined and the plain Float64xN solve on the same matrix and
     right-hand sides; reports times and Float64xN backward errors."""
     A = spd_kappa(n, kappa, N=N, seed=seed) if kind == "cholesky" else general_kappa(n, kappa, N=N, seed=seed)

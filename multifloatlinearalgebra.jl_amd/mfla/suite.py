@@ -5,7 +5,7 @@ priced by.
 Compute-bound operations report EFT FP64-TFLOP/s = MACs * F_N / t against the
 FP64 vector peak (78.6 TFLOP/s), with F_N the FLOPs of one multiply-accumulate
 network (2, 30, 111, 239 for N = 1..4), and the FP64 instruction issue rate
-(instructions per MAC: 2, 29, 108, 233; 99 on the x3 fused route) against
+(instructions per MAC: 2, 27, 108, 229; 99 on the x3 fused route) against
 39.3 T lane-instructions/s. Level-2 operations also report HBM GB/s against
 8 TB/s with compulsory bytes 8N(mn + m + n) per right-hand side.
 
@@ -25,7 +25,7 @@ import torch
 from . import api as M
 
 F_N = {1: 2, 2: 30, 3: 111, 4: 239}
-INSTR_N = {1: 2, 2: 29, 3: 108, 4: 233}
+INSTR_N = {1: 2, 2: 27, 3: 108, 4: 229}
 INSTR_X3_FUSED = 99
 PEAK_TF = 78.6
 PEAK_INSTR = 39.3e12

@@ -144,10 +144,10 @@ static inline MF<1> mul(const MF<1> &x, const MF<1> &y) { MF<1> r; r.x[0] = x.x[
 static inline MF<2> mul(const MF<2> &x, const MF<2> &y) {
     double p, e;
     two_prod(x.x[0], y.x[0], p, e);
-    double t = x.x[0] * y.x[1];
-    double u = x.x[1] * y.x[0];
-    t += u;
-    e += t;
+    // first-order cross terms fused into the error term (two FMAs = the same
+    // two multiplies and two additions, one rounding each)
+    e = __builtin_fma(x.x[0], y.x[1], e);
+    e = __builtin_fma(x.x[1], y.x[0], e);
     fast_two_sum(p, e, p, e);
     MF<2> r; r.x[0] = p; r.x[1] = e; return r;
 }
@@ -196,10 +196,6 @@ static inline MF<4> mul(const MF<4> &x, const MF<4> &y) {
     two_prod(x.x[0], y.x[2], p02, e02);
     two_prod(x.x[1], y.x[1], p11, e11);
     two_prod(x.x[2], y.x[0], p20, e20);
-    double q03 = x.x[0] * y.x[3];
-    double q12 = x.x[1] * y.x[2];
-    double q21 = x.x[2] * y.x[1];
-    double q30 = x.x[3] * y.x[0];
     // first-order diagonal
     double s1, t1, t2;
     two_sum(p01, p10, s1, t1);
@@ -214,20 +210,21 @@ static inline MF<4> mul(const MF<4> &x, const MF<4> &y) {
     two_sum(a, d, a, g);
     two_sum(t1, t2, h, i);
     two_sum(a, h, a, j);
-    // third-order diagonal (rounded)
-    double z = q03 + q30;
-    double w = q12 + q21;
-    z += w;
-    w = e02 + e20;
-    w += e11;
-    z += w;
-    w = b + c;
+    // third-order terms: the rounded errors of the lower orders summed, then
+    // the four third-order products fused in (FMA: one multiply and one
+    // addition each)
+    double z = e02 + e20;
+    z += e11;
+    double w = b + c;
     z += w;
     w = f + g;
     z += w;
     w = i + j;
     z += w;
-    // merge and renormalise
+    z = __builtin_fma(x.x[0], y.x[3], z);
+    z = __builtin_fma(x.x[3], y.x[0], z);
+    z = __builtin_fma(x.x[1], y.x[2], z);
+    z = __builtin_fma(x.x[2], y.x[1], z);
     two_sum(s1, a, s1, a);
     two_sum(a, z, a, z);
     fast_two_sum(p0, s1, p0, s1);

@@ -147,7 +147,7 @@ def test_native_refinement_reaches_working_precision(oracle, kind, uplo, n, nrhs
     for l in range(1, N):
         Bh[l] = Bh[l - 1] * 2.0**-53 * rng.uniform(-1, 1, (nrhs, n))
     B = torch.from_numpy(Bh).cuda()
-    X, hist = R.refine_solve(A, B, kind, uplo=uplo)
+    X, hist = R.refine_solve(A, B, kind, uplo=uplo, residual="limbs")
     assert 2 <= len(hist) <= 12 and hist[-1] <= 64 * 2.0 ** (-52 * N), hist
     Ah = A.cpu().numpy()
     if uplo == "lower":
