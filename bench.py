@@ -36,7 +36,7 @@ PEAK_FP64_INSTR = 39.3e12
 # HBM bytes per launch of the Float64x4 n=4096 GEMM kernel: rocprofv3 FETCH_SIZE
 # and WRITE_SIZE of one dispatch, each scaled by the same counter's reading for
 # a plain 512 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
-MEASURED_TRAFFIC = {(4, 4096): {"bytes": 39.52e9, "fetch_kib": 19033320, "write_kib": 524288, "fetch_scale": 2.0,
+MEASURED_TRAFFIC = {(4, 4096): {"bytes": 39.29e9, "fetch_kib": 18923466, "write_kib": 524288, "fetch_scale": 2.0,
                                 "write_scale": 1.0}}
 
 

@@ -307,10 +307,14 @@ int mfla_refine_solve(mfla_handle h, int limbs, int kind, int uplo, mfla_matrix 
  * (the other triangle is not read), MFLA_GENERAL a dense operand.
  * mfla_dpotrf_inverse: binary64 Cholesky of the lower triangle of A in place
  * and the inverse of the factor into the lower triangle of Inv (its strict
- * upper triangle is used as scratch). */
+ * upper triangle is used as scratch).
+ * mfla_dgetrf: binary64 LU with partial pivoting of the square A in place
+ * (LAPACK dgetrf semantics: device ipiv of n one-based row indices, info =
+ * first zero pivot column, -1 for non-finite input; n <= 32768). */
 int mfla_dgemm(mfla_handle h, int trans_a, int trans_b, int tri_a, int tri_b, mfla_matrix C, mfla_matrix A, mfla_matrix B,
                double alpha, double beta);
 int mfla_dpotrf_inverse(mfla_handle h, mfla_matrix A, mfla_matrix Inv, int64_t *info);
+int mfla_dgetrf(mfla_handle h, mfla_matrix A, int64_t *ipiv, int64_t *info);
 
 /* ----------------------------------------------------- reverse-mode rules */
 /* MFLA has no differentiation rules (its dependencies are LinearAlgebra and

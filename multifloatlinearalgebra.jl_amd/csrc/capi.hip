@@ -1032,6 +1032,35 @@ hipError_t dpotrf_core(hipStream_t s, int64_t n, const DV &F, const DV &Inv, Sta
     return hipSuccess;
 }
 
+// Blocked right-looking binary64 LU with partial pivoting of the n x n F
+// (LAPACK dgetrf semantics: ipiv 1-based, info = first zero pivot column).
+// Each 64-column block is factored by register-resident leaves of
+// dlu_leaf_width(n) columns, right-looking inside the block (swap + unit
+// solve of the block's later columns, MFMA update below); the block's
+// interchanges then reach the columns left and right of it, the right part
+// gets U12 = L11^{-1} P A12 and the trailing matrix A22 -= L21 U12 on the
+// FP64 matrix cores. The per-column work stays inside one workgroup per leaf,
+// so a 4096 x 4096 factor costs 4096 leaf columns plus 64 block updates.
+hipError_t dgetrf_core(hipStream_t s, int64_t n, const DV &F, int64_t *ipiv, Status *st) {
+    const int W = mfk::dlu_leaf_width(n);
+    if (W == 0) return hipErrorInvalidValue;
+    for (int64_t j0 = 0; j0 < n; j0 += kDiagBlock) {
+        const int64_t jb = std::min<int64_t>(kDiagBlock, n - j0), jend = j0 + jb;
+        for (int64_t c0 = j0; c0 < jend; c0 += W) {
+            const int w = (int)std::min<int64_t>(W, jend - c0);
+            HIPRET(mfk::dlu_leaf(s, F, n, c0, w, ipiv, st));
+            HIPRET(mfk::dlu_swap_solve(s, F, c0, w, ipiv, j0, c0 - j0, c0 + w, jend - c0 - w));
+            if (c0 + w < jend && c0 + w < n)
+                HIPRET(mfk::dgemm(s, dg(n - c0 - w, jend - c0 - w, w, sub(F, c0 + w, c0), sub(F, c0, c0 + w), sub(F, c0 + w, c0 + w),
+                                        -1.0, 1.0), 1));
+        }
+        HIPRET(mfk::dlu_swap_solve(s, F, j0, (int)jb, ipiv, 0, j0, jend, n - jend));
+        if (jend < n)
+            HIPRET(mfk::dgemm(s, dg(n - jend, n - jend, jb, sub(F, jend, j0), sub(F, j0, jend), sub(F, jend, jend), -1.0, 1.0), 1));
+    }
+    return hipSuccess;
+}
+
 // Recursive doubling from inverted diagonal blocks of size nb to the full
 // inverse of a triangular factor T:
 //   lower: inv([T11 0; T21 T22]) = [X11 0; -X22 T21 X11  X22]
@@ -1205,6 +1234,23 @@ int mfla_dpotrf_inverse(mfla_handle h, mfla_matrix A, mfla_matrix Inv, int64_t *
     return MFLA_OK;
 }
 
+int mfla_dgetrf(mfla_handle h, mfla_matrix A, int64_t *ipiv, int64_t *info) {
+    if (check_desc(A, "A")) return MFLA_ERR_ARGUMENT;
+    if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "square matrix required");
+    const int64_t n = A.rows;
+    if (mfk::dlu_leaf_width(n) == 0) return fail(MFLA_ERR_ARGUMENT, "n <= 32768 required");
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    HIPCHK(mfk::status_reset(s, h->status));
+    HIPCHK(mfk::iota1(s, ipiv, n));
+    HIPCHK(mfk::finite_check(1, s, n, n, dvn(A), 0, h->status));
+    HIPCHK(dgetrf_core(s, n, dvn(A), ipiv, h->status));
+    const int64_t r = read_info(h);
+    if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+    *info = r;
+    return MFLA_OK;
+}
+
 int mfla_refine_workspace_size(int limbs, int kind, int64_t n, int64_t nrhs, int64_t *doubles) {
     if (limbs < 2 || limbs > 4) return fail(MFLA_ERR_ARGUMENT, "refinement needs a 2-4 limb working precision");
     if ((kind & ~MFLA_REFINE_OZAKI) != MFLA_REFINE_CHOLESKY && (kind & ~MFLA_REFINE_OZAKI) != MFLA_REFINE_LU)
@@ -1247,20 +1293,19 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
     StageClock clk{s, stage_ms != nullptr, {}};
 
     // ---- binary64 factor and inverse factors
+    if (kind == MFLA_REFINE_LU && mfk::dlu_leaf_width(n) == 0) return fail(MFLA_ERR_ARGUMENT, "LU refinement supports n <= 32768");
     clk.mark(ST_FACTOR);
-    if (kind == MFLA_REFINE_LU) {
-        HIPCHK(mfk::dcopy(s, n, n, F, Ahi, 0));
-        mfla_matrix Fm{w.F, n, n, n, n * n};
-        double om[4];
-        int64_t finfo = 0;
-        mfla_config c;
-        mfla_config_default(&c);
-        c.lu_block = 64;
-        if (int rc = mfla_getrf(h, 1, Fm, w.ipiv, om, &finfo, &c, nullptr)) return rc;
-        *info = finfo;
-        if (finfo != 0) return fail(MFLA_ERR_FAILED_FACTOR, "SingularException(%lld)", (long long)finfo);
-    }
     std::lock_guard<std::mutex> g(h->lock);
+    if (kind == MFLA_REFINE_LU) {
+        HIPCHK(mfk::status_reset(s, h->status));
+        HIPCHK(mfk::dcopy(s, n, n, F, Ahi, 0));
+        HIPCHK(mfk::finite_check(1, s, n, n, F, 0, h->status));
+        HIPCHK(dgetrf_core(s, n, F, w.ipiv, h->status));
+        const int64_t r = read_info(h);
+        if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+        *info = r;
+        if (r != 0) return fail(MFLA_ERR_FAILED_FACTOR, "SingularException(%lld)", (long long)r);
+    }
     if (kind == MFLA_REFINE_CHOLESKY) {
         HIPCHK(mfk::status_reset(s, h->status));
         HIPCHK(mfk::dcopy(s, n, n, F, Ahi, uplo == MFLA_UPPER ? 2 : 0));

@@ -265,6 +265,147 @@ __global__ void k_dgather_rows(int64_t m, int64_t ncols, DV dst, DV src, const i
     dst.p[i * dst.rs + c * dst.cs] = src.p[perm[i] * src.rs + c * src.cs];
 }
 
+// ------------------------------------------------- binary64 LU (leaves)
+// k_dlu_leaf: partial-pivoting LU of the w (<= W) columns [c0, c0 + w) over
+// rows [c0, m) by one 1024-thread workgroup with the panel in registers:
+// thread t holds rows c0 + t + 1024 i (i < RPT). Per column: workgroup argmax
+// of |a| (first maximum on ties), exchange of the pivot row through LDS,
+// scaling by the reciprocal pivot and a rank-1 update of the leaf's later
+// columns. ipiv gets 1-based global row indices. A zero (or NaN-only) pivot
+// column records info = column + 1 if no failure is recorded yet and is
+// skipped, as dgetf2 skips it.
+constexpr int LU_THREADS = 1024, LU_WAVES = LU_THREADS / 64;
+
+__device__ __forceinline__ void lu_pick(double &best, int &bi, double ob, int oi) {
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+}
+
+template <int RPT, int W>
+__global__ __launch_bounds__(LU_THREADS) void k_dlu_leaf(DV F, int64_t m, int64_t c0, int w, int64_t *ipiv, Status *st) {
+    __shared__ double red_v[LU_WAVES];
+    __shared__ int red_i[LU_WAVES];
+    __shared__ double prow[W], jrow[W];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int mr = (int)(m - c0);   // <= 32768
+    double v[RPT][W];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int r = t + LU_THREADS * i;
+#pragma unroll
+        for (int j = 0; j < W; ++j) v[i][j] = (r < mr && j < w) ? F.p[(c0 + r) * F.rs + (c0 + j) * F.cs] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        if (j < w) {                                         // uniform
+            double best = -1.0;
+            int bi = INT32_MAX;
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int r = t + LU_THREADS * i;
+                if (r < mr && r >= j) lu_pick(best, bi, fabs(v[i][j]), r);
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) lu_pick(best, bi, __shfl_xor(best, off), __shfl_xor(bi, off));
+            if (lane == 0) { red_v[wave] = best; red_i[wave] = bi; }
+            __syncthreads();
+            best = red_v[0]; bi = red_i[0];
+#pragma unroll
+            for (int k = 1; k < LU_WAVES; ++k) lu_pick(best, bi, red_v[k], red_i[k]);
+            if (best > 0.0) {                                // uniform
+                const int pt = bi % LU_THREADS, pi = bi / LU_THREADS;
+                if (t == pt) {
+#pragma unroll
+                    for (int i = 0; i < RPT; ++i)
+                        if (i == pi) {
+#pragma unroll
+                            for (int jj = 0; jj < W; ++jj) prow[jj] = v[i][jj];
+                        }
+                }
+                if (t == j) {
+#pragma unroll
+                    for (int jj = 0; jj < W; ++jj) jrow[jj] = v[0][jj];
+                }
+                __syncthreads();
+                if (bi != j) {
+                    if (t == j) {
+#pragma unroll
+                        for (int jj = 0; jj < W; ++jj) v[0][jj] = prow[jj];
+                    }
+                    if (t == pt) {
+#pragma unroll
+                        for (int i = 0; i < RPT; ++i)
+                            if (i == pi) {
+#pragma unroll
+                                for (int jj = 0; jj < W; ++jj) v[i][jj] = jrow[jj];
+                            }
+                    }
+                }
+                if (t == 0) ipiv[c0 + j] = c0 + bi + 1;
+                const double inv = 1.0 / prow[j];
+#pragma unroll
+                for (int i = 0; i < RPT; ++i) {
+                    const int r = t + LU_THREADS * i;
+                    if (r < mr && r > j) {
+                        const double l = v[i][j] * inv;
+                        v[i][j] = l;
+#pragma unroll
+                        for (int jj = j + 1; jj < W; ++jj) v[i][jj] = fma(-l, prow[jj], v[i][jj]);
+                    }
+                }
+            } else if (t == 0) {                             // zero column
+                if (st->info == 0) st->info = c0 + j + 1;
+                ipiv[c0 + j] = c0 + j + 1;
+            }
+            __syncthreads();                                 // LDS reuse
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int r = t + LU_THREADS * i;
+        if (r < mr) {
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if (j < w) F.p[(c0 + r) * F.rs + (c0 + j) * F.cs] = v[i][j];
+        }
+    }
+}
+
+// k_dlu_swap_solve: one thread per column. Every column of range A
+// ([a0, a0 + na)) and range B ([b0, b0 + nb)) gets the row interchanges
+// ipiv[k0 .. k0 + kn) in order (dlaswp); range B columns then solve
+// L11 x = (P a)(k0 : k0 + kn) with the unit lower triangle L11 =
+// F(k0 : k0 + kn, k0 : k0 + kn) by forward substitution (x in registers).
+template <int KN>
+__global__ __launch_bounds__(256) void k_dlu_swap_solve(DV F, int64_t k0, int kn, const int64_t *ipiv, int64_t a0, int64_t na,
+                                                        int64_t b0, int64_t nb) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= na + nb) return;
+    const bool solve = e >= na;
+    const int64_t j = solve ? b0 + (e - na) : a0 + e;
+    double *col = F.p + j * F.cs;
+    for (int k = 0; k < kn; ++k) {
+        const int64_t r = k0 + k, p = ipiv[r] - 1;
+        if (p != r) {
+            const double a = col[r * F.rs], b = col[p * F.rs];
+            col[r * F.rs] = b;
+            col[p * F.rs] = a;
+        }
+    }
+    if (!solve) return;
+    double x[KN];
+#pragma unroll
+    for (int r = 0; r < KN; ++r) {
+        if (r < kn) {
+            const double *lrow = F.p + (k0 + r) * F.rs + k0 * F.cs;
+            double acc = col[(k0 + r) * F.rs];
+#pragma unroll
+            for (int q = 0; q < r; ++q) acc = fma(-lrow[q * F.cs], x[q], acc);
+            x[r] = acc;
+            col[(k0 + r) * F.rs] = acc;
+        }
+    }
+}
+
 // ---------------------------------------------------------- launchers
 hipError_t dgemm(hipStream_t s, DGemmArgs g, int64_t batch) {
     if (g.m == 0 || g.n == 0 || batch == 0) return hipSuccess;
@@ -301,6 +442,39 @@ hipError_t drow_abs_sum(hipStream_t s, int64_t m, int64_t n, const DV &A, int up
 }
 
 int drow_abs_chunks() { return ROW_CHUNKS; }
+
+int dlu_leaf_width(int64_t m) {
+    if (m <= 2048) return 16;
+    if (m <= 4096) return 8;
+    if (m <= 8192) return 4;
+    if (m <= 16384) return 2;
+    return m <= 32768 ? 1 : 0;
+}
+
+hipError_t dlu_leaf(hipStream_t s, const DV &F, int64_t m, int64_t c0, int w, int64_t *ipiv, Status *st) {
+    // the instance is chosen by the rows left (mr <= m, so its width is at
+    // least dlu_leaf_width(m))
+    const int64_t mr = m - c0;
+    if (w < 1 || mr < 1 || w > dlu_leaf_width(mr)) return hipErrorInvalidValue;
+    const dim3 g(1), b(LU_THREADS);
+    if (mr <= 1024) hipLaunchKernelGGL((k_dlu_leaf<1, 16>), g, b, 0, s, F, m, c0, w, ipiv, st);
+    else if (mr <= 2048) hipLaunchKernelGGL((k_dlu_leaf<2, 16>), g, b, 0, s, F, m, c0, w, ipiv, st);
+    else if (mr <= 4096) hipLaunchKernelGGL((k_dlu_leaf<4, 8>), g, b, 0, s, F, m, c0, w, ipiv, st);
+    else if (mr <= 8192) hipLaunchKernelGGL((k_dlu_leaf<8, 4>), g, b, 0, s, F, m, c0, w, ipiv, st);
+    else if (mr <= 16384) hipLaunchKernelGGL((k_dlu_leaf<16, 2>), g, b, 0, s, F, m, c0, w, ipiv, st);
+    else hipLaunchKernelGGL((k_dlu_leaf<32, 1>), g, b, 0, s, F, m, c0, w, ipiv, st);
+    return hipGetLastError();
+}
+
+hipError_t dlu_swap_solve(hipStream_t s, const DV &F, int64_t k0, int kn, const int64_t *ipiv, int64_t a0, int64_t na, int64_t b0,
+                          int64_t nb) {
+    if (na + nb == 0 || kn == 0) return hipSuccess;
+    if (kn < 0 || kn > 64) return hipErrorInvalidValue;
+    const dim3 g((unsigned)cdiv(na + nb, 256)), b(256);
+    if (kn <= 16) hipLaunchKernelGGL(k_dlu_swap_solve<16>, g, b, 0, s, F, k0, kn, ipiv, a0, na, b0, nb);
+    else hipLaunchKernelGGL(k_dlu_swap_solve<64>, g, b, 0, s, F, k0, kn, ipiv, a0, na, b0, nb);
+    return hipGetLastError();
+}
 
 hipError_t dgather_rows(hipStream_t s, int64_t m, int64_t ncols, const DV &dst, const DV &src, const int64_t *perm) {
     if (m * ncols == 0) return hipSuccess;

@@ -20,8 +20,10 @@
 //    column (the rounded value is written back, so the caller uses exactly
 //    the value whose digits are multiplied) and split into T digits.
 //  * k_oz_gemm: for every u in [0, U], C_u(i,j) = sum_{s+t=u} sum_k dA_s dB_t
-//    as int32 planes; 64x64 output tile per workgroup (4 waves x 2x2 MFMA
-//    16x16x64 blocks), k-steps of 64 staged in LDS; grid z = u.
+//    as int32 planes; 128 x 128 (or 128 x 64 for narrow outputs) tile per
+//    256-thread workgroup, each wave a 4 x 4 (4 x 2) block of 16 x 16 MFMA
+//    tiles, 128-byte k-stages prefetched into registers and staged through
+//    padded LDS rows; grid z = u.
 //  * k_oz_combine<N>: acc(i,j) = sum_u C_u(i,j) 2^(ea_i + eb_j - 7(u+2)) in
 //    Float64xN arithmetic, u ascending, then C = alpha*acc + beta*C (or
 //    R = R - acc for a residual update).

@@ -537,6 +537,19 @@ def dpotrf_inverse_(A, Inv):
     return int(info.value)
 
 
+def dgetrf_(A, ipiv):
+    """Binary64 LU with partial pivoting of the square A's leading limb plane
+    in place (LAPACK dgetrf semantics); ipiv: int64 CUDA tensor of n entries,
+    one-based. Returns the status (0, -1 non-finite input, k > 0 first zero
+    pivot)."""
+    n = rows_cols(A)[0]
+    if ipiv.dtype != torch.int64 or not ipiv.is_cuda or ipiv.numel() < n:
+        raise ArgumentError("ipiv must be an int64 CUDA tensor of n entries")
+    info = ctypes.c_int64(0)
+    _raise(L.load().mfla_dgetrf(_handle(), _mat(A), ctypes.c_void_p(ipiv.data_ptr()), ctypes.byref(info)))
+    return int(info.value)
+
+
 # --------------------------------------------------------------- interop
 def from_interleaved(dst, src):
     """Copy a Julia-layout (AoS) Float64xN matrix -- a CUDA float64 tensor of

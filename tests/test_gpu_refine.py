@@ -129,6 +129,52 @@ def test_dpotrf_inverse_reports_failing_pivot():
     assert M.dpotrf_inverse_(A, Inv) == -1
 
 
+def _lu_check(A0, F, ipiv, pivots_like_lapack):
+    import scipy.linalg as sl
+
+    n = A0.shape[0]
+    PA = A0.copy()
+    for k in range(n):
+        p = ipiv[k] - 1
+        assert k <= p < n
+        PA[[k, p]] = PA[[p, k]]
+    Lf = np.tril(F, -1) + np.eye(n)
+    U = np.triu(F)
+    assert np.abs(np.tril(F, -1)).max(initial=0.0) <= 1.0
+    assert np.abs(PA - Lf @ U).max() <= 1e-15 * n * np.abs(A0).max() * 4
+    if pivots_like_lapack:
+        _, piv = sl.lu_factor(A0)
+        assert np.array_equal(ipiv - 1, piv)
+
+
+@pytest.mark.parametrize("n", [1, 5, 8, 17, 64, 65, 130, 300, 1100, 2100])
+def test_dgetrf_blocked_lu(n):
+    from mfla import api as M
+
+    rng = np.random.default_rng(100 + n)
+    A0 = rng.standard_normal((n, n))
+    A = torch.from_numpy(A0.T.copy()[None]).cuda()
+    ipiv = torch.zeros(n, dtype=torch.int64, device="cuda")
+    assert M.dgetrf_(A, ipiv) == 0
+    _lu_check(A0, _np(A)[0].T, ipiv.cpu().numpy(), pivots_like_lapack=n <= 300)
+
+
+def test_dgetrf_zero_pivot_and_nonfinite():
+    from mfla import api as M
+
+    n = 100
+    rng = np.random.default_rng(5)
+    A0 = rng.standard_normal((n, n))
+    A0[:, 37] = 0.0                                   # inside the third 16-column leaf
+    A = torch.from_numpy(A0.T.copy()[None]).cuda()
+    ipiv = torch.zeros(n, dtype=torch.int64, device="cuda")
+    assert M.dgetrf_(A, ipiv) == 38
+    assert ipiv.cpu().numpy()[37] == 38
+    A0[3, 3] = np.nan
+    A = torch.from_numpy(A0.T.copy()[None]).cuda()
+    assert M.dgetrf_(A, ipiv) == -1
+
+
 @pytest.mark.parametrize("kind,uplo,n,nrhs", [("cholesky", "general", 256, 4), ("cholesky", "lower", 200, 3),
                                               ("lu", "general", 192, 5)])
 @pytest.mark.parametrize("N", [2, 3, 4])
