@@ -765,6 +765,250 @@ template <int N> int arith_t(int op, int64_t count, const double *x, const doubl
 // ======================================================================
 // extern "C" surface for oracle/oracle.py (ctypes)
 // ======================================================================
+// ------------------------------------------------------------------- QR
+// src/factorizations/qr.jl. Column norms are kept as a scale-safe pair
+// (scale, scaled_sum) with norm = scale * sqrt(scaled_sum) (qr.jl:87-165).
+// eps(Float64xN) = eps(Float64)^N = 2^-52N (MultiFloats' definition), so the
+// pivot margin sqrt(eps) = 2^-26N and the downdate floor 16 eps = 2^(4-52N)
+// (qr.jl:721-722) are exact powers of two.
+template <int N> struct NormState { MF<N> scale, sum; };
+
+template <int N> static inline bool le(const MF<N> &x, const MF<N> &y) { return lt(x, y) || eq(x, y); }
+
+// qr.jl:96-122 (value_at(row) = A[row, col]) and qr.jl:229-262 (the delayed
+// variant: value_at(row) = A[row, col] - sum_prior A[row, bs+prior] F[prior, col]).
+template <int N, class Value>
+NormState<N> qr_norm_scan(int64_t first_row, int64_t m, Value value_at) {
+    const MF<N> one = mf_from<N>(1.0);
+    MF<N> scale = mf_zero<N>(), ssum = one;
+    bool seen = false;
+    for (int64_t row = first_row; row < m; ++row) {
+        const MF<N> v = mf_abs(value_at(row));
+        if (iszero(v)) continue;
+        if (!seen) { scale = v; ssum = one; seen = true; }
+        else if (gt(v, scale)) { const MF<N> r = mf_div(scale, v); ssum = add(one, mul(mul(ssum, r), r)); scale = v; }
+        else { const MF<N> r = mf_div(v, scale); ssum = add(ssum, mul(r, r)); }
+    }
+    return {scale, ssum};
+}
+
+template <int N> NormState<N> qr_col_state(int64_t m, View<N> A, int64_t first_row, int64_t col) {
+    return qr_norm_scan<N>(first_row, m, [&](int64_t r) { return A.get(r, col); });
+}
+
+template <int N> NormState<N> qr_delayed_state(int64_t m, View<N> A, View<N> F, int64_t bs, int64_t pc, int64_t first_row, int64_t col) {
+    return qr_norm_scan<N>(first_row, m, [&](int64_t r) {
+        MF<N> v = A.get(r, col);
+        for (int64_t p = 0; p < pc; ++p) v = sub(v, mul(A.get(r, bs + p), F.get(p, col)));
+        return v;
+    });
+}
+
+template <int N> MF<N> qr_norm_of(const NormState<N> &st) { return iszero(st.scale) ? mf_zero<N>() : mul(st.scale, mf_sqrt(st.sum)); }
+
+// qr.jl:606-622: Householder reflector of column step; returns tau.
+template <int N> MF<N> qr_make_reflector(int64_t m, View<N> A, int64_t step) {
+    const MF<N> nrm = qr_norm_of(qr_col_state(m, A, step, step));
+    if (iszero(nrm)) return mf_zero<N>();
+    const MF<N> alpha = A.get(step, step);
+    const MF<N> beta = ge(alpha, mf_zero<N>()) ? mf_neg(nrm) : nrm;
+    const MF<N> leading = sub(alpha, beta);
+    const MF<N> tau = mf_div(sub(beta, alpha), beta);
+    for (int64_t row = step + 1; row < m; ++row) A.set(row, step, mf_div(A.get(row, step), leading));
+    A.set(step, step, beta);
+    return tau;
+}
+
+// qr.jl:624-642 (and the apply_q! form qr.jl:890-929): column col of T gets
+// H = I - tau v v' with v = (1, V[step+1:m, vcol]).
+template <int N> void qr_reflect_column(int64_t m, View<N> V, int64_t vcol, int64_t step, const MF<N> &tau, View<N> T, int64_t col) {
+    MF<N> proj = T.get(step, col);
+    for (int64_t row = step + 1; row < m; ++row) proj = add(proj, mul(V.get(row, vcol), T.get(row, col)));
+    proj = mul(proj, tau);
+    T.set(step, col, sub(T.get(step, col), proj));
+    for (int64_t row = step + 1; row < m; ++row) T.set(row, col, sub(T.get(row, col), mul(V.get(row, vcol), proj)));
+}
+
+// qr.jl:644-669
+template <int N> void qr_apply_trailing(int64_t m, int64_t n, View<N> A, const MF<N> &tau, int64_t step) {
+    if (iszero(tau)) return;
+    for (int64_t c = step + 1; c < n; ++c) qr_reflect_column(m, A, step, step, tau, A, c);
+}
+
+template <int N> struct QrState {
+    std::vector<MF<N>> scale, sum;
+    std::vector<char> dirty;
+};
+
+// qr.jl:167-182
+template <int N> MF<N> qr_recompute(int64_t m, View<N> A, QrState<N> &S, int64_t first_row, int64_t col) {
+    const NormState<N> st = qr_col_state(m, A, first_row, col);
+    S.scale[col] = st.scale; S.sum[col] = st.sum; S.dirty[col] = 0;
+    return qr_norm_of(st);
+}
+
+template <int N> void qr_swap(int64_t m, View<N> A, int64_t *perm, QrState<N> &S, int64_t a, int64_t b) {
+    if (a == b) return;
+    for (int64_t r = 0; r < m; ++r) { const MF<N> t = A.get(r, a); A.set(r, a, A.get(r, b)); A.set(r, b, t); }
+    std::swap(perm[a], perm[b]);
+    std::swap(S.scale[a], S.scale[b]); std::swap(S.sum[a], S.sum[b]); std::swap(S.dirty[a], S.dirty[b]);
+}
+
+// qr.jl:264-301 (F empty: pc = 0)
+template <int N> void qr_update_norms(int64_t m, int64_t n, View<N> A, const MF<N> &tau, int64_t step, QrState<N> &S, const MF<N> &floor_,
+                                      View<N> F, int64_t bs, int64_t pc) {
+    if (iszero(tau)) return;
+    for (int64_t c = step + 1; c < n; ++c) {
+        if (S.dirty[c]) continue;
+        const MF<N> cs = S.scale[c], removed = mf_abs(A.get(step, c));
+        if (iszero(cs)) { S.dirty[c] = !iszero(removed); continue; }
+        const MF<N> ratio = mf_div(removed, cs);
+        const MF<N> ns = sub(S.sum[c], mul(ratio, ratio));
+        if (!isfin(ns) || le(ns, floor_)) {
+            const NormState<N> st = pc > 0 ? qr_delayed_state(m, A, F, bs, pc, step + 1, c) : qr_col_state(m, A, step + 1, c);
+            S.scale[c] = st.scale; S.sum[c] = st.sum; S.dirty[c] = 0;
+        } else {
+            S.sum[c] = ns;
+        }
+    }
+}
+
+// qr.jl:763-799: unpivoted Householder QR; tau has min(m, n) entries.
+template <int N> void geqrf_t(int64_t m, int64_t n, View<N> A, View<N> tau) {
+    const int64_t k = std::min(m, n);
+    for (int64_t step = 0; step < k; ++step) {
+        const MF<N> t = qr_make_reflector(m, A, step);
+        tau.set(step, 0, t);
+        qr_apply_trailing(m, n, A, t, step);
+    }
+}
+
+// qr.jl:192-224
+template <int N> int64_t qr_pivot_hybrid(int64_t m, int64_t n, View<N> A, const int64_t *perm, int64_t step, QrState<N> &S, const MF<N> &onem) {
+    int64_t pivot = step;
+    MF<N> pn = qr_recompute(m, A, S, step, step);
+    for (int64_t c = step + 1; c < n; ++c) {
+        MF<N> cand = qr_norm_of(NormState<N>{S.scale[c], S.sum[c]});
+        if (S.dirty[c] || iszero(pn) || ge(cand, mul(onem, pn))) cand = qr_recompute(m, A, S, step, c);
+        if (gt(cand, pn) || (eq(cand, pn) && perm[c] < perm[pivot])) { pivot = c; pn = cand; }
+    }
+    return pivot;
+}
+
+// qr.jl:344-362
+template <int N> int64_t qr_pivot_downdated(int64_t n, const int64_t *perm, int64_t step, const QrState<N> &S) {
+    int64_t pivot = step;
+    MF<N> pn = qr_norm_of(NormState<N>{S.scale[step], S.sum[step]});
+    for (int64_t c = step + 1; c < n; ++c) {
+        const MF<N> cand = qr_norm_of(NormState<N>{S.scale[c], S.sum[c]});
+        if (gt(cand, pn) || (eq(cand, pn) && perm[c] < perm[pivot])) { pivot = c; pn = cand; }
+    }
+    return pivot;
+}
+
+// qr.jl:692-743 with the unblocked route (qr.jl:364-398) and the blocked
+// DLAQPS route (qr.jl:450-604, blocked when min(m, n) >= 16 and m*n >= 16384,
+// qr.jl:303-314). perm is one-based. Returns 0 or -1 (non-finite input: A,
+// tau and perm untouched apart from tau = 0, perm = identity).
+template <int N> int64_t geqp3_t(int64_t m, int64_t n, View<N> A, View<N> tau, int64_t *perm, int route) {
+    const int64_t k = std::min(m, n);
+    for (int64_t c = 0; c < n; ++c) perm[c] = c + 1;
+    for (int64_t s = 0; s < k; ++s) tau.set(s, 0, mf_zero<N>());
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < m; ++i)
+            if (!isfin(A.get(i, j))) return -1;
+    QrState<N> S;
+    S.scale.resize(n); S.sum.resize(n); S.dirty.assign(n, 0);
+    for (int64_t c = 0; c < n; ++c) { const NormState<N> st = qr_col_state(m, A, 0, c); S.scale[c] = st.scale; S.sum[c] = st.sum; }
+    const MF<N> one = mf_from<N>(1.0), mone = mf_from<N>(-1.0);
+    const MF<N> onem = sub(one, mf_from<N>(std::ldexp(1.0, -26 * N)));
+    const MF<N> floor_ = mf_from<N>(std::ldexp(1.0, 4 - 52 * N));
+    const bool blocked = route == 2 || (route == 0 && k >= 16 && m * n >= 16384);
+    if (!blocked) {
+        View<N> none{nullptr, 0, 0};
+        for (int64_t step = 0; step < k; ++step) {
+            const int64_t pivot = qr_pivot_hybrid(m, n, A, perm, step, S, onem);
+            qr_swap(m, A, perm, S, step, pivot);
+            const MF<N> t = qr_make_reflector(m, A, step);
+            tau.set(step, 0, t);
+            qr_apply_trailing(m, n, A, t, step);
+            qr_update_norms(m, n, A, t, step, S, floor_, none, step, 0);
+        }
+        return 0;
+    }
+    const int64_t bsz = std::min<int64_t>(16, k);
+    std::vector<double> Fbuf((size_t)N * bsz * n), auxbuf((size_t)N * bsz), ybuf((size_t)N * n);
+    View<N> F{Fbuf.data(), bsz, bsz * n}, aux{auxbuf.data(), bsz, bsz};
+    int64_t bs = 0;
+    while (bs < k) {
+        const int64_t requested = std::min(bsz, k - bs);
+        std::fill(Fbuf.begin(), Fbuf.end(), 0.0);
+        int64_t actual = 0;
+        for (int64_t ls = 0; ls < requested; ++ls) {
+            const int64_t step = bs + ls;
+            const int64_t pivot = qr_pivot_downdated(n, perm, step, S);
+            if (pivot != step) {
+                for (int64_t r = 0; r < m; ++r) { const MF<N> t = A.get(r, step); A.set(r, step, A.get(r, pivot)); A.set(r, pivot, t); }
+                for (int64_t p = 0; p < ls; ++p) { const MF<N> t = F.get(p, step); F.set(p, step, F.get(p, pivot)); F.set(p, pivot, t); }
+                std::swap(perm[step], perm[pivot]);
+                std::swap(S.scale[step], S.scale[pivot]); std::swap(S.sum[step], S.sum[pivot]); std::swap(S.dirty[step], S.dirty[pivot]);
+            }
+            if (ls > 0)
+                for (int64_t r = step; r < m; ++r) {
+                    MF<N> corr = mf_zero<N>();
+                    for (int64_t p = 0; p < ls; ++p) corr = add(corr, mul(A.get(r, bs + p), F.get(p, step)));
+                    A.set(r, step, sub(A.get(r, step), corr));
+                }
+            const MF<N> t = qr_make_reflector(m, A, step);
+            tau.set(step, 0, t);
+            const MF<N> diagonal = A.get(step, step);
+            A.set(step, step, one);
+            if (step < n - 1) {   // F[ls, step+1:n] = tau * A[step:m, step+1:n]' v   (gemv!, trans = :T)
+                const int64_t cols = n - step - 1;
+                View<N> y{ybuf.data(), 1, cols};
+                gemv_t<N>(1, m - step, cols, A.sub(step, step + 1), A.sub(step, step), y, t, mf_zero<N>());
+                for (int64_t o = 0; o < cols; ++o) F.set(ls, step + 1 + o, y.get(o, 0));
+            }
+            if (ls > 0) {
+                gemv_t<N>(1, m - step, ls, A.sub(step, bs), A.sub(step, step), View<N>{aux.p, 1, bsz}, mf_neg(t), mf_zero<N>());
+                for (int64_t c = step + 1; c < n; ++c) {
+                    MF<N> corr = mf_zero<N>();
+                    for (int64_t p = 0; p < ls; ++p) corr = add(corr, mul(F.get(p, c), aux.get(p, 0)));
+                    F.set(ls, c, add(F.get(ls, c), corr));
+                }
+            }
+            for (int64_t c = step + 1; c < n; ++c) {
+                MF<N> corr = mf_zero<N>();
+                for (int64_t p = 0; p <= ls; ++p) corr = add(corr, mul(A.get(step, bs + p), F.get(p, c)));
+                A.set(step, c, sub(A.get(step, c), corr));
+            }
+            A.set(step, step, diagonal);
+            qr_update_norms(m, n, A, t, step, S, floor_, F, bs, ls + 1);
+            actual = ls + 1;
+            bool unreliable = false;
+            for (int64_t c = step + 1; c < n; ++c) unreliable = unreliable || S.dirty[c];
+            if (unreliable) break;
+        }
+        const int64_t be = bs + actual - 1;
+        if (be + 1 < m && be + 1 < n)
+            gemm_t<N>(m - be - 1, n - be - 1, actual, A.sub(be + 1, bs), View<N>{F.p + (be + 1) * bsz, bsz, bsz * n}, A.sub(be + 1, be + 1), mone, one,
+                      true, 1);
+        for (int64_t c = be + 1; c < n; ++c) qr_recompute(m, A, S, be + 1, c);
+        bs = be + 1;
+    }
+    return 0;
+}
+
+// qr.jl:938-956: B <- Q B (trans = 0, reflectors in reverse) or Q' B.
+template <int N> void ormqr_t(int trans, int64_t m, int64_t k, View<N> Fq, View<N> tau, int64_t nrhs, View<N> B) {
+    for (int64_t i = 0; i < k; ++i) {
+        const int64_t step = trans ? i : k - 1 - i;
+        const MF<N> t = tau.get(step, 0);
+        if (iszero(t)) continue;
+        for (int64_t c = 0; c < nrhs; ++c) qr_reflect_column(m, Fq, step, step, t, B, c);
+    }
+}
+
 #define DISPATCH(N, ...) \
     switch (N) { case 1: { constexpr int NN = 1; __VA_ARGS__; } break; case 2: { constexpr int NN = 2; __VA_ARGS__; } break; \
                  case 3: { constexpr int NN = 3; __VA_ARGS__; } break; case 4: { constexpr int NN = 4; __VA_ARGS__; } break; default: return -100; }
@@ -893,5 +1137,22 @@ int orc_residual_mixed(int NS, int NR, int uplo, int64_t m, int64_t n, int64_t n
 }
 
 int orc_threads(void) { return nthreads_or(0); }
+
+int orc_geqrf(int N, int64_t m, int64_t n, double *A, int64_t lda, int64_t lsa, double *tau, int64_t lst) {
+    DISPATCH(N, geqrf_t<NN>(m, n, View<NN>{A, lda, lsa}, View<NN>{tau, 1, lst}));
+    return 0;
+}
+
+// route: 0 MFLA's choice, 1 unblocked, 2 blocked
+int64_t orc_geqp3(int N, int64_t m, int64_t n, double *A, int64_t lda, int64_t lsa, double *tau, int64_t lst, int64_t *perm, int route) {
+    DISPATCH(N, return geqp3_t<NN>(m, n, View<NN>{A, lda, lsa}, View<NN>{tau, 1, lst}, perm, route));
+    return 0;
+}
+
+int orc_ormqr(int N, int trans, int64_t m, int64_t k, double *F, int64_t ldf, int64_t lsf, double *tau, int64_t lst, int64_t nrhs, double *B,
+              int64_t ldb, int64_t lsb) {
+    DISPATCH(N, ormqr_t<NN>(trans, m, k, View<NN>{F, ldf, lsf}, View<NN>{tau, 1, lst}, nrhs, View<NN>{B, ldb, lsb}));
+    return 0;
+}
 
 }  // extern "C"

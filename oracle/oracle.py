@@ -23,6 +23,12 @@ Parity pinning per function (what checks the restatement itself):
 * x2 add/mul, x4 mul, div, inv, sqrt -- PARITY UNPINNED against MultiFloats.jl
   bits (the library is not in the reference tree); pinned by mpmath accuracy,
   exactness on representable cases and closed-form identities only.
+* geqrf / geqp3 / ormqr (qr!, rrqr!, apply_q!) -- pinned by the reference's
+  rank / permutation known-answer tests (test/runtests.jl:753-930,
+  test/qr_delayed_norm.jl), by LAPACK's float64 dgeqrf / dgeqp3 (same
+  reflector convention: factors, tau and pivot order agree to float64
+  accuracy on well-separated norms) and by exact orthogonality / QR = AP
+  residuals; blocked and unblocked routes agree on ranks.
 * gemm / lower_update / trsm / gemv / symv / potrf / getrf / sytrf / solves /
   residuals / backward error -- pinned by the reference's known-answer tests
   (test/adversarial.jl, test/runtests.jl) re-created in tests/, by the exact
@@ -64,6 +70,7 @@ def lib():
         _lib.orc_getrf.restype = c_i64
         _lib.orc_sytrf.restype = c_i64
         _lib.orc_sytrs.restype = c_i64
+        _lib.orc_geqp3.restype = c_i64
         _lib.orc_exact_sum_diff.restype = ctypes.c_double
     return _lib
 
@@ -223,6 +230,34 @@ def getrf(A, block=16, fused=True, threads=0):
     info = lib().orc_getrf(c_int(N), c_i64(m), c_i64(n), *_mat(A), c_i64(block), ipiv.ctypes.data_as(c_i64p), _ptr(om),
                            c_int(1 if fused else 0), c_int(threads))
     return int(info), ipiv, om
+
+
+def geqrf(A):
+    """qr! (src/factorizations/qr.jl:763-799) in place; returns tau (N, k)."""
+    N, n, m = A.shape
+    tau = np.zeros((N, min(m, n)))
+    lib().orc_geqrf(c_int(N), c_i64(m), c_i64(n), *_mat(A), _ptr(tau), c_i64(tau.shape[1]))
+    return tau
+
+
+def geqp3(A, route="auto"):
+    """rrqr! (src/factorizations/qr.jl:692-743) in place. route: "auto" (MFLA's
+    choice), "unblocked" or "blocked". Returns (info, tau (N, k), perm one-based)."""
+    N, n, m = A.shape
+    tau = np.zeros((N, min(m, n)))
+    perm = np.zeros(n, dtype=np.int64)
+    r = {"auto": 0, "unblocked": 1, "blocked": 2}[route]
+    info = lib().orc_geqp3(c_int(N), c_i64(m), c_i64(n), *_mat(A), _ptr(tau), c_i64(tau.shape[1]), perm.ctypes.data_as(c_i64p), c_int(r))
+    return int(info), tau, perm
+
+
+def ormqr(F, tau, B, trans="N"):
+    """apply_q! (src/factorizations/qr.jl:938-956): B <- Q B or Q' B in place."""
+    N, _, m = F.shape
+    B3 = _rhs3(B)
+    lib().orc_ormqr(c_int(N), c_int(1 if trans == "T" else 0), c_i64(m), c_i64(tau.shape[1]), *_mat(F), _ptr(tau), c_i64(tau.shape[1]),
+                    c_i64(B3.shape[1]), *_mat(B3))
+    return B
 
 
 def sytrf(A, strategy="auto", block=0, crossover=512, threads=0):
