@@ -192,6 +192,20 @@ int mfla_getrf(mfla_handle h, int limbs, mfla_matrix A, int64_t *ipiv_dev, doubl
 int mfla_sytrf(mfla_handle h, int limbs, mfla_matrix A, mfla_matrix dsub, int64_t *pivots_dev, uint8_t *blocks_dev,
                double *original_maximum, int64_t *info, const mfla_config *config, mfla_workspace ws);
 
+/* Householder QR. tau is a min(m, n) x 1 device matrix, perm_dev n int64
+ * (one-based, A_original[:, perm] = Q R). Non-finite input returns info = -1
+ * with tau = 0 and the identity permutation; A is then untouched.
+ * qr! (src/factorizations/qr.jl:763-799), no pivoting; the fixed-order form
+ * qr!(A, permutation) (qr.jl:813-853) reorders the columns first. */
+int mfla_geqrf(mfla_handle h, int limbs, mfla_matrix A, mfla_matrix tau, int64_t *info);
+/* rrqr! (src/factorizations/qr.jl:692-743): route 0 = MFLA's choice (the
+ * delayed-norm panel route of qr.jl:450-604 when min(m, n) >= 16 and
+ * m n >= 16384, else qr.jl:364-398), 1 = unblocked, 2 = blocked */
+int mfla_geqp3(mfla_handle h, int limbs, mfla_matrix A, mfla_matrix tau, int64_t *perm_dev, int route, int64_t *info);
+/* apply_q! (src/factorizations/qr.jl:938-956): B <- Q B (MFLA_NOTRANS) or
+ * Q' B (MFLA_TRANS); B has the factor's row count and must not alias it */
+int mfla_ormqr(mfla_handle h, int limbs, int trans, mfla_matrix F, mfla_matrix tau, mfla_matrix B);
+
 /* ------------------------------------------------------------ solves */
 /* ldiv!(B, F::MFCholesky) (src/solve.jl:23-83) */
 int mfla_potrs(mfla_handle h, int limbs, mfla_matrix F, int64_t info, mfla_matrix B, const mfla_config *config);

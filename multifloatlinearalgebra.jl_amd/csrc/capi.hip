@@ -21,6 +21,7 @@
 #include "k_factor.hpp"
 #include "k_level3x.hpp"
 #include "k_ozaki.hpp"
+#include "k_qr.hpp"
 #include "k_refine.hpp"
 #include "k_trsm.hpp"
 #include "mf_arith.hpp"
@@ -1653,6 +1654,152 @@ int mfla_syrk_packed(mfla_handle h, int N, mfla_matrix packed, mfla_matrix P, co
     const mfla_matrix Pr{P.data + (reduction_first - 1), reduction_last - reduction_first + 1, n, P.ld, P.limb_stride};
     HIPCHK(lower_launch(h, N, c.gemm_tile, n, Pr.rows, dvt(Pr), dvn(Pr), D, alpha ? alpha : kOne, be, nullptr));
     HIPCHK(mfk::pack_lower(N, s, n, D, packed.data, packed.limb_stride, 0));
+    return MFLA_OK;
+}
+
+}  // extern "C"
+
+// ================================================================== QR
+namespace {
+
+// Device state of one QR call, carved from the handle's scratch.
+int qr_state(mfla_handle h, int N, int64_t n, mfk::QrState &q) {
+    const size_t vec = (size_t)N * n, dbl = 4 * vec + (size_t)N * mfk::QR_PANEL * n + (size_t)N * mfk::QR_PANEL + N;
+    const size_t bytes = sizeof(double) * dbl + sizeof(int) * 2 * (size_t)n + 64 + 64;
+    char *base = (char *)scratch(h, bytes);
+    if (!base) return fail(MFLA_ERR_HIP, "scratch allocation failed (%zu bytes)", bytes);
+    double *d = (double *)base;
+    q.scale = d; q.sum = d + vec; q.tscale = d + 2 * vec; q.tsum = d + 3 * vec;
+    q.F = d + 4 * vec; q.aux = q.F + (size_t)N * mfk::QR_PANEL * n; q.diag = q.aux + (size_t)N * mfk::QR_PANEL;
+    char *tail = (char *)(q.diag + N);
+    q.pivot = (int64_t *)tail;
+    q.unreliable = (int *)(tail + 8);
+    q.dirty = (int *)(tail + 64);
+    q.tflag = q.dirty + n;
+    return MFLA_OK;
+}
+
+int qr_check(int N, const mfla_matrix &A, const mfla_matrix &tau) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(A, "A") || check_desc(tau, "tau")) return MFLA_ERR_ARGUMENT;
+    if (tau.rows != std::min(A.rows, A.cols) || tau.cols != 1) return fail(MFLA_ERR_DIMENSION, "tau must hold min(m, n) reflectors");
+    if (extents_overlap(A, tau, N)) return fail(MFLA_ERR_ALIAS, "tau must not alias the matrix");
+    return MFLA_OK;
+}
+
+// Non-finite input (qr.jl:710-720): status -1, tau = 0, identity permutation.
+int qr_finite(mfla_handle h, int N, const mfla_matrix &A, const mfla_matrix &tau, int64_t *perm, int64_t *info) {
+    hipStream_t s = h->stream;
+    HIPCHK(mfk::status_reset(s, h->status));
+    HIPCHK(mfk::zero_fill(N, s, tau.rows, 1, dvn(tau)));
+    if (perm) HIPCHK(mfk::iota1(s, perm, A.cols));
+    HIPCHK(mfk::finite_check(N, s, A.rows, A.cols, dvn(A), 0, h->status));
+    const int64_t r = read_info(h);
+    if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+    *info = r;
+    return MFLA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfla_geqrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix tau, int64_t *info) {
+    if (int rc = qr_check(N, A, tau)) return rc;
+    const int64_t m = A.rows, n = A.cols, k = std::min(m, n);
+    std::lock_guard<std::mutex> g(h->lock);
+    if (int rc = qr_finite(h, N, A, tau, nullptr, info)) return rc;
+    if (*info != 0 || k == 0) return MFLA_OK;
+    mfk::QrState q;
+    if (int rc = qr_state(h, N, n, q)) return rc;
+    hipStream_t s = h->stream;
+    const DV a = dvn(A), t = dvn(tau);
+    for (int64_t step = 0; step < k; ++step) {   // qr.jl:789-792
+        HIPCHK(mfk::qr_reflector(N, s, m, step, a, t, 0, q));
+        HIPCHK(mfk::qr_apply(N, s, m, step, a, step, a, step + 1, n - step - 1, t));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return MFLA_OK;
+}
+
+int mfla_geqp3(mfla_handle h, int N, mfla_matrix A, mfla_matrix tau, int64_t *perm, int route, int64_t *info) {
+    if (int rc = qr_check(N, A, tau)) return rc;
+    if (route < 0 || route > 2) return fail(MFLA_ERR_ARGUMENT, "route must be 0 (auto), 1 (unblocked) or 2 (blocked)");
+    const int64_t m = A.rows, n = A.cols, k = std::min(m, n);
+    std::lock_guard<std::mutex> g(h->lock);
+    if (int rc = qr_finite(h, N, A, tau, perm, info)) return rc;
+    if (*info != 0 || k == 0) return MFLA_OK;
+    mfk::QrState q;
+    if (int rc = qr_state(h, N, n, q)) return rc;
+    hipStream_t s = h->stream;
+    const DV a = dvn(A), t = dvn(tau);
+    HIPCHK(mfk::qr_norms(N, s, m, n, a, 0, 0, n, q));
+    // qr.jl:303-314: the DLAQPS panel route once there are 16 reflectors and
+    // m n >= 16384
+    const bool blocked = route == 2 || (route == 0 && k >= mfk::QR_PANEL && m * n >= 16384);
+    if (!blocked) {
+        for (int64_t step = 0; step < k; ++step) {   // qr.jl:364-398
+            HIPCHK(mfk::qr_pivot_hybrid(N, s, m, n, step, a, perm, q));
+            HIPCHK(mfk::qr_swap(N, s, m, n, step, 0, a, perm, q));
+            HIPCHK(mfk::qr_reflector(N, s, m, step, a, t, 0, q));
+            HIPCHK(mfk::qr_apply(N, s, m, step, a, step, a, step + 1, n - step - 1, t));
+            HIPCHK(mfk::qr_norm_update(N, s, m, n, step, a, t, step, 0, 0, q));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        return MFLA_OK;
+    }
+    const int64_t bsz = std::min<int64_t>(mfk::QR_PANEL, k);
+    const DV F{q.F, 1, mfk::QR_PANEL, (int64_t)mfk::QR_PANEL * n};
+    int unreliable = 0;
+    for (int64_t bs = 0; bs < k;) {   // qr.jl:450-604
+        const int64_t requested = std::min(bsz, k - bs);
+        HIPCHK(mfk::qr_panel_reset(N, s, n, q));
+        int64_t actual = 0;
+        for (int64_t ls = 0; ls < requested; ++ls) {
+            const int64_t step = bs + ls;
+            HIPCHK(mfk::qr_pivot_downdated(N, s, n, step, perm, q));
+            HIPCHK(mfk::qr_swap(N, s, m, n, step, ls, a, perm, q));
+            HIPCHK(mfk::qr_delayed_column(N, s, m, n, step, bs, ls, a, q));
+            HIPCHK(mfk::qr_reflector(N, s, m, step, a, t, 1, q));
+            HIPCHK(mfk::qr_aux(N, s, m, n, step, bs, ls, a, t, q));
+            HIPCHK(mfk::qr_frow(N, s, m, n, step, bs, ls, a, t, q));
+            HIPCHK(mfk::qr_norm_update(N, s, m, n, step, a, t, bs, ls + 1, 1, q));
+            actual = ls + 1;
+            // a cancellation-sensitive norm ends the panel (qr.jl:588-592)
+            HIPCHK(hipMemcpyAsync(&unreliable, q.unreliable, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (unreliable) break;
+        }
+        const int64_t be = bs + actual - 1;
+        if (be + 1 < m && be + 1 < n) {   // A22 -= A21 F(0:actual, be+1:n)
+            const DV Fv{q.F + (be + 1) * mfk::QR_PANEL, 1, mfk::QR_PANEL, (int64_t)mfk::QR_PANEL * n};
+            HIPCHK(gemm_launch(h, N, MFLA_GEMM_AUTO, -1, m - be - 1, n - be - 1, actual, sub(a, be + 1, bs), Fv, sub(a, be + 1, be + 1),
+                               kMinusOne, kOne, h->status));
+        }
+        HIPCHK(mfk::qr_norms(N, s, m, n, a, be + 1, be + 1, n - be - 1, q));
+        bs = be + 1;
+    }
+    (void)F;
+    HIPCHK(hipStreamSynchronize(s));
+    return MFLA_OK;
+}
+
+int mfla_ormqr(mfla_handle h, int N, int trans, mfla_matrix F, mfla_matrix tau, mfla_matrix B) {
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (check_desc(F, "F") || check_desc(tau, "tau") || check_desc(B, "B")) return MFLA_ERR_ARGUMENT;
+    if (trans != MFLA_NOTRANS && trans != MFLA_TRANS) return fail(MFLA_ERR_ARGUMENT, "trans must be :N or :T");
+    if (B.rows != F.rows) return fail(MFLA_ERR_DIMENSION, "apply_q! destination row count differs");
+    if (tau.rows != std::min(F.rows, F.cols) || tau.cols != 1) return fail(MFLA_ERR_DIMENSION, "tau must hold min(m, n) reflectors");
+    if (extents_overlap(B, F, N) || extents_overlap(B, tau, N))
+        return fail(MFLA_ERR_ALIAS, "apply_q! destination must not alias QR storage");
+    const int64_t m = F.rows, k = tau.rows, nrhs = B.cols;
+    if (nrhs == 0 || k == 0) return MFLA_OK;
+    std::lock_guard<std::mutex> g(h->lock);
+    hipStream_t s = h->stream;
+    for (int64_t i = 0; i < k; ++i) {   // qr.jl:951-954
+        const int64_t step = trans == MFLA_TRANS ? i : k - 1 - i;
+        HIPCHK(mfk::qr_apply(N, s, m, step, dvn(F), step, dvn(B), 0, nrhs, dvn(tau)));
+    }
     return MFLA_OK;
 }
 

@@ -103,6 +103,9 @@ PROTOTYPES = {
     "mfla_dgemm": (_I, [_P, _I, _I, _I, _I, _M, _M, _M, ctypes.c_double, ctypes.c_double]),
     "mfla_dpotrf_inverse": (_I, [_P, _M, _M, ctypes.POINTER(_I64)]),
     "mfla_dgetrf": (_I, [_P, _M, _P, ctypes.POINTER(_I64)]),
+    "mfla_geqrf": (_I, [_P, _I, _M, _M, ctypes.POINTER(_I64)]),
+    "mfla_geqp3": (_I, [_P, _I, _M, _M, _P, _I, ctypes.POINTER(_I64)]),
+    "mfla_ormqr": (_I, [_P, _I, _I, _M, _M, _M]),
     "mfla_trmm": (_I, [_P, _I, _I, _I, _I, _I, _M, _M, _DP, _CP]),
     "mfla_dot": (_I, [_P, _I, _M, _M, _DP]),
     "mfla_syrk_packed": (_I, [_P, _I, _M, _M, _DP, _DP, _I64, _I64, _CP]),

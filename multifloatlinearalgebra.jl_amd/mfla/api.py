@@ -374,6 +374,128 @@ class MFLDLT(AbstractMFFactorization):
     kind = "ldlt"
 
 
+@dataclass
+class MFQR(AbstractMFFactorization):
+    factors: torch.Tensor
+    tau: torch.Tensor              # (N, min(m, n)) Householder scalars
+    permutation: torch.Tensor      # int64, one-based: A_original[:, p] = Q R
+    permutation_cycle_leaders: list
+    info: int
+    kind = "qr"
+
+
+def _cycle_leaders(perm):
+    """qr.jl:43-61: first index of every permutation cycle longer than one."""
+    p = [int(v) for v in perm]
+    seen = [False] * len(p)
+    leaders = []
+    for start in range(len(p)):
+        if seen[start]:
+            continue
+        cur, length = start, 0
+        while not seen[cur]:
+            seen[cur] = True
+            cur = p[cur] - 1
+            length += 1
+        if length > 1:
+            leaders.append(start + 1)
+    return leaders
+
+
+def _qr_finish(A, tau, perm, info, check, name):
+    if check and info == -1:
+        raise DomainError(f"{name}: input matrix contains non-finite entries")
+    return MFQR(A, tau, perm, _cycle_leaders(perm.cpu().numpy()), info)
+
+
+_QR_ROUTE = {"auto": 0, "unblocked": 1, "blocked": 2}
+
+
+def rrqr_(A, *, check=True, workspace=None, route="auto"):
+    """rrqr! (src/factorizations/qr.jl:692-743): deterministic column-pivoted
+    Householder QR in place; A_original[:, p] = Q R with p =
+    factor_permutation(F). route: "auto" (MFLA's choice: the blocked DLAQPS
+    route when min(m, n) >= 16 and m n >= 16384), "unblocked" or "blocked".
+    The workspace argument is accepted for API parity (results do not depend
+    on it)."""
+    N = _limbs(A)
+    m, n = rows_cols(A)
+    r = _sym(route, _QR_ROUTE, "route must be :auto, :unblocked, or :blocked")
+    tau = torch.zeros((N, min(m, n)), dtype=torch.float64, device=A.device)
+    perm = torch.arange(1, n + 1, dtype=torch.int64, device=A.device)
+    info = ctypes.c_int64(0)
+    _raise(L.load().mfla_geqp3(_handle(), N, _mat(A), _mat(tau), ctypes.c_void_p(perm.data_ptr()), r, ctypes.byref(info)))
+    return _qr_finish(A, tau, perm, int(info.value), check, "rrqr!")
+
+
+def qr_(A, permutation=None, *, check=True, workspace=None):
+    """qr! (src/factorizations/qr.jl:763-853): Householder QR without
+    pivoting, or in the caller's column order (permutation[i] = original
+    column of factor column i; the columns of A are reordered in place)."""
+    N = _limbs(A)
+    m, n = rows_cols(A)
+    if permutation is not None:
+        perm_h = np.asarray([int(v) for v in permutation], dtype=np.int64)
+        if perm_h.shape[0] != n:
+            raise DimensionMismatch(f"permutation length {perm_h.shape[0]} does not match {n} columns")
+        seen = np.zeros(n, dtype=bool)
+        for c in perm_h:
+            if not 1 <= c <= n:
+                raise ArgumentError(f"permutation entry {c} is outside 1:{n}")
+            if seen[c - 1]:
+                raise ArgumentError(f"permutation repeats column {c}")
+            seen[c - 1] = True
+        idx = torch.from_numpy(perm_h - 1).to(A.device)
+        A.copy_(A.index_select(1, idx))
+    tau = torch.zeros((N, min(m, n)), dtype=torch.float64, device=A.device)
+    info = ctypes.c_int64(0)
+    _raise(L.load().mfla_geqrf(_handle(), N, _mat(A), _mat(tau), ctypes.byref(info)))
+    if permutation is not None:
+        perm = torch.from_numpy(perm_h).to(A.device)
+    else:
+        perm = torch.arange(1, n + 1, dtype=torch.int64, device=A.device)
+    return _qr_finish(A, tau, perm, int(info.value), check, "qr!")
+
+
+def apply_q_(B, F, *, trans="N"):
+    """apply_q! (src/factorizations/qr.jl:938-956): B <- Q B (trans = "N") or
+    Q' B ("T") from the compact reflectors; B is a vector (N, m) or a matrix."""
+    if not isinstance(F, MFQR) or F.info != 0:
+        raise ArgumentError("apply_q! requires a successful QR factorization")
+    t = _sym(trans, {"N": 0, "T": 1}, "trans must be :N or :T")
+    N = _limbs(B, F.factors)
+    X = _as_matrix(B)
+    _raise(L.load().mfla_ormqr(_handle(), N, t, _mat(F.factors), _mat(F.tau), _mat(X)))
+    return B
+
+
+def solve_r_(B, F, rank, *, trans="N", config=None):
+    """solve_r! (src/factorizations/qr.jl:958-1014): solve the leading
+    rank x rank block of R (or its transpose) in place; B has rank rows."""
+    if not isinstance(F, MFQR) or F.info != 0:
+        raise ArgumentError("solve_r! requires a successful QR factorization")
+    if trans not in ("N", "T"):
+        raise ArgumentError("trans must be :N or :T")
+    rank = int(rank)
+    kmax = min(rows_cols(F.factors))
+    if not 0 <= rank <= kmax:
+        raise ArgumentError(f"rank must be between zero and {kmax}")
+    if rows_cols(_as_matrix(B))[0] != rank:
+        raise DimensionMismatch("solve_r! destination row count must equal rank")
+    if rank == 0:
+        return B
+    d = torch.diagonal(F.factors, dim1=1, dim2=2)[:, :rank]
+    zero = torch.all(d == 0, dim=0).cpu().numpy()
+    if zero.any():
+        raise SingularException(int(np.flatnonzero(zero)[0]) + 1)
+    lead = F.factors[:, :rank, :rank]
+    if B.dim() == 2:
+        trsv_(B, lead, uplo="upper", trans=trans, diag="nonunit", config=config)
+    else:
+        trsm_(B, lead, side="left", uplo="upper", trans=trans, diag="nonunit", config=config)
+    return B
+
+
 def cholesky_(A, *, check=True, config=None):
     """cholesky! (src/factorizations/cholesky.jl:68-107)."""
     N = _limbs(A)

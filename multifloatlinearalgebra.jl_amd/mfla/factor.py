@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .api import (MFLU, MFLDLT, AbstractMFFactorization, ArgumentError, MFCholesky, _handle, _mat, _raise, host_arith, rows_cols)
+from .api import (MFLU, MFLDLT, MFQR, AbstractMFFactorization, ArgumentError, MFCholesky, _handle, _mat, _raise, host_arith, rows_cols)
 
 
 class _H:
@@ -77,8 +77,16 @@ def factor_pivots(F: MFLDLT): return F.pivots.cpu().numpy().copy()
 def factor_blocks(F: MFLDLT): return F.blocks.cpu().numpy().copy()
 
 
-def factor_permutation(F: MFLDLT):
-    """ldlt.jl:41-63: p with A_original[p, p] = L*D*L' (one-based)."""
+def factor_rdiag(F: MFQR):
+    """qr.jl:73-85: copy of the signed diagonal of R, shape (N, min(m, n))."""
+    return _diag(F)
+
+
+def factor_permutation(F):
+    """ldlt.jl:41-63: p with A_original[p, p] = L*D*L' (one-based); for a QR
+    factor (qr.jl:63-71) the column permutation with A_original[:, p] = Q R."""
+    if isinstance(F, MFQR):
+        return F.permutation.cpu().numpy().copy()
     blocks = factor_blocks(F)
     piv = factor_pivots(F)
     n = len(blocks)
@@ -215,8 +223,8 @@ def factor_inertia(F: MFLDLT):
     return {"positive": pos, "negative": neg, "zero": zer}
 
 
-def factor_diagnostics(F: AbstractMFFactorization):
-    """diagnostics.jl:32-246."""
+def factor_diagnostics(F: AbstractMFFactorization, *, atol=0.0, rtol=0.0):
+    """diagnostics.jl:32-275 (atol / rtol: the QR rank thresholds)."""
     N = F.factors.shape[0]
     h = _H(N)
     base = {
@@ -224,6 +232,16 @@ def factor_diagnostics(F: AbstractMFFactorization):
         "precision": factor_precision(F), "provider": factor_provider(F),
         "failure_location": F.info if F.info > 0 else None,
     }
+    if isinstance(F, MFQR):
+        diag = _diag(F)
+        k = diag.shape[1]
+        mn, mx = _magnitude_range(h, diag, k) if k else (None, None)
+        tau_ok = bool(torch.isfinite(F.tau).all().item())
+        base.update(failure_location=None, permutation=factor_permutation(F), rdiag=diag, minimum_rdiag=mn, maximum_rdiag=mx,
+                    rdiag_spread=_optional_spread(h, mn, mx),
+                    rank_at_threshold=numerical_rank(F, atol=atol, rtol=rtol) if issuccess(F) else 0,
+                    atol=atol, rtol=rtol, finite=_finite(F.factors, 0) and tau_ok)
+        return base
     if isinstance(F, MFCholesky):
         n = rows_cols(F.factors)[0]
         accepted = n if F.info == 0 else (F.info - 1 if F.info > 0 else 0)
