@@ -13,7 +13,7 @@ refinement against the plain Float64x4 Cholesky solve on the same SPD matrix
 (A = Q diag(sigma) Q^T, kappa = 1e8), reporting both times and backward errors.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--no-refine] [--no-cpu-baseline]
-       python bench.py --suite blas3,factor,blas2,refine,batched,autodiff   (one JSON line per case)
+       python bench.py --suite blas3,factor,blas2,refine,ozaki,qr,batched,autodiff   (one JSON line per case)
 """
 from __future__ import annotations
 
@@ -146,7 +146,7 @@ def main():
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-ozaki", action="store_true")
     ap.add_argument("--suite", default="",
-                    help="comma list of suite groups (blas3,factor,blas2,refine,batched,autodiff): print one JSON line "
+                    help="comma list of suite groups (blas3,factor,blas2,refine,ozaki,qr,batched,autodiff): print one JSON line "
                          "per measured case instead of the headline line")
     ap.add_argument("--suite-out", default="", help="also append the suite's JSON lines to this file")
     ap.add_argument("--copy-calibration", type=int, default=0,

@@ -132,6 +132,23 @@ def case_factor(kind, N, n):
     return _compute(f"{kind} n={n}", N, macs, t)
 
 
+def case_qr(N, m, n, kind="rrqr"):
+    """rrqr! / qr! of a seeded m x n Float64xN matrix; work = the Householder
+    MACs m n k - (m + n) k^2 / 2 + k^3 / 3 (k = min(m, n))."""
+    A0 = rand_dev(N, m, n, 40)
+    fn = M.rrqr_ if kind == "rrqr" else M.qr_
+    fn(rand_dev(N, 64, 32, 41))                      # kernel loading
+    A = A0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn(A)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    k = min(m, n)
+    macs = m * n * k - (m + n) * k * k // 2 + k ** 3 // 3
+    return _compute(f"{kind} {m}x{n}", N, macs, t)
+
+
 def case_gemv(N, n, reps=5):
     A = rand_dev(N, n, n, 8)
     x = rand_dev(N, n, 1, 9).reshape(N, n).contiguous()
@@ -247,6 +264,12 @@ def run(which=("blas3", "factor", "blas2", "refine", "batched", "autodiff"), out
             r = bench_refine(kind=kind, residual="ozaki")
             r["case"] = f"refine {kind} n=4096 nrhs=64 (ozaki residual)"
             emit(r)
+    if "qr" in which:
+        for N in (2, 4):
+            emit(case_qr(N, 500, 40))
+            emit(case_qr(N, 1024, 256))
+        emit(case_qr(4, 1024, 1024))
+        emit(case_qr(4, 1024, 1024, kind="qr"))
     if "batched" in which:
         emit(case_batched())
     if "autodiff" in which:
