@@ -133,21 +133,25 @@ const double kZero[4] = {0.0, 0.0, 0.0, 0.0};
 // GEMM body shared by gemm!, LU's trailing update and the matrix residual.
 // x3 auto/fused route: MFLA's four-row vector blocks use mulacc_x3 and the
 // m mod 4 remainder rows the standard network (gemm.jl:411-452).
+// stream: null = the handle's stream; status_limit as in TileArgs.
 hipError_t gemm_launch(mfla_handle h, int N, int strategy, int tile, int64_t m, int64_t n, int64_t k, const DV &A, const DV &B,
-                       const DV &C, const double *alpha, const double *beta, const Status *st) {
+                       const DV &C, const double *alpha, const double *beta, const Status *st, hipStream_t stream = nullptr,
+                       int64_t status_limit = 0) {
     const bool fused = (N == 3) && (strategy == MFLA_GEMM_AUTO || strategy == MFLA_GEMM_FUSED);
+    hipStream_t s = stream ? stream : h->stream;
     mfk::TileArgs t{};
     t.mode = mfk::MODE_AB; t.lower = 0; t.beta_zero = scalar_is_zero(beta, N) ? 1 : 0; t.tile = tile; t.n = n; t.K = k;
+    t.status_limit = status_limit;
     if (fused) {
         const int64_t m4 = (m / 4) * 4;
         t.net = mf::NET_FUSED3; t.row_lo = 0; t.rows = m4;
-        hipError_t e = mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
+        hipError_t e = mfk::tile_launch_n(N, s, t, A, B, C, alpha, beta, st);
         if (e != hipSuccess) return e;
         t.net = mf::NET_STD; t.row_lo = m4; t.rows = m - m4;
-        return mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
+        return mfk::tile_launch_n(N, s, t, A, B, C, alpha, beta, st);
     }
     t.net = mf::NET_STD; t.row_lo = 0; t.rows = m;
-    return mfk::tile_launch_n(N, h->stream, t, A, B, C, alpha, beta, st);
+    return mfk::tile_launch_n(N, s, t, A, B, C, alpha, beta, st);
 }
 
 // Lower-triangle update C = alpha*a*b + beta*C over i >= j (syrk!/gemmt!).
@@ -684,22 +688,65 @@ int mfla_getrf(mfla_handle h, int N, mfla_matrix A, int64_t *ipiv, double *origm
     if (int rc = original_maximum(h, N, A, 0, om)) return rc;
     const int64_t block = std::max<int64_t>(c.lu_block, 1);
     const DV a = dvn(A);
+    // Blocked right-looking LU (lu.jl:128-195). Row interchanges reach the
+    // columns outside the current panel once the panel is done (lu_laswp),
+    // before anything reads them. With look-ahead, the trsm!/gemm! update of
+    // the next panel's columns runs on the main stream and the rest of the
+    // trailing update on a second stream while the next panel is factored;
+    // every element receives the same operations in the same order as in
+    // MFLA's sequence, so factors and pivots are bitwise identical.
+    const bool lookahead = n - std::min<int64_t>(block, n) > 4 * block && kmax > block;
+    hipStream_t s1 = s;
+    if (lookahead) {
+        HIPCHK(ensure_side(h));
+        s1 = h->side;
+        HIPCHK(hipEventRecord(h->ev_join, s));
+        HIPCHK(hipStreamWaitEvent(s1, h->ev_join, 0));
+    }
+    bool side_pending = false;
+    // trsm!(U12, L11; side=:left, uplo=:lower, trans=:N, diag=:unit) and
+    // gemm!(A22, A21, U12, -1, 1) on the columns [c0, c0 + cols)
+    auto update = [&](hipStream_t st_, int64_t first, int64_t last, int64_t c0, int64_t cols, int64_t limit) -> hipError_t {
+        const int64_t nb = last - first + 1;
+        mfk::TrsmArgs ta{};
+        ta.n = nb; ta.nrhs = cols; ta.forward = 1; ta.unit = 1; ta.exact_order = 1; ta.status_limit = limit;
+        const DV U12 = sub(a, first, c0), L11 = sub(a, first, first);
+        hipError_t e = mfk::trsm_solve(N, st_, ta, U12, L11, L11, st);
+        if (e != hipSuccess || last + 1 >= m) return e;
+        return gemm_launch(h, N, c.gemm_strategy, c.gemm_tile, m - last - 1, cols, nb, sub(a, last + 1, first), U12, sub(a, last + 1, c0),
+                           kMinusOne, kOne, st, st_, limit);
+    };
     for (int64_t first = 0; first < kmax; first += block) {
-        const int64_t last = std::min(first + block, kmax) - 1, nb = last - first + 1;
+        const int64_t last = std::min(first + block, kmax) - 1;
         for (int64_t k = first; k <= last; ++k) {
-            HIPCHK(mfk::lu_pivot_swap(N, s, a, k, m, n, ipiv, st));
+            HIPCHK(mfk::lu_pivot_swap(N, s, a, k, m, first, last + 1, ipiv, st));
             HIPCHK(mfk::lu_scale_update(N, s, a, k, last, m, st));
         }
-        if (last + 1 < n) {
-            // trsm!(U12, L11; side=:left, uplo=:lower, trans=:N, diag=:unit)
-            mfk::TrsmArgs ta{};
-            ta.n = nb; ta.nrhs = n - last - 1; ta.forward = 1; ta.unit = 1; ta.exact_order = 1;
-            const DV U12 = sub(a, first, last + 1), L11 = sub(a, first, first);
-            HIPCHK(mfk::trsm_solve(N, s, ta, U12, L11, L11, st));
-            if (last + 1 < m)  // gemm!(A22, A21, U12, -1, 1)
-                HIPCHK(gemm_launch(h, N, c.gemm_strategy, c.gemm_tile, m - last - 1, n - last - 1, nb, sub(a, last + 1, first), U12,
-                                   sub(a, last + 1, last + 1), kMinusOne, kOne, st));
+        if (side_pending) {   // the previous trailing update owns the columns right of the next panel
+            HIPCHK(hipStreamWaitEvent(s, h->ev_update, 0));
+            side_pending = false;
         }
+        HIPCHK(mfk::lu_laswp(N, s, a, first, last - first + 1, ipiv, 0, first, last + 1, n - last - 1));
+        if (last + 1 >= n) continue;
+        const int64_t rest = n - last - 1, nxt = std::min(block, rest);
+        if (!lookahead || rest <= nxt || last + 1 >= kmax) {
+            HIPCHK(update(s, first, last, last + 1, rest, 0));
+            continue;
+        }
+        HIPCHK(hipEventRecord(h->ev_panel, s));
+        HIPCHK(update(s, first, last, last + 1, nxt, 0));
+        // the rest on the side stream; it skips its work only for failures in
+        // panels up to this one (a later failure comes after this update in
+        // MFLA's order)
+        HIPCHK(hipStreamWaitEvent(s1, h->ev_panel, 0));
+        HIPCHK(update(s1, first, last, last + 1 + nxt, rest - nxt, last + 1));
+        HIPCHK(hipEventRecord(h->ev_update, s1));
+        side_pending = true;
+    }
+    if (side_pending) HIPCHK(hipStreamWaitEvent(s, h->ev_update, 0));
+    if (lookahead) {
+        HIPCHK(hipEventRecord(h->ev_join, s1));
+        HIPCHK(hipStreamWaitEvent(s, h->ev_join, 0));
     }
     const int64_t r = read_info(h);
     if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");

@@ -115,9 +115,12 @@ __global__ __launch_bounds__(256) void k_chol_panel(DV A, int64_t first, int64_t
 
 // -------------------------------------------------------------------- LU
 // lu.jl:91-102: pivot search over rows k..m-1 of column k, record ipiv, then
-// swap whole rows across all n columns. One workgroup.
+// swap the rows within the panel columns [c0, c1) (MFLA swaps whole rows;
+// the other columns get the same interchanges from k_lu_laswp once the panel
+// is done, before anything reads them, so every element ends identical).
+// One workgroup.
 template <int N>
-__global__ __launch_bounds__(1024) void k_lu_pivot_swap(DV A, int64_t k, int64_t m, int64_t n, int64_t *ipiv, Status *st) {
+__global__ __launch_bounds__(1024) void k_lu_pivot_swap(DV A, int64_t k, int64_t m, int64_t c0, int64_t c1, int64_t *ipiv, Status *st) {
     if (st->info != 0) return;
     __shared__ double sv_buf[1024 * N];
     __shared__ int64_t si_buf[1024];
@@ -131,7 +134,29 @@ __global__ __launch_bounds__(1024) void k_lu_pivot_swap(DV A, int64_t k, int64_t
     const int64_t piv = bi;
     if (threadIdx.x == 0) { ipiv[k] = piv + 1; st->aux = piv; }
     if (piv != k)
-        for (int64_t j = threadIdx.x; j < n; j += blockDim.x) swap_el<N>(A, k, j, piv, j);
+        for (int64_t j = c0 + threadIdx.x; j < c1; j += blockDim.x) swap_el<N>(A, k, j, piv, j);
+}
+
+// The panel's interchanges ipiv[first .. first + count) applied in order to
+// the columns [a0, a0 + na) and [b0, b0 + nb), one thread per (column, limb).
+// No status check: after a failing pivot MFLA has already swapped whole rows
+// up to that column, and the unused ipiv entries are the identity.
+template <int N>
+__global__ void k_lu_laswp(DV A, int64_t first, int64_t count, const int64_t *ipiv, int64_t a0, int64_t na, int64_t b0, int64_t nb) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (na + nb) * N) return;
+    const int64_t c = e / N;
+    const int l = (int)(e % N);
+    const int64_t j = c < na ? a0 + c : b0 + (c - na);
+    double *col = A.p + j * A.cs + l * A.ls;
+    for (int64_t k = first; k < first + count; ++k) {
+        const int64_t p = ipiv[k] - 1;
+        if (p != k) {
+            const double x = col[k * A.rs], y = col[p * A.rs];
+            col[k * A.rs] = y;
+            col[p * A.rs] = x;
+        }
+    }
 }
 
 // lu.jl:104-109 + rank-1 update lu.jl:47-82 restricted to the panel columns.
@@ -382,8 +407,16 @@ hipError_t chol_panel(int N, hipStream_t s, const DV &A, int64_t first, int64_t 
     return hipGetLastError();
 }
 
-hipError_t lu_pivot_swap(int N, hipStream_t s, const DV &A, int64_t k, int64_t m, int64_t n, int64_t *ipiv, Status *st) {
-    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_pivot_swap<NN>, dim3(1), dim3(1024), 0, s, A, k, m, n, ipiv, st));
+hipError_t lu_pivot_swap(int N, hipStream_t s, const DV &A, int64_t k, int64_t m, int64_t c0, int64_t c1, int64_t *ipiv, Status *st) {
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_pivot_swap<NN>, dim3(1), dim3(1024), 0, s, A, k, m, c0, c1, ipiv, st));
+    return hipGetLastError();
+}
+
+hipError_t lu_laswp(int N, hipStream_t s, const DV &A, int64_t first, int64_t count, const int64_t *ipiv, int64_t a0, int64_t acount,
+                    int64_t b0, int64_t bcount) {
+    if (acount + bcount <= 0 || count <= 0) return hipSuccess;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_laswp<NN>, dim3(nb((acount + bcount) * N, 256)), dim3(256), 0, s, A, first, count, ipiv, a0, acount,
+                                      b0, bcount));
     return hipGetLastError();
 }
 

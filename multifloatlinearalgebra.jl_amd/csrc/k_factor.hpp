@@ -5,7 +5,13 @@
 namespace mfk {
 
 hipError_t chol_panel(int N, hipStream_t s, const DV &A, int64_t first, int64_t last, Status *st);
-hipError_t lu_pivot_swap(int N, hipStream_t s, const DV &A, int64_t k, int64_t m, int64_t n, int64_t *ipiv, Status *st);
+// Pivot of column k over rows k..m-1; the row interchange is applied to the
+// panel columns [c0, c1) only (lu_laswp brings it to the other columns).
+hipError_t lu_pivot_swap(int N, hipStream_t s, const DV &A, int64_t k, int64_t m, int64_t c0, int64_t c1, int64_t *ipiv, Status *st);
+// ipiv[first .. first + count) applied in order to columns [a0, a0 + acount)
+// and [b0, b0 + bcount).
+hipError_t lu_laswp(int N, hipStream_t s, const DV &A, int64_t first, int64_t count, const int64_t *ipiv, int64_t a0, int64_t acount,
+                    int64_t b0, int64_t bcount);
 hipError_t lu_scale_update(int N, hipStream_t s, const DV &A, int64_t k, int64_t last, int64_t m, Status *st);
 hipError_t apply_ipiv(int N, hipStream_t s, const DV &B, int64_t count, int64_t nrhs, const int64_t *ipiv);
 hipError_t ldlt_unblocked(int N, hipStream_t s, const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks,

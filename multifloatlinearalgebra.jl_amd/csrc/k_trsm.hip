@@ -32,8 +32,8 @@ constexpr int TC = 4;    // right-hand sides per workgroup (threads in y)
 
 template <int N, bool FWD>
 __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit,
-                                                      int swap, const Status *st) {
-    if (st && st->info != 0) return;
+                                                      int swap, const Status *st, int64_t lim) {
+    if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
     __shared__ double xs[TC * N];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int64_t c = (int64_t)blockIdx.x * TC + ty;
@@ -62,8 +62,8 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
 
 // Exact MFLA order for backward substitution: one thread per right-hand side.
 template <int N>
-__global__ void k_trsm_seq_bwd(int64_t n, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap, const Status *st) {
-    if (st && st->info != 0) return;
+__global__ void k_trsm_seq_bwd(int64_t n, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap, const Status *st, int64_t lim) {
+    if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nrhs) return;
     for (int64_t p = n - 1; p >= 0; --p) {
@@ -79,10 +79,10 @@ static DV sub(const DV &v, int64_t i, int64_t j) { return DV{v.p + i * v.rs + j 
 
 template <int N>
 static hipError_t diag(hipStream_t s, bool fwd, int64_t p0, int nbk, int64_t nrhs, const DV &X, const DV &CA, const DV &D, int unit,
-                       int swap, const Status *st) {
+                       int swap, const Status *st, int64_t lim) {
     dim3 grid((unsigned)cdiv(nrhs, TC)), block(TB, TC);
-    if (fwd) hipLaunchKernelGGL((k_trsm_diag<N, true>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st);
-    else hipLaunchKernelGGL((k_trsm_diag<N, false>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st);
+    if (fwd) hipLaunchKernelGGL((k_trsm_diag<N, true>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
+    else hipLaunchKernelGGL((k_trsm_diag<N, false>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
     return hipGetLastError();
 }
 
@@ -94,10 +94,11 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
     TileArgs t{};
     t.mode = MODE_SUB; t.net = mf::NET_STD; t.lower = 0; t.beta_zero = 0; t.tile = -1; t.row_lo = 0; t.n = nrhs;
     t.swap_product = a.swap_product;
+    t.status_limit = a.status_limit;
     if (a.forward) {
         for (int64_t p0 = 0; p0 < n; p0 += TB) {
             const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
-            if ((e = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st)) != hipSuccess) return e;
+            if ((e = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit)) != hipSuccess) return e;
             const int64_t p1 = p0 + nbk;
             if (p1 < n) {
                 t.rows = n - p1; t.K = nbk;
@@ -107,13 +108,13 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
         return hipSuccess;
     }
     if (a.exact_order) {
-        hipLaunchKernelGGL(k_trsm_seq_bwd<N>, dim3((unsigned)cdiv(nrhs, 64)), dim3(64), 0, s, n, nrhs, X, CA, D, a.unit, a.swap_product, st);
+        hipLaunchKernelGGL(k_trsm_seq_bwd<N>, dim3((unsigned)cdiv(nrhs, 64)), dim3(64), 0, s, n, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit);
         return hipGetLastError();
     }
     for (int64_t p1 = n; p1 > 0; p1 -= TB) {
         const int64_t p0 = p1 > TB ? p1 - TB : 0;
         const int nbk = (int)(p1 - p0);
-        if ((e = diag<N>(s, false, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st)) != hipSuccess) return e;
+        if ((e = diag<N>(s, false, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit)) != hipSuccess) return e;
         if (p0 > 0) {
             t.rows = p0; t.K = nbk;
             if ((e = tile_launch_n(N, s, t, sub(CA, 0, p0), sub(X, p0, 0), X, nullptr, nullptr, st)) != hipSuccess) return e;

@@ -12,6 +12,8 @@ struct TrsmArgs {
     int exact_order;  // backward only: 1 = MFLA's ascending-q chain, 0 = blocked
     int swap_product; // 1: each term is X(q)*CA(p,q) (MFLA's right-side solves,
                       // trsm.jl:136-149, 159-172); 0: CA(p,q)*X(q) (left side)
+    int64_t status_limit;  // 0: stop on any recorded failure; > 0: only on
+                           // failures at (one-based) columns <= status_limit
 };
 
 // Solves in place on the solution view X(p, c) with coefficient view CA(p, q)

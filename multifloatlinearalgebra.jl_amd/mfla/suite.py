@@ -134,7 +134,7 @@ def case_factor(kind, N, n):
 
 def case_qr(N, m, n, kind="rrqr"):
     """rrqr! / qr! of a seeded m x n Float64xN matrix; work = the Householder
-    MACs m n k - (m + n) k^2 / 2 + k^3 / 3 (k = min(m, n))."""
+    MACs 2 (m n k - (m + n) k^2 / 2 + k^3 / 3) (k = min(m, n); a projection and an update per reflector and column)."""
     A0 = rand_dev(N, m, n, 40)
     fn = M.rrqr_ if kind == "rrqr" else M.qr_
     fn(rand_dev(N, 64, 32, 41))                      # kernel loading
@@ -145,7 +145,7 @@ def case_qr(N, m, n, kind="rrqr"):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     k = min(m, n)
-    macs = m * n * k - (m + n) * k * k // 2 + k ** 3 // 3
+    macs = 2 * (m * n * k - (m + n) * k * k // 2 + k ** 3 // 3)
     return _compute(f"{kind} {m}x{n}", N, macs, t)
 
 

@@ -109,8 +109,28 @@ def test_cholesky_bitwise(M, oracle, N, n, block):
     assert_bitwise(host(x), oracle.potrs(ref, b.copy()), "potrs vector")
 
 
+@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("bad", [5, 150, 290])
+def test_lu_failure_in_a_late_panel_bitwise(M, oracle, N, bad):
+    """A zero pivot in panel j leaves the matrix and the pivots exactly as
+    MFLA's order does (whole-row interchanges up to the failing column, every
+    trailing update of panels < j, nothing after), with look-ahead active."""
+    rng = np.random.default_rng(bad * 3 + N)
+    n, block = 300, 16
+    A = rand_mf(rng, N, n, n)
+    A[:, bad, :] = 0.0                           # column `bad` stays exactly zero
+    ref = A.copy()
+    info_ref, ipiv_ref, _ = oracle.getrf(ref, block)
+    Ad = dev(A)
+    F = M.lu_(Ad, check=False, config=M.KernelConfig(lu_block=block))
+    assert F.info == info_ref == bad + 1
+    assert np.array_equal(F.ipiv.cpu().numpy(), ipiv_ref)
+    assert_bitwise(host(Ad), ref, f"failed lu x{N} at {info_ref}")
+
+
 @pytest.mark.parametrize("N", [1, 2, 3, 4])
-@pytest.mark.parametrize("m,n,block", [(40, 40, 16), (67, 67, 16), (90, 50, 8), (50, 90, 16), (129, 129, 32)])
+@pytest.mark.parametrize("m,n,block", [(40, 40, 16), (67, 67, 16), (90, 50, 8), (50, 90, 16), (129, 129, 32), (300, 300, 16),
+                                       (300, 200, 16), (200, 300, 16), (257, 257, 8)])
 def test_lu_bitwise(M, oracle, N, m, n, block):
     rng = np.random.default_rng(m * 100 + n + N + block)
     A = rand_mf(rng, N, m, n)
