@@ -21,6 +21,7 @@
 // (k_trsm_seq, one thread per right-hand side) and a blocked right-looking
 // order (descending blocks) whose rounding differs from MFLA's but stays
 // within the same c*n*u bound.
+#include <type_traits>
 #include "mf_common.hpp"
 #include "k_blas3.hpp"
 #include "k_trsm.hpp"
@@ -60,6 +61,40 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
     }
 }
 
+// The same diagonal-block solve without barriers: a group of G lanes holds
+// one right-hand side (lane p_local = unknown p0 + p_local). At step s the
+// owner lane's running value is final; the group reads it by shuffle and
+// every lane of the group forms the same quotient x_s (so no lane waits for
+// another), then the later unknowns subtract their term. Per element the
+// operations and their order are those of k_trsm_diag.
+template <int N, bool FWD, int G>
+__global__ __launch_bounds__(256) void k_trsm_diag_wave(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap,
+                                                        const Status *st, int64_t lim) {
+    if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
+    constexpr int RPW = 64 / G;
+    const int lane = threadIdx.x & 63, grp = lane / G, pl = lane % G;
+    const int64_t c = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + grp;
+    const bool valid = pl < nbk && c < nrhs;
+    const int64_t p = p0 + pl;
+    mf::V<N> v = valid ? ld<N>(X, p, c) : mf::zero<N>();
+    for (int s0 = 0; s0 < nbk; ++s0) {
+        const int s = FWD ? s0 : nbk - 1 - s0;
+        const int64_t q = p0 + s;
+        mf::V<N> xq;
+#pragma unroll
+        for (int l = 0; l < N; ++l) xq.x[l] = __shfl(v.x[l], grp * G + s, 64);
+        if (!unit) xq = mf::div(xq, ld<N>(D, q, q));
+        if (valid) {
+            if (pl == s) {
+                v = xq;
+                stv<N>(X, q, c, xq);
+            } else if (FWD ? pl > s : pl < s) {
+                v = swap ? mf::msub(v, xq, ld<N>(CA, p, q)) : mf::msub(v, ld<N>(CA, p, q), xq);
+            }
+        }
+    }
+}
+
 // Exact MFLA order for backward substitution: one thread per right-hand side.
 template <int N>
 __global__ void k_trsm_seq_bwd(int64_t n, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap, const Status *st, int64_t lim) {
@@ -80,9 +115,21 @@ static DV sub(const DV &v, int64_t i, int64_t j) { return DV{v.p + i * v.rs + j 
 template <int N>
 static hipError_t diag(hipStream_t s, bool fwd, int64_t p0, int nbk, int64_t nrhs, const DV &X, const DV &CA, const DV &D, int unit,
                        int swap, const Status *st, int64_t lim) {
-    dim3 grid((unsigned)cdiv(nrhs, TC)), block(TB, TC);
-    if (fwd) hipLaunchKernelGGL((k_trsm_diag<N, true>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
-    else hipLaunchKernelGGL((k_trsm_diag<N, false>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
+    // one group of G lanes per right-hand side, 4 waves per workgroup
+    auto launch = [&](auto gtag) {
+        constexpr int G = decltype(gtag)::value;
+        const dim3 grid((unsigned)cdiv(cdiv(nrhs, 64 / G), 4)), block(256);
+        if (fwd) hipLaunchKernelGGL((k_trsm_diag_wave<N, true, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
+        else hipLaunchKernelGGL((k_trsm_diag_wave<N, false, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
+    };
+    if (nbk <= 16) launch(std::integral_constant<int, 16>());
+    else if (nbk <= 32) launch(std::integral_constant<int, 32>());
+    else if (nbk <= TB) launch(std::integral_constant<int, 64>());
+    else {
+        dim3 grid((unsigned)cdiv(nrhs, TC)), block(TB, TC);
+        if (fwd) hipLaunchKernelGGL((k_trsm_diag<N, true>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
+        else hipLaunchKernelGGL((k_trsm_diag<N, false>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
+    }
     return hipGetLastError();
 }
 
