@@ -94,27 +94,39 @@ __device__ int64_t wg_potrf(const DV &A, int64_t n, int64_t block, double *piv_s
 
 // Systolic triangular solve on up to RHS_MAX right-hand sides: thread p owns
 // row p. Forward: rows subtract pivots in ascending order (MFLA order).
-// Backward: rows subtract pivots in descending order.
+// Backward: rows subtract pivots in descending order. The right-hand sides go
+// through in chunks of RHS_CHUNK, each chunk's values in registers.
+constexpr int RHS_CHUNK = 4;
+
 template <int N>
 __device__ void wg_systolic(const DV &X, const DV &CA, const DV &D, int64_t n, int nrhs, bool forward, bool unit, double *xs) {
     const int64_t p = threadIdx.x;
-    mf::V<N> v[RHS_MAX];
-    for (int c = 0; c < RHS_MAX; ++c) v[c] = (p < n && c < nrhs) ? ld<N>(X, p, c) : mf::zero<N>();
-    for (int64_t s = 0; s < n; ++s) {
-        const int64_t q = forward ? s : n - 1 - s;
-        if (p == q) {
-            for (int c = 0; c < nrhs; ++c) {
-                if (!unit) v[c] = mf::div(v[c], ld<N>(D, q, q));
-                stv<N>(X, q, c, v[c]);
-                lds_put<N>(&xs[c * N], v[c]);
+    for (int c0 = 0; c0 < nrhs; c0 += RHS_CHUNK) {
+        const int nc = nrhs - c0 < RHS_CHUNK ? nrhs - c0 : RHS_CHUNK;
+        mf::V<N> v[RHS_CHUNK];   // every index below is a compile-time constant
+#pragma unroll
+        for (int c = 0; c < RHS_CHUNK; ++c) v[c] = (p < n && c < nc) ? ld<N>(X, p, c0 + c) : mf::zero<N>();
+        for (int64_t s = 0; s < n; ++s) {
+            const int64_t q = forward ? s : n - 1 - s;
+            if (p == q) {
+#pragma unroll
+                for (int c = 0; c < RHS_CHUNK; ++c) {
+                    if (c < nc) {
+                        if (!unit) v[c] = mf::div(v[c], ld<N>(D, q, q));
+                        stv<N>(X, q, c0 + c, v[c]);
+                        lds_put<N>(&xs[c * N], v[c]);
+                    }
+                }
             }
+            __syncthreads();
+            if (p < n && (forward ? p > q : p < q)) {
+                const mf::V<N> a = ld<N>(CA, p, q);
+#pragma unroll
+                for (int c = 0; c < RHS_CHUNK; ++c)
+                    if (c < nc) v[c] = mf::msub(v[c], a, lds_get<N>(&xs[c * N]));
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (p < n && (forward ? p > q : p < q)) {
-            const mf::V<N> a = ld<N>(CA, p, q);
-            for (int c = 0; c < nrhs; ++c) v[c] = mf::msub(v[c], a, lds_get<N>(&xs[c * N]));
-        }
-        __syncthreads();
     }
 }
 
@@ -147,6 +159,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BatchEntry *E, int64_t cou
     __shared__ int64_t si_buf[BT];
     __shared__ double xs[RHS_MAX * N];
     __shared__ double piv_s[N];
+    __shared__ double cbuf[2 * BT * N];
     __shared__ int flag;
     __shared__ int64_t next_s;
     __shared__ int64_t info_s;
@@ -204,7 +217,7 @@ __global__ __launch_bounds__(BT) void k_batched(const BatchEntry *E, int64_t cou
                     info_s = 0;
                 }
                 __syncthreads();
-                ldlt_unblocked_body<N>(A, n, dsub, piv, blk, alpha, &info_s, sv_buf, si_buf);
+                ldlt_unblocked_body<N>(A, n, dsub, piv, blk, alpha, &info_s, sv_buf, si_buf, cbuf);
                 __syncthreads();
                 st = info_s;
                 __syncthreads();

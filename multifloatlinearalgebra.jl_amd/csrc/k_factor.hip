@@ -194,10 +194,11 @@ __global__ __launch_bounds__(512) void k_ldlt_unblocked(DV A, int64_t n, DV dsub
     if (st->info != 0) return;
     __shared__ double sv_buf[512 * N];
     __shared__ int64_t si_buf[512];
+    __shared__ double cbuf[2 * 512 * N];
     __shared__ int64_t info_s;
     if (threadIdx.x == 0) info_s = 0;
     __syncthreads();
-    ldlt_unblocked_body<N>(A, n, dsub, pivots, blocks, sv(alpha), &info_s, sv_buf, si_buf);
+    ldlt_unblocked_body<N>(A, n, dsub, pivots, blocks, sv(alpha), &info_s, sv_buf, si_buf, n <= 512 ? cbuf : nullptr);
     __syncthreads();
     if (threadIdx.x == 0 && info_s != 0) st->info = info_s;
 }

@@ -149,12 +149,28 @@ __device__ BKChoice wg_select_bk(int64_t n, int64_t k, const mf::V<N> &alpha, En
     return BKChoice{2, imax};
 }
 
+// (row, col) = (e mod w, e div w), in 32-bit arithmetic when w * w fits.
+__device__ __forceinline__ void sq_index(int64_t e, int64_t w, int64_t &row, int64_t &col) {
+    if (w * w < 2147483647LL) {
+        const unsigned ue = (unsigned)e, uw = (unsigned)w;
+        col = ue / uw;
+        row = ue - (unsigned)col * uw;
+    } else {
+        col = e / w;
+        row = e - col * w;
+    }
+}
+
 // Unblocked right-looking Bunch-Kaufman LDLT (ldlt.jl:352-412), one
 // workgroup per matrix. Matrices are addressed through a batch stride so the
-// same body serves single and batched factorizations.
+// same body serves single and batched factorizations. MFLA forms one
+// coefficient per trailing column (coef = d * L(col, k), or c1 / c2 of the
+// 2 x 2 step) before its row loop; with cbuf (2 * n * N doubles of LDS, or
+// null) they are formed once per column here too, otherwise per element
+// (same value either way).
 template <int N>
 __device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks, const mf::V<N> &alpha,
-                                    int64_t *info_out, double *sv_buf, int64_t *si_buf) {
+                                    int64_t *info_out, double *sv_buf, int64_t *si_buf, double *cbuf = nullptr) {
     auto entry = [&](int64_t i, int64_t j) { return ld<N>(A, i, j); };
     int64_t k = 0;
     while (k < n) {
@@ -169,10 +185,17 @@ __device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int6
             __syncthreads();
             // rank-1 update (ldlt.jl:221-262), both triangles written
             const int64_t w = n - k - 1;
+            if (cbuf) {
+                for (int64_t j = threadIdx.x; j < w; j += blockDim.x) lds_put<N>(&cbuf[j * N], mf::mul(d, ld<N>(A, k + 1 + j, k)));
+                __syncthreads();
+            }
             for (int64_t e = threadIdx.x; e < w * w; e += blockDim.x) {
-                const int64_t col = k + 1 + e / w, row = k + 1 + e % w;
+                int64_t row, col;
+                sq_index(e, w, row, col);
                 if (row < col) continue;
-                const mf::V<N> coef = mf::mul(d, ld<N>(A, col, k));
+                const mf::V<N> coef = cbuf ? lds_get<N>(&cbuf[col * N]) : mf::mul(d, ld<N>(A, k + 1 + col, k));
+                row += k + 1;
+                col += k + 1;
                 const mf::V<N> v = mf::msub(ld<N>(A, row, col), ld<N>(A, row, k), coef);
                 stv<N>(A, row, col, v);
                 stv<N>(A, col, row, v);
@@ -205,12 +228,33 @@ __device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int6
             // scalar form on the remainder rows of each column
             if (k + 1 < n - 1) {
                 const int64_t w = n - k - 2;
-                for (int64_t e = threadIdx.x; e < w * w; e += blockDim.x) {
-                    const int64_t col = k + 2 + e / w, row = k + 2 + e % w;
-                    if (row < col) continue;
+                auto coefs = [&](int64_t col, mf::V<N> &c1, mf::V<N> &c2) {
                     const mf::V<N> l1 = ld<N>(A, col, k), l2 = ld<N>(A, col, k + 1);
-                    const mf::V<N> c1 = mf::add(mf::mul(d11, l1), mf::mul(d21, l2));
-                    const mf::V<N> c2 = mf::add(mf::mul(d21, l1), mf::mul(d22, l2));
+                    c1 = mf::add(mf::mul(d11, l1), mf::mul(d21, l2));
+                    c2 = mf::add(mf::mul(d21, l1), mf::mul(d22, l2));
+                };
+                if (cbuf) {
+                    for (int64_t j = threadIdx.x; j < w; j += blockDim.x) {
+                        mf::V<N> c1, c2;
+                        coefs(k + 2 + j, c1, c2);
+                        lds_put<N>(&cbuf[j * N], c1);
+                        lds_put<N>(&cbuf[(n + j) * N], c2);
+                    }
+                    __syncthreads();
+                }
+                for (int64_t e = threadIdx.x; e < w * w; e += blockDim.x) {
+                    int64_t row, col;
+                    sq_index(e, w, row, col);
+                    if (row < col) continue;
+                    mf::V<N> c1, c2;
+                    if (cbuf) {
+                        c1 = lds_get<N>(&cbuf[col * N]);
+                        c2 = lds_get<N>(&cbuf[(n + col) * N]);
+                    } else {
+                        coefs(k + 2 + col, c1, c2);
+                    }
+                    row += k + 2;
+                    col += k + 2;
                     const int64_t vec_end = col + ((n - col) / 4) * 4;
                     mf::V<N> v;
                     if (row < vec_end) v = mf::sub(ld<N>(A, row, col), mf::add(mf::mul(ld<N>(A, row, k), c1), mf::mul(ld<N>(A, row, k + 1), c2)));
