@@ -211,6 +211,12 @@ bool trsm_exact(const mfla_config &c, int64_t n) {
     return n <= 256;
 }
 
+#define HIPRET_(expr)                       \
+    do {                                    \
+        hipError_t e_ = (expr);             \
+        if (e_ != hipSuccess) return e_;    \
+    } while (0)
+
 // trsm! core (trsm.jl:247-301) after argument checks; alpha scaling included.
 hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int diag, const mfla_matrix &B, const mfla_matrix &A,
                      const double *alpha, const mfla_config &c, const Status *st) {
@@ -235,6 +241,11 @@ hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int di
         X = dvt(B); a.nrhs = B.rows;
         CA = transposed ? dvn(A) : dvt(A);
         a.forward = !eff_lower;
+    }
+    if (n > 3 * 64 && (a.forward || !a.exact_order) && ensure_side(h) == hipSuccess) {
+        a.side = h->side; a.ev_main = h->ev_panel; a.ev_side = h->ev_update;
+        HIPRET_(hipEventRecord(h->ev_join, h->stream));
+        HIPRET_(hipStreamWaitEvent(h->side, h->ev_join, 0));
     }
     return mfk::trsm_solve(N, h->stream, a, X, CA, dvn(A), st);
 }

@@ -142,17 +142,48 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
     t.mode = MODE_SUB; t.net = mf::NET_STD; t.lower = 0; t.beta_zero = 0; t.tile = -1; t.row_lo = 0; t.n = nrhs;
     t.swap_product = a.swap_product;
     t.status_limit = a.status_limit;
+    // X(r0 : r0 + rows) -= CA(r0 : , q0 : q0 + K) X(q0 : q0 + K) on stream ss
+    auto update = [&](hipStream_t ss, int64_t r0, int64_t rows, int64_t q0, int64_t K) -> hipError_t {
+        if (rows <= 0) return hipSuccess;
+        TileArgs u = t;
+        u.rows = rows; u.K = K;
+        return tile_launch_n(N, ss, u, sub(CA, r0, q0), sub(X, q0, 0), sub(X, r0, 0), nullptr, nullptr, st);
+    };
+    // Look-ahead: block b's update of the next block's rows runs on s after
+    // the side stream's update from block b-1 reached them (every element
+    // keeps MFLA's block order); its update of the rows beyond runs on the
+    // side stream while s solves the next block.
+    const bool ahead = a.side != nullptr && n > 3 * TB;
+    bool pending = false;
+    auto split_update = [&](int64_t q0, int64_t K, int64_t near0, int64_t nnear, int64_t far0, int64_t nfar) -> hipError_t {
+        hipError_t r;
+        if (!ahead) {
+            if ((r = update(s, near0, nnear, q0, K)) != hipSuccess) return r;
+            return update(s, far0, nfar, q0, K);
+        }
+        if ((r = hipEventRecord(a.ev_main, s)) != hipSuccess) return r;
+        if (pending && (r = hipStreamWaitEvent(s, a.ev_side, 0)) != hipSuccess) return r;
+        pending = false;
+        if ((r = update(s, near0, nnear, q0, K)) != hipSuccess) return r;
+        if (nfar <= 0) return hipSuccess;
+        if ((r = hipStreamWaitEvent(a.side, a.ev_main, 0)) != hipSuccess) return r;
+        if ((r = update(a.side, far0, nfar, q0, K)) != hipSuccess) return r;
+        if ((r = hipEventRecord(a.ev_side, a.side)) != hipSuccess) return r;
+        pending = true;
+        return hipSuccess;
+    };
+    auto finish = [&]() -> hipError_t { return pending ? hipStreamWaitEvent(s, a.ev_side, 0) : hipSuccess; };
     if (a.forward) {
         for (int64_t p0 = 0; p0 < n; p0 += TB) {
             const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
             if ((e = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit)) != hipSuccess) return e;
             const int64_t p1 = p0 + nbk;
             if (p1 < n) {
-                t.rows = n - p1; t.K = nbk;
-                if ((e = tile_launch_n(N, s, t, sub(CA, p1, p0), sub(X, p0, 0), sub(X, p1, 0), nullptr, nullptr, st)) != hipSuccess) return e;
+                const int64_t nn = (n - p1) < TB ? (n - p1) : TB;
+                if ((e = split_update(p0, nbk, p1, nn, p1 + nn, n - p1 - nn)) != hipSuccess) return e;
             }
         }
-        return hipSuccess;
+        return finish();
     }
     if (a.exact_order) {
         hipLaunchKernelGGL(k_trsm_seq_bwd<N>, dim3((unsigned)cdiv(nrhs, 64)), dim3(64), 0, s, n, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit);
@@ -163,11 +194,11 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
         const int nbk = (int)(p1 - p0);
         if ((e = diag<N>(s, false, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit)) != hipSuccess) return e;
         if (p0 > 0) {
-            t.rows = p0; t.K = nbk;
-            if ((e = tile_launch_n(N, s, t, sub(CA, 0, p0), sub(X, p0, 0), X, nullptr, nullptr, st)) != hipSuccess) return e;
+            const int64_t nn = p0 < TB ? p0 : TB;
+            if ((e = split_update(p0, nbk, p0 - nn, nn, 0, p0 - nn)) != hipSuccess) return e;
         }
     }
-    return hipSuccess;
+    return finish();
 }
 
 hipError_t trsm_solve(int N, hipStream_t s, const TrsmArgs &a, const DV &X, const DV &CA, const DV &D, const Status *st) {

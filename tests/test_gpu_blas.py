@@ -156,6 +156,23 @@ def test_trsm_all_variants_exact_order(mfla, oracle, N, side, uplo, trans, diag)
 
 
 @pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("side,uplo,trans", [("left", "lower", "N"), ("left", "upper", "T"), ("right", "upper", "N"),
+                                             ("right", "lower", "T")])
+def test_trsm_forward_lookahead_bitwise(mfla, oracle, N, side, uplo, trans):
+    """Forward orders at n = 330 (six 64-row blocks): the diagonal solves
+    overlap the previous block's trailing update on a second stream; every
+    element keeps MFLA's ascending order, so the result is bitwise."""
+    rng = np.random.default_rng(hash((N, side, uplo, trans)) % 2**32)
+    n, other = 330, 7
+    A = _triangular(rng, N, n, uplo, False)
+    B = rand_mf(rng, N, n, other) if side == "left" else rand_mf(rng, N, other, n)
+    Bd = dev(B)
+    mfla.trsm_(Bd, dev(A), side=side, uplo=uplo, trans=trans)
+    ref = oracle.trsm(B.copy(), A, 1.0, side, uplo, trans, "nonunit")
+    assert_bitwise(host(Bd), ref, f"trsm look-ahead x{N} {side} {uplo} {trans}")
+
+
+@pytest.mark.parametrize("N", [2, 4])
 @pytest.mark.parametrize("uplo,trans", [("upper", "N"), ("lower", "T")])
 def test_trsm_blocked_backward_within_tolerance(mfla, oracle, N, uplo, trans):
     rng = np.random.default_rng(31 + N)
