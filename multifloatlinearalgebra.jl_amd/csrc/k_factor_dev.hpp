@@ -29,29 +29,35 @@ template <int N> __device__ __forceinline__ void swap_el(const DV &A, int64_t i1
 // Workgroup argmax of |value| with MFLA's first-strict-maximum rule: the
 // largest value wins, ties go to the smallest index (lu.jl:93-100,
 // ldlt.jl:192-200). Every thread passes its best (value, index); idx < 0 means
-// "no candidate". Result broadcast through LDS.
+// "no candidate". The rule is a total order on (value, index), so a wave
+// butterfly (shuffles) and one pass over the per-wave winners give the same
+// winner as any other reduction order. blockDim.x is a multiple of 64.
 template <int N>
 __device__ void wg_argmax(mf::V<N> &val, int64_t &idx, double *sv_buf, int64_t *si_buf) {
-    const int t = threadIdx.x, T = blockDim.x;
-    lds_put<N>(&sv_buf[t * N], val);
-    si_buf[t] = idx;
-    __syncthreads();
-    for (int s = T / 2; s > 0; s >>= 1) {
-        if (t < s) {
-            mf::V<N> a = lds_get<N>(&sv_buf[t * N]), b = lds_get<N>(&sv_buf[(t + s) * N]);
-            int64_t ia = si_buf[t], ib = si_buf[t + s];
-            bool take_b;
-            if (ib < 0) take_b = false;
-            else if (ia < 0) take_b = true;
-            else if (mf::gt(b, a)) take_b = true;
-            else if (mf::eq(a, b)) take_b = ib < ia;
-            else take_b = false;
-            if (take_b) { lds_put<N>(&sv_buf[t * N], b); si_buf[t] = ib; }
-        }
-        __syncthreads();
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = (int)(blockDim.x >> 6);
+    auto better = [](const mf::V<N> &b, int64_t ib, const mf::V<N> &a, int64_t ia) {
+        if (ib < 0) return false;
+        if (ia < 0) return true;
+        if (mf::gt(b, a)) return true;
+        return mf::eq(a, b) && ib < ia;
+    };
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        mf::V<N> b;
+#pragma unroll
+        for (int l = 0; l < N; ++l) b.x[l] = __shfl_xor(val.x[l], off, 64);
+        const int64_t ib = __shfl_xor(idx, off, 64);
+        if (better(b, ib, val, idx)) { val = b; idx = ib; }
     }
+    if (lane == 0) { lds_put<N>(&sv_buf[wave * N], val); si_buf[wave] = idx; }
+    __syncthreads();
     val = lds_get<N>(&sv_buf[0]);
     idx = si_buf[0];
+    for (int w = 1; w < nw; ++w) {
+        const mf::V<N> b = lds_get<N>(&sv_buf[w * N]);
+        const int64_t ib = si_buf[w];
+        if (better(b, ib, val, idx)) { val = b; idx = ib; }
+    }
     __syncthreads();
 }
 
