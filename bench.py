@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--limbs", type=int, default=4)
-    ap.add_argument("--tile", type=int, default=-1, help="GEMM output tile: -1 auto, 0 16x16, 1 32x32, 2 32x64")
+    ap.add_argument("--tile", type=int, default=-1, help="GEMM output tile: -1 auto, 0 16x16, 1 32x32, 2 32x64, 3 64x64 (x1/x2)")
     ap.add_argument("--cpu-cols", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -154,13 +154,20 @@ static hipError_t pick(hipStream_t s, int tile, int64_t rows, int64_t n, int64_t
     // thread, eight independent MAC chains per thread); tile 0: 16x16 tiles
     // (1x1 per thread) for small outputs where larger tiles would leave most
     // CUs idle.
-    if (tile == 2) return go<N, NET, 2, 4, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
+    // tile 3: 64x64 tiles (4x4 per thread) for Float64x1/x2, whose short MAC
+    // networks need sixteen chains per LDS read to keep the FP64 pipes busy.
+    if constexpr (N <= 2) {
+        if (tile == 3) return go<N, NET, 4, 4, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
+    }
+    if (tile == 2 || tile == 3) return go<N, NET, 2, 4, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
     if (tile == 1) return go<N, NET, 2, 2, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
     return go<N, NET, 1, 1, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
 }
 
-static int auto_tile(int64_t rows, int64_t n) {
-    // 32x32 tiles once they give at least ~2 workgroups per CU (256 CUs).
+static int auto_tile(int N, int64_t rows, int64_t n) {
+    // 64x64 tiles for up to two limbs, 32x32 tiles otherwise, once they give
+    // at least ~2 workgroups per CU (256 CUs).
+    if (N <= 2 && cdiv(rows, 64) * cdiv(n, 64) >= 512) return 3;
     return (cdiv(rows, 32) * cdiv(n, 32) >= 512) ? 1 : 0;
 }
 
@@ -170,7 +177,7 @@ hipError_t tile_launch(hipStream_t s, const TileArgs &t, const DV &A, const DV &
     if (t.rows <= 0 || t.n <= 0) return hipSuccess;
     S<N> al, be;
     for (int l = 0; l < N; ++l) { al.x[l] = alpha ? alpha[l] : 0.0; be.x[l] = beta ? beta[l] : 0.0; }
-    const int tile = t.tile >= 0 ? t.tile : auto_tile(t.rows, t.n);
+    const int tile = t.tile >= 0 ? t.tile : auto_tile(N, t.rows, t.n);
     const int bz = t.beta_zero;
     if (t.mode == MODE_SUB) {
         if (t.lower) return pick<N, mf::NET_STD, MODE_SUB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);

@@ -80,16 +80,16 @@ def test_gemm_matches_oracle(mfla, oracle, N, shape):
             assert_bitwise(host(Cd), ref, f"x{N} {shape} {strat}")
 
 
-@pytest.mark.parametrize("tile", [0, 1])
-def test_gemm_tile_shapes_bitwise_identical(mfla, tile):
-    rng = np.random.default_rng(5)
-    A, B = rand_mf(rng, 4, 97, 40), rand_mf(rng, 4, 40, 75)
-    out = []
-    for t in (tile, 1 - tile):
-        C = torch.zeros((4, 75, 97), dtype=torch.float64, device="cuda")
+@pytest.mark.parametrize("N", [1, 2, 4])
+def test_gemm_tile_shapes_bitwise_identical(mfla, oracle, N):
+    """Every tile shape (16x16, 32x32, 32x64, 64x64) gives MFLA's bits."""
+    rng = np.random.default_rng(5 + N)
+    A, B = rand_mf(rng, N, 131, 40), rand_mf(rng, N, 40, 75)
+    ref = oracle.gemm(np.zeros((N, 75, 131)), A, B, 1.0, 0.0)
+    for t in (0, 1, 2, 3):
+        C = torch.zeros((N, 75, 131), dtype=torch.float64, device="cuda")
         mfla.gemm_(C, dev(A), dev(B), config=mfla.KernelConfig(gemm_tile=t))
-        out.append(host(C))
-    assert_bitwise(out[0], out[1], "tile independence")
+        assert_bitwise(host(C), ref, f"tile {t} x{N}")
 
 
 def test_gemm_dimension_and_alias_errors(mfla):
