@@ -15,3 +15,6 @@ from .factor import (  # noqa: F401
     factor_blocks, factor_diagnostics, factor_inertia, factor_kind, factor_matrix, factor_permutation, factor_pivots,
     factor_precision, factor_provider, factor_state, factor_status, issuccess, numerical_rank, factor_rdiag,
 )
+from .calibration import (  # noqa: F401
+    GemmProfile, calibrate_gemm, default_gemm_profile, machine_fingerprint, profile_compatible, with_gemm_profile,
+)
