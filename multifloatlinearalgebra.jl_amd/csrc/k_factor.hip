@@ -20,58 +20,80 @@ namespace mfk {
 // Panels of up to 32 columns keep their lower triangle in LDS.
 constexpr int CHOL_LDS = 32;
 
+// Diagonal block of one panel (cholesky.jl:10-34), nb <= 32, in LDS. Every
+// element (r, c), r >= c, keeps its running value v = A(r, c) - sum_k
+// L(r, k) L(c, k) in a register and receives term k as soon as column k is
+// final, so a column's critical path is one msub, the square root and the
+// division instead of its whole chain; each element still applies its terms
+// in ascending k, then the square root (diagonal) or the division by the
+// pivot (below it), exactly MFLA's sequence.
+constexpr int CHOL_EPT = (CHOL_LDS * (CHOL_LDS + 1) / 2 + 255) / 256;   // elements per thread
+
 template <int N>
 __global__ __launch_bounds__(256) void k_chol_panel_lds(DV A, int64_t first, int64_t last, Status *st) {
     if (st->info != 0) return;
-    __shared__ double P[CHOL_LDS * CHOL_LDS * N];     // P[(q * 32 + r) * N + l] = A(first + r, first + q)
+    __shared__ double P[CHOL_LDS * CHOL_LDS * N];     // P[(q * 32 + r) * N + l] = L(first + r, first + q)
     __shared__ double piv_s[N];
     __shared__ int fail_s;
     const int nb = (int)(last - first + 1);
     const int t = threadIdx.x;
-    for (int e = t; e < nb * nb; e += blockDim.x) {
-        const int r = e % nb, q = e / nb;
-        if (r >= q) {
-            const mf::V<N> v = ld<N>(A, first + r, first + q);
+    const int count = nb * (nb + 1) / 2;
+    int er[CHOL_EPT], ec[CHOL_EPT];
+    mf::V<N> v[CHOL_EPT];
 #pragma unroll
-            for (int l = 0; l < N; ++l) P[(q * CHOL_LDS + r) * N + l] = v.x[l];
+    for (int i = 0; i < CHOL_EPT; ++i) {
+        const int e = t + 256 * i;
+        er[i] = -1; ec[i] = -1;
+        if (e < count) {                               // packed lower triangle, by columns
+            int c = 0, start = 0;
+            while (start + (nb - c) <= e) { start += nb - c; ++c; }
+            ec[i] = c;
+            er[i] = c + (e - start);
+            v[i] = ld<N>(A, first + er[i], first + c);
+        } else {
+            v[i] = mf::zero<N>();
         }
     }
-    __syncthreads();
     auto get = [&](int r, int q) {
-        mf::V<N> v;
+        mf::V<N> x;
 #pragma unroll
-        for (int l = 0; l < N; ++l) v.x[l] = P[(q * CHOL_LDS + r) * N + l];
-        return v;
+        for (int l = 0; l < N; ++l) x.x[l] = P[(q * CHOL_LDS + r) * N + l];
+        return x;
     };
-    for (int c = 0; c < nb; ++c) {
-        const int r = c + t;
-        mf::V<N> v = mf::zero<N>();
-        if (r < nb) {
-            v = get(r, c);
-            for (int k = 0; k < c; ++k) v = mf::msub(v, get(r, k), get(c, k));
-        }
-        if (t == 0) {
-            if (!mf::gt(v, mf::zero<N>())) {
-                fail_s = 1;
-                st->info = first + c + 1;
-            } else {
-                fail_s = 0;
-                const mf::V<N> p = mf::sqrt(v);
-                stv<N>(A, first + c, first + c, p);
-                lds_put<N>(piv_s, p);
+    for (int k = 0; k < nb; ++k) {
 #pragma unroll
-                for (int l = 0; l < N; ++l) P[(c * CHOL_LDS + c) * N + l] = p.x[l];
+        for (int i = 0; i < CHOL_EPT; ++i) {
+            if (er[i] == k && ec[i] == k) {            // the pivot
+                if (!mf::gt(v[i], mf::zero<N>())) {
+                    fail_s = 1;
+                    st->info = first + k + 1;
+                } else {
+                    fail_s = 0;
+                    const mf::V<N> p = mf::sqrt(v[i]);
+                    stv<N>(A, first + k, first + k, p);
+#pragma unroll
+                    for (int l = 0; l < N; ++l) { piv_s[l] = p.x[l]; P[(k * CHOL_LDS + k) * N + l] = p.x[l]; }
+                }
             }
         }
         __syncthreads();
         if (fail_s) return;
-        if (t > 0 && r < nb) {
-            const mf::V<N> q = mf::div(v, lds_get<N>(piv_s));
-            stv<N>(A, first + r, first + c, q);
+        mf::V<N> p;
 #pragma unroll
-            for (int l = 0; l < N; ++l) P[(c * CHOL_LDS + r) * N + l] = q.x[l];
+        for (int l = 0; l < N; ++l) p.x[l] = piv_s[l];
+#pragma unroll
+        for (int i = 0; i < CHOL_EPT; ++i) {
+            if (ec[i] == k && er[i] > k) {             // column k below the pivot
+                const mf::V<N> q = mf::div(v[i], p);
+                stv<N>(A, first + er[i], first + k, q);
+#pragma unroll
+                for (int l = 0; l < N; ++l) P[(k * CHOL_LDS + er[i]) * N + l] = q.x[l];
+            }
         }
         __syncthreads();
+#pragma unroll
+        for (int i = 0; i < CHOL_EPT; ++i)             // term k of every later element
+            if (ec[i] > k) v[i] = mf::msub(v[i], get(er[i], k), get(ec[i], k));
     }
 }
 
