@@ -167,6 +167,33 @@ __device__ __forceinline__ void sq_index(int64_t e, int64_t w, int64_t &row, int
     }
 }
 
+// Symmetric interchange f <-> s (f <= s) of ldlt.jl:155-181 on the lower
+// triangle of a matrix whose block from `active` on is symmetric: the same
+// lower-triangle values MFLA's full row + column swap leaves (swap rows f, s
+// left of f; the transposed pairs (r, f) <-> (s, r) for f < r < s; columns
+// f, s below s; the two diagonals).
+template <int N>
+__device__ void wg_sym_swap_lower(const DV &A, int64_t n, int64_t f, int64_t s) {
+    if (f == s) return;
+    for (int64_t c = threadIdx.x; c < f; c += blockDim.x) swap_el<N>(A, f, c, s, c);
+    for (int64_t r = f + 1 + threadIdx.x; r < s; r += blockDim.x) swap_el<N>(A, r, f, s, r);
+    for (int64_t r = s + 1 + threadIdx.x; r < n; r += blockDim.x) swap_el<N>(A, r, f, r, s);
+    if (threadIdx.x == 0) swap_el<N>(A, f, f, s, s);
+    __syncthreads();
+}
+
+// Upper triangle of the block [k0, n) from its lower triangle.
+template <int N>
+__device__ void wg_mirror_block(const DV &A, int64_t n, int64_t k0) {
+    const int64_t w = n - k0;
+    for (int64_t e = threadIdx.x; e < w * w; e += blockDim.x) {
+        int64_t r, c;
+        sq_index(e, w, r, c);
+        if (r > c) stv<N>(A, k0 + c, k0 + r, ld<N>(A, k0 + r, k0 + c));
+    }
+    __syncthreads();
+}
+
 // Unblocked right-looking Bunch-Kaufman LDLT (ldlt.jl:352-412), one
 // workgroup per matrix. Matrices are addressed through a batch stride so the
 // same body serves single and batched factorizations. MFLA forms one
@@ -174,22 +201,38 @@ __device__ __forceinline__ void sq_index(int64_t e, int64_t w, int64_t &row, int
 // 2 x 2 step) before its row loop; with cbuf (2 * n * N doubles of LDS, or
 // null) they are formed once per column here too, otherwise per element
 // (same value either way).
+//
+// MFLA updates both triangles at every step. Its upper row i is never
+// touched again once column i is finalized, and at that moment it equals
+// the lower column i before scaling (the trailing block is symmetric after
+// each interchange). So the updates here write the lower triangle only, each
+// upper row is written once, when its column is finalized, and a failure
+// mirrors the still-symmetric trailing block: every element of the returned
+// matrix equals MFLA's, with O(n^2) instead of O(n^3) strided stores.
 template <int N>
 __device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks, const mf::V<N> &alpha,
                                     int64_t *info_out, double *sv_buf, int64_t *si_buf, double *cbuf = nullptr) {
     auto entry = [&](int64_t i, int64_t j) { return ld<N>(A, i, j); };
+    auto fail = [&](int64_t code, int64_t k0) {
+        wg_mirror_block<N>(A, n, k0);
+        if (threadIdx.x == 0) *info_out = code;
+    };
     int64_t k = 0;
     while (k < n) {
         BKChoice ch = wg_select_bk<N>(n, k, alpha, entry, sv_buf, si_buf);
-        if (ch.bs == 0) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+        if (ch.bs == 0) { fail(k + 1, k); return; }
         if (ch.bs == 1) {
             if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 1; }
-            wg_sym_swap<N>(A, n, k, ch.pivot, k);
+            wg_sym_swap_lower<N>(A, n, k, ch.pivot);
             const mf::V<N> d = ld<N>(A, k, k);
-            if (mf::is_zero(d)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
-            for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(ld<N>(A, r, k), d));
+            if (mf::is_zero(d)) { fail(k + 1, k); return; }
+            for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) {
+                const mf::V<N> pre = ld<N>(A, r, k);
+                stv<N>(A, k, r, pre);                                       // MFLA's frozen upper row k
+                stv<N>(A, r, k, mf::div(pre, d));
+            }
             __syncthreads();
-            // rank-1 update (ldlt.jl:221-262), both triangles written
+            // rank-1 update (ldlt.jl:221-262), lower triangle
             const int64_t w = n - k - 1;
             if (cbuf) {
                 for (int64_t j = threadIdx.x; j < w; j += blockDim.x) lds_put<N>(&cbuf[j * N], mf::mul(d, ld<N>(A, k + 1 + j, k)));
@@ -202,20 +245,24 @@ __device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int6
                 const mf::V<N> coef = cbuf ? lds_get<N>(&cbuf[col * N]) : mf::mul(d, ld<N>(A, k + 1 + col, k));
                 row += k + 1;
                 col += k + 1;
-                const mf::V<N> v = mf::msub(ld<N>(A, row, col), ld<N>(A, row, k), coef);
-                stv<N>(A, row, col, v);
-                stv<N>(A, col, row, v);
+                stv<N>(A, row, col, mf::msub(ld<N>(A, row, col), ld<N>(A, row, k), coef));
             }
             __syncthreads();
             k += 1;
         } else {
-            if (!(k < n - 1)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+            if (!(k < n - 1)) { fail(k + 1, k); return; }
             if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 2; blocks[k + 1] = 0; }
-            wg_sym_swap<N>(A, n, k + 1, ch.pivot, k);
+            wg_sym_swap_lower<N>(A, n, k + 1, ch.pivot);
             const mf::V<N> d11 = ld<N>(A, k, k), d21 = ld<N>(A, k + 1, k), d22 = ld<N>(A, k + 1, k + 1);
             if (threadIdx.x == 0) stv<N>(dsub, k, 0, d21);
             mf::V<N> s1, s2;
-            if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+            if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) { fail(k + 1, k); return; }
+            // MFLA's frozen upper rows k and k+1 (pre-solve lower columns)
+            if (threadIdx.x == 0) stv<N>(A, k, k + 1, d21);
+            for (int64_t r = k + 2 + threadIdx.x; r < n; r += blockDim.x) {
+                stv<N>(A, k, r, ld<N>(A, r, k));
+                stv<N>(A, k + 1, r, ld<N>(A, r, k + 1));
+            }
             // rows k+2..n-1: find the first failing row, write only rows before it
             int64_t myfail = INT64_MAX;
             for (int64_t r = k + 2 + threadIdx.x; r < n; r += blockDim.x) {
@@ -229,9 +276,9 @@ __device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int6
                 stv<N>(A, r, k + 1, s2);
             }
             __syncthreads();
-            if (firstfail != INT64_MAX) { if (threadIdx.x == 0) *info_out = k + 1; return; }
+            if (firstfail != INT64_MAX) { fail(k + 1, k + 2); return; }
             // rank-2 update (ldlt.jl:264-319): vector form on four-row blocks,
-            // scalar form on the remainder rows of each column
+            // scalar form on the remainder rows of each column; lower triangle
             if (k + 1 < n - 1) {
                 const int64_t w = n - k - 2;
                 auto coefs = [&](int64_t col, mf::V<N> &c1, mf::V<N> &c2) {
@@ -266,7 +313,6 @@ __device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int6
                     if (row < vec_end) v = mf::sub(ld<N>(A, row, col), mf::add(mf::mul(ld<N>(A, row, k), c1), mf::mul(ld<N>(A, row, k + 1), c2)));
                     else v = mf::msub(mf::msub(ld<N>(A, row, col), ld<N>(A, row, k), c1), ld<N>(A, row, k + 1), c2);
                     stv<N>(A, row, col, v);
-                    stv<N>(A, col, row, v);
                 }
                 __syncthreads();
             }

@@ -170,6 +170,27 @@ def test_ldlt_kkt_bitwise(M, oracle, N, m, strategy, block):
     assert inertia["positive"] == m and inertia["negative"] == n - m and inertia["zero"] == 0
 
 
+@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("zero", [0, 17, 44])
+def test_ldlt_failure_state_bitwise(M, oracle, N, zero):
+    """A zero row / column stops the unblocked Bunch-Kaufman factorization
+    when it becomes the pivot column: the returned matrix (both triangles),
+    pivots, blocks and dsub are MFLA's at that point."""
+    rng = np.random.default_rng(zero + 3 * N)
+    K = kkt(rng, N, 30)
+    K[:, zero, :] = 0.0
+    K[:, :, zero] = 0.0
+    ref = K.copy()
+    info_ref, dsub_ref, piv_ref, blk_ref, _ = oracle.sytrf(ref, "unblocked", 0, 1)
+    assert info_ref > 0
+    Kd = dev(K)
+    F = M.ldlt_(Kd, check=False, config=M.KernelConfig(ldlt_strategy="unblocked"))
+    assert F.info == info_ref
+    assert np.array_equal(F.pivots.cpu().numpy(), piv_ref) and np.array_equal(F.blocks.cpu().numpy(), blk_ref)
+    assert_bitwise(host(F.dsub), dsub_ref, "dsub")
+    assert_bitwise(host(Kd), ref, f"failed ldlt x{N} zero column {zero}")
+
+
 def test_status_codes_and_check(M):
     A = torch.zeros((2, 3, 3), dtype=torch.float64, device="cuda")
     A[0, 1, 2] = float("nan")                       # (row 2, col 1): lower triangle
