@@ -1066,58 +1066,112 @@ mfk::DGemmArgs dg(int64_t m, int64_t n, int64_t k, const DV &A, const DV &B, con
         if (e_ != hipSuccess) return e_;    \
     } while (0)
 
+// Side stream and events of the handle for a binary64 look-ahead (none if
+// the stream cannot be created: the factorizations then run in order).
+struct LookAhead {
+    hipStream_t side;
+    hipEvent_t ev_main, ev_side;
+};
+LookAhead lookahead(mfla_handle h);
+
 constexpr int64_t kDiagBlock = 64;
 
 // Blocked right-looking binary64 Cholesky of F's lower triangle; the inverse
 // of every 64 x 64 diagonal factor block lands in the same block of Inv.
-hipError_t dpotrf_core(hipStream_t s, int64_t n, const DV &F, const DV &Inv, Status *st) {
+// With a side stream the trailing update is split as in dgetrf_core: the
+// next block's 64 columns on the main stream (ahead of its diagonal
+// factorization), the rest of the lower triangle on the side stream. The
+// main stream waits for the previous side update before it writes the next
+// block's columns, the only region both streams touch.
+hipError_t dpotrf_core(hipStream_t s, int64_t n, const DV &F, const DV &Inv, Status *st, const LookAhead *la) {
+    bool side_pending = false;
     for (int64_t j0 = 0; j0 < n; j0 += kDiagBlock) {
         const int nbk = (int)std::min<int64_t>(kDiagBlock, n - j0);
         HIPRET(mfk::dpotrf_diag(s, F, Inv, j0, nbk, st));
-        const int64_t rest = n - j0 - nbk;
+        const int64_t jend = j0 + nbk, rest = n - jend;
         if (rest == 0) break;
-        const DV A21 = sub(F, j0 + nbk, j0);
+        const DV A21 = sub(F, jend, j0);
         // L21 = A21 * inv(L11)^T in place: a 64-row output tile spans all nbk
         // (<= 64) columns of its rows, reads them completely before it writes,
         // and no other tile touches those rows.
         mfk::DGemmArgs g = dg(rest, nbk, nbk, A21, dT(sub(Inv, j0, j0)), A21, 1.0, 0.0);
         g.b_tri = mfk::TRI_UPPER;
         HIPRET(mfk::dgemm(s, g, 1));
-        // A22 -= L21 * L21^T on the lower triangle
-        g = dg(rest, rest, nbk, A21, dT(A21), sub(F, j0 + nbk, j0 + nbk), -1.0, 1.0);
+        if (side_pending) {
+            HIPRET(hipStreamWaitEvent(s, la->ev_side, 0));
+            side_pending = false;
+        }
+        // A22 -= L21 * L21^T on the lower triangle: the next block's columns
+        // here, the rest on the side stream
+        const int64_t next = la ? std::min<int64_t>(kDiagBlock, rest) : rest;
+        g = dg(rest, next, nbk, A21, dT(sub(F, jend, j0)), sub(F, jend, jend), -1.0, 1.0);
         g.c_lower = 1;
         HIPRET(mfk::dgemm(s, g, 1));
+        if (next < rest) {
+            HIPRET(hipEventRecord(la->ev_main, s));
+            HIPRET(hipStreamWaitEvent(la->side, la->ev_main, 0));
+            const DV L2 = sub(F, jend + next, j0);
+            g = dg(rest - next, rest - next, nbk, L2, dT(L2), sub(F, jend + next, jend + next), -1.0, 1.0);
+            g.c_lower = 1;
+            HIPRET(mfk::dgemm(la->side, g, 1));
+            HIPRET(hipEventRecord(la->ev_side, la->side));
+            side_pending = true;
+        }
     }
+    if (side_pending) HIPRET(hipStreamWaitEvent(s, la->ev_side, 0));
     return hipSuccess;
 }
 
 // Blocked right-looking binary64 LU with partial pivoting of the n x n F
 // (LAPACK dgetrf semantics: ipiv 1-based, info = first zero pivot column).
 // Each 64-column block is factored by register-resident leaves of
-// dlu_leaf_width(n) columns, right-looking inside the block (swap + unit
-// solve of the block's later columns, MFMA update below); the block's
-// interchanges then reach the columns left and right of it, the right part
-// gets U12 = L11^{-1} P A12 and the trailing matrix A22 -= L21 U12 on the
-// FP64 matrix cores. The per-column work stays inside one workgroup per leaf,
-// so a 4096 x 4096 factor costs 4096 leaf columns plus 64 block updates.
-hipError_t dgetrf_core(hipStream_t s, int64_t n, const DV &F, int64_t *ipiv, Status *st) {
-    const int W = mfk::dlu_leaf_width(n);
-    if (W == 0) return hipErrorInvalidValue;
+// dlu_leaf_width(n - j0) columns; a leaf also interchanges and solves the
+// block's other columns, and an MFMA product updates the block's later
+// columns below it. The block's interchanges then reach the columns left
+// and right of it (dlu_rows, which also forms U12 = L11^{-1} P A12). With a
+// side stream, the trailing update A22 -= L21 U12 is split: the next
+// block's 64 columns on the main stream, the rest on the side stream, where
+// it overlaps the next block's leaves (the main stream waits for it before
+// the next dlu_rows, the first call that touches those columns or reorders
+// the rows of L21).
+
+hipError_t dgetrf_core(hipStream_t s, int64_t n, const DV &F, int64_t *ipiv, Status *st, const LookAhead *la) {
+    if (mfk::dlu_leaf_width(n) == 0) return hipErrorInvalidValue;
+    bool side_pending = false;
     for (int64_t j0 = 0; j0 < n; j0 += kDiagBlock) {
         const int64_t jb = std::min<int64_t>(kDiagBlock, n - j0), jend = j0 + jb;
+        const int W = mfk::dlu_leaf_width(n - j0);
         for (int64_t c0 = j0; c0 < jend; c0 += W) {
             const int w = (int)std::min<int64_t>(W, jend - c0);
-            HIPRET(mfk::dlu_leaf(s, F, n, c0, w, ipiv, st));
-            HIPRET(mfk::dlu_swap_solve(s, F, c0, w, ipiv, j0, c0 - j0, c0 + w, jend - c0 - w));
-            if (c0 + w < jend && c0 + w < n)
+            HIPRET(mfk::dlu_leaf(s, F, n, c0, w, ipiv, st, j0, (int)(c0 - j0), c0 + w, (int)(jend - c0 - w)));
+            if (c0 + w < jend)
                 HIPRET(mfk::dgemm(s, dg(n - c0 - w, jend - c0 - w, w, sub(F, c0 + w, c0), sub(F, c0, c0 + w), sub(F, c0 + w, c0 + w),
                                         -1.0, 1.0), 1));
         }
-        HIPRET(mfk::dlu_swap_solve(s, F, j0, (int)jb, ipiv, 0, j0, jend, n - jend));
-        if (jend < n)
-            HIPRET(mfk::dgemm(s, dg(n - jend, n - jend, jb, sub(F, jend, j0), sub(F, j0, jend), sub(F, jend, jend), -1.0, 1.0), 1));
+        if (side_pending) {
+            HIPRET(hipStreamWaitEvent(s, la->ev_side, 0));
+            side_pending = false;
+        }
+        HIPRET(mfk::dlu_rows(s, F, n, j0, (int)jb, ipiv));
+        if (jend == n) break;
+        const int64_t next = la ? std::min<int64_t>(kDiagBlock, n - jend) : n - jend;
+        HIPRET(mfk::dgemm(s, dg(n - jend, next, jb, sub(F, jend, j0), sub(F, j0, jend), sub(F, jend, jend), -1.0, 1.0), 1));
+        if (jend + next < n) {
+            HIPRET(hipEventRecord(la->ev_main, s));
+            HIPRET(hipStreamWaitEvent(la->side, la->ev_main, 0));
+            HIPRET(mfk::dgemm(la->side, dg(n - jend, n - jend - next, jb, sub(F, jend, j0), sub(F, j0, jend + next),
+                                           sub(F, jend, jend + next), -1.0, 1.0), 1));
+            HIPRET(hipEventRecord(la->ev_side, la->side));
+            side_pending = true;
+        }
     }
+    if (side_pending) HIPRET(hipStreamWaitEvent(s, la->ev_side, 0));
     return hipSuccess;
+}
+
+LookAhead lookahead(mfla_handle h) {
+    if (ensure_side(h) != hipSuccess) return LookAhead{nullptr, nullptr, nullptr};
+    return LookAhead{h->side, h->ev_panel, h->ev_update};
 }
 
 // Recursive doubling from inverted diagonal blocks of size nb to the full
@@ -1282,7 +1336,8 @@ int mfla_dpotrf_inverse(mfla_handle h, mfla_matrix A, mfla_matrix Inv, int64_t *
     hipStream_t s = h->stream;
     HIPCHK(mfk::status_reset(s, h->status));
     HIPCHK(mfk::finite_check(1, s, n, n, dvn(A), 1, h->status));
-    HIPCHK(dpotrf_core(s, n, dvn(A), dvn(Inv), h->status));
+    const LookAhead la = lookahead(h);
+    HIPCHK(dpotrf_core(s, n, dvn(A), dvn(Inv), h->status, la.side ? &la : nullptr));
     const int64_t r = read_info(h);
     if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
     *info = r;
@@ -1303,7 +1358,8 @@ int mfla_dgetrf(mfla_handle h, mfla_matrix A, int64_t *ipiv, int64_t *info) {
     HIPCHK(mfk::status_reset(s, h->status));
     HIPCHK(mfk::iota1(s, ipiv, n));
     HIPCHK(mfk::finite_check(1, s, n, n, dvn(A), 0, h->status));
-    HIPCHK(dgetrf_core(s, n, dvn(A), ipiv, h->status));
+    const LookAhead la = lookahead(h);
+    HIPCHK(dgetrf_core(s, n, dvn(A), ipiv, h->status, la.side ? &la : nullptr));
     const int64_t r = read_info(h);
     if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
     *info = r;
@@ -1359,7 +1415,8 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
         HIPCHK(mfk::status_reset(s, h->status));
         HIPCHK(mfk::dcopy(s, n, n, F, Ahi, 0));
         HIPCHK(mfk::finite_check(1, s, n, n, F, 0, h->status));
-        HIPCHK(dgetrf_core(s, n, F, w.ipiv, h->status));
+        const LookAhead la = lookahead(h);
+        HIPCHK(dgetrf_core(s, n, F, w.ipiv, h->status, la.side ? &la : nullptr));
         const int64_t r = read_info(h);
         if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
         *info = r;
@@ -1369,7 +1426,8 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
         HIPCHK(mfk::status_reset(s, h->status));
         HIPCHK(mfk::dcopy(s, n, n, F, Ahi, uplo == MFLA_UPPER ? 2 : 0));
         HIPCHK(mfk::finite_check(1, s, n, n, F, 1, h->status));
-        HIPCHK(dpotrf_core(s, n, F, Inv, h->status));
+        const LookAhead la = lookahead(h);
+        HIPCHK(dpotrf_core(s, n, F, Inv, h->status, la.side ? &la : nullptr));
         const int64_t r = read_info(h);
         if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
         *info = r;

@@ -3,13 +3,16 @@
 // constraints), so the GEMM-shaped work goes to the CDNA4 FP64 matrix cores:
 //
 //  * k_dgemm: C = alpha * op(A) * op(B) + beta * C on strided views, 64x64
-//    tile per 256-thread workgroup (four waves, each 32x32 = 2x2 MFMA
-//    16x16x4 f64 blocks), A/B k-panels of 16 staged in LDS. Optional
-//    triangular structure of A / B (the k range of each tile is clipped and
-//    the zero triangle is never read), a lower-triangular output mask, a
-//    batch index (grid z) with element strides, and a split-k mode that
-//    writes partial sums for k_dgemm_reduce (fixed summation order, so the
-//    result is deterministic).
+//    tile per 256-thread workgroup, each wave a 32x32 block of 2x2 MFMA
+//    16x16x4 f64 accumulators. No LDS and no barriers in the k loop: every
+//    lane loads its own operand values straight from global memory (L1/L2
+//    serve the reuse between the waves of a tile) one 16-deep k chunk
+//    ahead of the MFMAs that consume them, and C is prefetched before the
+//    loop. Optional triangular structure of A / B (the k range of each tile
+//    is clipped and the zero triangle is never read), a lower-triangular
+//    output mask, a batch index (grid z) with element strides, and a split-k
+//    mode that writes partial sums for k_dgemm_reduce (fixed summation
+//    order, so the result is deterministic).
 //  * k_dpotrf_diag: Cholesky of one nb x nb (nb <= 64) diagonal block in LDS,
 //    plus the inverse of the resulting triangle (needed by the panel update
 //    L21 = A21 * inv(L11)^T and by the correction solve).
@@ -22,19 +25,51 @@ namespace mfk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int DG_T = 64, DG_BK = 16, DG_PAD = 1;
+constexpr int DG_T = 64, DG_BK = 16;
 
 // v_mfma_f64_16x16x4_f64 operand layout (lane l of the wave), as measured
 // on gfx950 with integer-valued operands:
 //   first operand:  P[l % 16][l / 16]   (16 x 4)
 //   second operand: Q[l / 16][l % 16]   (4 x 16)
 //   D item v:       D[l / 16 + 4 * v][l % 16]   (D = P Q)
-// The kernel forms the transposed tile D = B^T A^T (P = B^T, Q = A^T), so
+// The kernel forms the transposed block D = B^T A^T (P = B^T, Q = A^T), so
 // lane l holds C rows i = l % 16: the 16 lanes of a row group store 16
-// consecutive elements of a column-major C column.
+// consecutive elements of a column-major C column. Inside a 16-deep chunk
+// at k0, MFMA step s gives slot q = l / 16 the index k = k0 + 4 q + s, so
+// each lane reads four consecutive k of its row of A / column of B.
+struct DgOps {
+    double a[2][4], b[2][4];
+};
+
+// Operands of one chunk: pa[h] / pb[h] address A(ib + 16 h, k) / B(k, jb + 16 h)
+// at this lane's first k; indices are 32-bit (m, n, k < 2^31).
+__device__ __forceinline__ void dg_load(const DGemmArgs &g, const double *const (&pa)[2], const double *const (&pb)[2], int ib, int jb,
+                                        int k, int khi, DgOps &o) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int kk = k + s, i = ib + 16 * h, j = jb + 16 * h;
+            bool oa = i < (int)g.m && kk < khi, ob = j < (int)g.n && kk < khi;
+            if (g.a_tri == TRI_LOWER) oa = oa && kk <= i;
+            if (g.a_tri == TRI_UPPER) oa = oa && kk >= i;
+            if (g.b_tri == TRI_LOWER) ob = ob && kk >= j;
+            if (g.b_tri == TRI_UPPER) ob = ob && kk <= j;
+            o.a[h][s] = oa ? pa[h][s * g.A.cs] : 0.0;
+            o.b[h][s] = ob ? pb[h][s * g.B.rs] : 0.0;
+        }
+}
+
+__device__ __forceinline__ void dg_mma(const DgOps &o, d4 (&acc)[2][2]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.b[bj][s], o.a[bi][s], acc[bi][bj], 0, 0, 0);
+}
+
 __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
-    __shared__ double As[DG_BK][DG_T + DG_PAD];
-    __shared__ double Bs[DG_BK][DG_T + DG_PAD];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wy = w & 1, wx = w >> 1;
     const int64_t tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
@@ -42,7 +77,7 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
     const int64_t z = blockIdx.z;
     const double *Ap = g.A.p + z * g.za, *Bp = g.B.p + z * g.zb;
     double *Cp = g.C.p + z * g.zc;
-    if (g.c_lower && j0 > i0 + DG_T - 1) return;  // tile entirely above the diagonal
+    if (g.c_lower && j0 > i0 + DG_T - 1) return;  // tile entirely above the diagonal (uniform)
 
     // k range of this tile (zero triangles of A / B contribute nothing)
     int64_t klo = 0, khi = g.k;
@@ -60,52 +95,62 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
         khi = b < khi ? b : khi;
     }
 
+    const int64_t ib = i0 + wy * 32 + (lane & 15), jb = j0 + wx * 32 + (lane & 15);
+    const int64_t jc = j0 + wx * 32 + (lane >> 4);   // C column of item v of block bj: jc + 16 bj + 4 v
+    const int kq = 4 * (lane >> 4);
+
+    // C values for the epilogue, fetched before the k loop
+    double cv[2][2][4];
+    const bool readc = !g.beta_zero && g.ksplit == 1;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t i = ib + 16 * bi, j = jc + 16 * bj + 4 * v;
+                const bool ok = readc && i < g.m && j < g.n && !(g.c_lower && i < j);
+                cv[bi][bj][v] = ok ? Cp[i * g.C.rs + j * g.C.cs] : 0.0;
+            }
+
     d4 acc[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
 
-    const bool a_rows_fast = g.A.rs == 1, b_cols_fast = g.B.cs == 1;
-    for (int64_t k0 = klo; k0 < khi; k0 += DG_BK) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            int ii, kk;
-            if (a_rows_fast) { ii = t & 63; kk = (t >> 6) + 4 * q; }
-            else { kk = t & 15; ii = (t >> 4) + 16 * q; }
-            const int64_t gi = i0 + ii, gk = k0 + kk;
-            bool ok = gi < g.m && gk < khi;
-            if (g.a_tri == TRI_LOWER) ok = ok && gk <= gi;
-            if (g.a_tri == TRI_UPPER) ok = ok && gk >= gi;
-            As[kk][ii] = ok ? Ap[gi * g.A.rs + gk * g.A.cs] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            int jj, kk;
-            if (b_cols_fast) { jj = t & 63; kk = (t >> 6) + 4 * q; }
-            else { kk = t & 15; jj = (t >> 4) + 16 * q; }
-            const int64_t gj = j0 + jj, gk = k0 + kk;
-            bool ok = gj < g.n && gk < khi;
-            if (g.b_tri == TRI_LOWER) ok = ok && gk >= gj;
-            if (g.b_tri == TRI_UPPER) ok = ok && gk <= gj;
-            Bs[kk][jj] = ok ? Bp[gk * g.B.rs + gj * g.B.cs] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k4 = 0; k4 < DG_BK / 4; ++k4) {
-            const int kr = k4 * 4 + (lane >> 4);
-            double a[2], b[2];
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi) a[bi] = As[kr][wy * 32 + bi * 16 + (lane & 15)];
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj) b[bj] = Bs[kr][wx * 32 + bj * 16 + (lane & 15)];
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[bj], a[bi], acc[bi][bj], 0, 0, 0);
-        }
-        __syncthreads();
+    // two operand sets: the next chunk is in flight while the current one
+    // feeds the MFMAs
+    DgOps o0, o1;
+    // per-lane operand pointers at the chunk being loaded; a chunk advances
+    // them by 16 k
+    const double *pa[2], *pb[2];
+    {
+        const int64_t k = klo + kq;
+        pa[0] = Ap + ib * g.A.rs + k * g.A.cs;
+        pa[1] = pa[0] + 16 * g.A.rs;
+        pb[0] = Bp + jb * g.B.cs + k * g.B.rs;
+        pb[1] = pb[0] + 16 * g.B.cs;
     }
+    const int64_t sa = DG_BK * g.A.cs, sb = DG_BK * g.B.rs;
+    const int ib32 = (int)ib, jb32 = (int)jb, khi32 = (int)khi;
+    int kn = (int)klo + kq;
+    auto advance = [&]() {
+        pa[0] += sa; pa[1] += sa; pb[0] += sb; pb[1] += sb;
+        kn += DG_BK;
+    };
+    if (klo < khi) { dg_load(g, pa, pb, ib32, jb32, kn, khi32, o0); advance(); }
+    for (int64_t k0 = klo; k0 < khi; k0 += 2 * DG_BK) {
+        const bool more = k0 + DG_BK < khi;
+        if (more) { dg_load(g, pa, pb, ib32, jb32, kn, khi32, o1); advance(); }
+        dg_mma(o0, acc);
+        if (!more) break;
+        if (k0 + 2 * DG_BK < khi) { dg_load(g, pa, pb, ib32, jb32, kn, khi32, o0); advance(); }
+        dg_mma(o1, acc);
+    }
+    // C may alias A (the in-place panel product L21 = A21 inv(L11)^T): every
+    // wave has read its operands before any wave of the tile writes.
+    __syncthreads();
 
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
@@ -113,8 +158,7 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t i = i0 + wy * 32 + bi * 16 + (lane & 15);
-                const int64_t j = j0 + wx * 32 + bj * 16 + (lane >> 4) + 4 * v;
+                const int64_t i = ib + 16 * bi, j = jc + 16 * bj + 4 * v;
                 if (i >= g.m || j >= g.n) continue;
                 if (g.c_lower && i < j) continue;
                 const double r = acc[bi][bj][v];
@@ -122,7 +166,7 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
                     g.P[((int64_t)ks * g.n + j) * g.m + i] = r;
                 } else {
                     double *c = Cp + i * g.C.rs + j * g.C.cs;
-                    *c = g.beta_zero ? g.alpha * r : g.alpha * r + g.beta * *c;
+                    *c = g.beta_zero ? g.alpha * r : g.alpha * r + g.beta * cv[bi][bj][v];
                 }
             }
 }
@@ -267,98 +311,169 @@ __global__ void k_dgather_rows(int64_t m, int64_t ncols, DV dst, DV src, const i
 
 // ------------------------------------------------- binary64 LU (leaves)
 // k_dlu_leaf: partial-pivoting LU of the w (<= W) columns [c0, c0 + w) over
-// rows [c0, m) by one 1024-thread workgroup with the panel in registers:
-// thread t holds rows c0 + t + 1024 i (i < RPT). Per column: workgroup argmax
-// of |a| (first maximum on ties), exchange of the pivot row through LDS,
-// scaling by the reciprocal pivot and a rank-1 update of the leaf's later
-// columns. ipiv gets 1-based global row indices. A zero (or NaN-only) pivot
-// column records info = column + 1 if no failure is recorded yet and is
-// skipped, as dgetf2 skips it.
-constexpr int LU_THREADS = 1024, LU_WAVES = LU_THREADS / 64;
+// rows [c0, m) by one 512-thread workgroup with the panel in registers:
+// thread t holds rows c0 + t + 512 i (i < RPT); two waves per SIMD keep the
+// per-column instruction stream short. One barrier per column: every wave
+// reduces its (|a|, row) candidates with DPP row rotations and lane reads
+// (maximum first, then the smallest row holding it), and the lane holding the wave's
+// winner publishes that whole row (and the owner of row j publishes row j)
+// in a double-buffered LDS slot before the barrier, so after it every
+// thread picks the pivot, swaps, scales by the reciprocal pivot and applies
+// the rank-1 update of the leaf's later columns without a second exchange.
+// ipiv gets 1-based global row indices; a zero (or NaN-only) pivot column
+// records info = column + 1 if no failure is recorded yet and is skipped, as
+// dgetf2 skips it.
+//
+// The leaf also brings the other columns of its 64-column block up to date:
+// a row id travels with every panel row, so the final permutation of the
+// leaf's rows is known; columns [a0, a0 + na) get the row interchanges and
+// columns [b0, b0 + nb) get them and the unit lower solve of their top w
+// rows (U12 = L11^{-1} P A12). All reads of those columns happen before a
+// barrier and all writes after it.
+constexpr int LU_THREADS = 512, LU_WAVES = LU_THREADS / 64;
 
-__device__ __forceinline__ void lu_pick(double &best, int &bi, double ob, int oi) {
-    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+template <int CTRL>
+__device__ __forceinline__ double lu_dpp_d(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Every lane of a 16-lane row ends with the row's maximum (quad
+// permutations, then rotations by 4 and 8). The values are |a| or -1; a NaN
+// never wins (fmax returns the other operand).
+__device__ __forceinline__ double lu_row_max(double x) {
+    x = fmax(x, lu_dpp_d<0xB1>(x));    // quad_perm [1, 0, 3, 2]
+    x = fmax(x, lu_dpp_d<0x4E>(x));    // quad_perm [2, 3, 0, 1]
+    x = fmax(x, lu_dpp_d<0x124>(x));   // row_ror 4
+    return fmax(x, lu_dpp_d<0x128>(x)); // row_ror 8
+}
+
+__device__ __forceinline__ int lu_row_min(int i) {
+    i = min(i, __builtin_amdgcn_update_dpp(0, i, 0xB1, 0xf, 0xf, false));
+    i = min(i, __builtin_amdgcn_update_dpp(0, i, 0x4E, 0xf, 0xf, false));
+    i = min(i, __builtin_amdgcn_update_dpp(0, i, 0x124, 0xf, 0xf, false));
+    return min(i, __builtin_amdgcn_update_dpp(0, i, 0x128, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ double lu_readlane(double x, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l), __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
+// Wave argmax in two passes: the maximum value, then the smallest index
+// among the lanes holding it (LAPACK's first maximum). Every lane ends with
+// the result.
+__device__ __forceinline__ void lu_wave_argmax(double &best, int &bi) {
+    const double r = lu_row_max(best);
+    const double m = fmax(fmax(lu_readlane(r, 0), lu_readlane(r, 16)), fmax(lu_readlane(r, 32), lu_readlane(r, 48)));
+    const int c = lu_row_min(best == m ? bi : INT32_MAX);
+    bi = min(min(__builtin_amdgcn_readlane(c, 0), __builtin_amdgcn_readlane(c, 16)),
+             min(__builtin_amdgcn_readlane(c, 32), __builtin_amdgcn_readlane(c, 48)));
+    best = m;
 }
 
 template <int RPT, int W>
-__global__ __launch_bounds__(LU_THREADS) void k_dlu_leaf(DV F, int64_t m, int64_t c0, int w, int64_t *ipiv, Status *st) {
-    __shared__ double red_v[LU_WAVES];
-    __shared__ int red_i[LU_WAVES];
-    __shared__ double prow[W], jrow[W];
+__global__ __launch_bounds__(LU_THREADS) void k_dlu_leaf(DV F, int64_t m, int64_t c0, int w, int64_t *ipiv, Status *st, int64_t a0, int na,
+                                                         int64_t b0, int nb) {
+    __shared__ double cand[2][LU_WAVES][W], cval[2][LU_WAVES], jrow[2][W];
+    __shared__ int cidx[2][LU_WAVES], crid[2][LU_WAVES], jrid[2];
+    __shared__ double l11[W][W];
+    __shared__ int rtop[W], mpos[W], msrc[W], nmoved;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int mr = (int)(m - c0);   // <= 32768
     double v[RPT][W];
+    int rid[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
         const int r = t + LU_THREADS * i;
+        rid[i] = r;
 #pragma unroll
         for (int j = 0; j < W; ++j) v[i][j] = (r < mr && j < w) ? F.p[(c0 + r) * F.rs + (c0 + j) * F.cs] : 0.0;
     }
+    if (t == 0) nmoved = 0;
+    int piv[W];   // local pivot row of each column (uniform); stored after the loop so that no
+                  // global store is outstanding at the column barriers
 #pragma unroll
     for (int j = 0; j < W; ++j) {
         if (j < w) {                                         // uniform
+            const int par = j & 1;
             double best = -1.0;
             int bi = INT32_MAX;
 #pragma unroll
-            for (int i = 0; i < RPT; ++i) {
+            for (int i = 0; i < RPT; ++i) {            // rows ascending: first maximum kept
                 const int r = t + LU_THREADS * i;
-                if (r < mr && r >= j) lu_pick(best, bi, fabs(v[i][j]), r);
+                const double a = fabs(v[i][j]);
+                const bool take = r < mr && r >= j && a > best;
+                best = take ? a : best;
+                bi = take ? r : bi;
             }
+            lu_wave_argmax(best, bi);
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) lu_pick(best, bi, __shfl_xor(best, off), __shfl_xor(bi, off));
-            if (lane == 0) { red_v[wave] = best; red_i[wave] = bi; }
+            for (int i = 0; i < RPT; ++i)
+                if (t + LU_THREADS * i == bi) {
+#pragma unroll
+                    for (int jj = 0; jj < W; ++jj) cand[par][wave][jj] = v[i][jj];
+                    crid[par][wave] = rid[i];
+                }
+            if (lane == 0) { cval[par][wave] = best; cidx[par][wave] = bi; }
+            if (t == j) {
+#pragma unroll
+                for (int jj = 0; jj < W; ++jj) jrow[par][jj] = v[0][jj];
+                jrid[par] = rid[0];
+            }
             __syncthreads();
-            best = red_v[0]; bi = red_i[0];
-#pragma unroll
-            for (int k = 1; k < LU_WAVES; ++k) lu_pick(best, bi, red_v[k], red_i[k]);
+            // the wave winners: one LDS read per lane of row 0, a row reduction
+            {
+                const double cb = lane < LU_WAVES ? cval[par][lane] : -1.0;
+                const int ci = lane < LU_WAVES ? cidx[par][lane] : INT32_MAX;
+                best = lu_readlane(lu_row_max(cb), 0);
+                bi = __builtin_amdgcn_readlane(lu_row_min(cb == best ? ci : INT32_MAX), 0);
+            }
+            const int ww = (bi & (LU_THREADS - 1)) >> 6;     // the wave holding row bi (when best > 0)
             if (best > 0.0) {                                // uniform
-                const int pt = bi % LU_THREADS, pi = bi / LU_THREADS;
-                if (t == pt) {
-#pragma unroll
-                    for (int i = 0; i < RPT; ++i)
-                        if (i == pi) {
-#pragma unroll
-                            for (int jj = 0; jj < W; ++jj) prow[jj] = v[i][jj];
-                        }
-                }
-                if (t == j) {
-#pragma unroll
-                    for (int jj = 0; jj < W; ++jj) jrow[jj] = v[0][jj];
-                }
-                __syncthreads();
+                const double *prow = cand[par][ww];
                 if (bi != j) {
                     if (t == j) {
 #pragma unroll
                         for (int jj = 0; jj < W; ++jj) v[0][jj] = prow[jj];
+                        rid[0] = crid[par][ww];
                     }
-                    if (t == pt) {
 #pragma unroll
-                        for (int i = 0; i < RPT; ++i)
-                            if (i == pi) {
+                    for (int i = 0; i < RPT; ++i)
+                        if (t + LU_THREADS * i == bi) {
 #pragma unroll
-                                for (int jj = 0; jj < W; ++jj) v[i][jj] = jrow[jj];
-                            }
+                            for (int jj = 0; jj < W; ++jj) v[i][jj] = jrow[par][jj];
+                            rid[i] = jrid[par];
+                        }
+                }
+                piv[j] = bi;
+                const double inv = 1.0 / prow[j];
+#pragma unroll
+                for (int jj = j + 1; jj < W; ++jj) {
+                    const double u = prow[jj];
+#pragma unroll
+                    for (int i = 0; i < RPT; ++i) {
+                        const int r = t + LU_THREADS * i;
+                        if (r < mr && r > j) v[i][jj] = fma(-(v[i][j] * inv), u, v[i][jj]);
                     }
                 }
-                if (t == 0) ipiv[c0 + j] = c0 + bi + 1;
-                const double inv = 1.0 / prow[j];
 #pragma unroll
                 for (int i = 0; i < RPT; ++i) {
                     const int r = t + LU_THREADS * i;
-                    if (r < mr && r > j) {
-                        const double l = v[i][j] * inv;
-                        v[i][j] = l;
-#pragma unroll
-                        for (int jj = j + 1; jj < W; ++jj) v[i][jj] = fma(-l, prow[jj], v[i][jj]);
-                    }
+                    if (r < mr && r > j) v[i][j] *= inv;
                 }
-            } else if (t == 0) {                             // zero column
-                if (st->info == 0) st->info = c0 + j + 1;
-                ipiv[c0 + j] = c0 + j + 1;
+            } else {                                         // zero column: no interchange
+                piv[j] = j;
+                if (t == 0 && st->info == 0) st->info = c0 + j + 1;
             }
-            __syncthreads();                                 // LDS reuse
         }
     }
+    __syncthreads();   // the last column's slots are read; nmoved is visible
+
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+        if (t == j && j < w) ipiv[c0 + j] = c0 + piv[j] + 1;
+    // the factored panel
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
         const int r = t + LU_THREADS * i;
@@ -368,40 +483,233 @@ __global__ __launch_bounds__(LU_THREADS) void k_dlu_leaf(DV F, int64_t m, int64_
                 if (j < w) F.p[(c0 + r) * F.rs + (c0 + j) * F.cs] = v[i][j];
         }
     }
-}
-
-// k_dlu_swap_solve: one thread per column. Every column of range A
-// ([a0, a0 + na)) and range B ([b0, b0 + nb)) gets the row interchanges
-// ipiv[k0 .. k0 + kn) in order (dlaswp); range B columns then solve
-// L11 x = (P a)(k0 : k0 + kn) with the unit lower triangle L11 =
-// F(k0 : k0 + kn, k0 : k0 + kn) by forward substitution (x in registers).
-template <int KN>
-__global__ __launch_bounds__(256) void k_dlu_swap_solve(DV F, int64_t k0, int kn, const int64_t *ipiv, int64_t a0, int64_t na,
-                                                        int64_t b0, int64_t nb) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= na + nb) return;
-    const bool solve = e >= na;
-    const int64_t j = solve ? b0 + (e - na) : a0 + e;
-    double *col = F.p + j * F.cs;
-    for (int k = 0; k < kn; ++k) {
-        const int64_t r = k0 + k, p = ipiv[r] - 1;
-        if (p != r) {
-            const double a = col[r * F.rs], b = col[p * F.rs];
-            col[r * F.rs] = b;
-            col[p * F.rs] = a;
+    // row permutation of the leaf: top rows and the moved rows below them
+    if (t < w) {
+        rtop[t] = rid[0];
+#pragma unroll
+        for (int q = 0; q < W; ++q) l11[t][q] = v[0][q];
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int r = t + LU_THREADS * i;
+        if (r < mr && r >= w && rid[i] != r) {           // at most one such row per column
+            const int k = atomicAdd(&nmoved, 1);
+            mpos[k] = r;
+            msrc[k] = rid[i];
         }
     }
-    if (!solve) return;
-    double x[KN];
+    __syncthreads();
+    const int nm = nmoved, ncols = na + nb;
+    const int items = nm * ncols + w * na;               // <= 2 * 16 * 63 < 4 * LU_THREADS
+    double val[4];
+    int64_t dst[4];
 #pragma unroll
-    for (int r = 0; r < KN; ++r) {
-        if (r < kn) {
-            const double *lrow = F.p + (k0 + r) * F.rs + k0 * F.cs;
-            double acc = col[(k0 + r) * F.rs];
+    for (int q = 0; q < 4; ++q) {
+        const int e = t + LU_THREADS * q;
+        dst[q] = -1;
+        if (e < items) {
+            int pos, src, ci;
+            if (e < nm * ncols) {
+                const int mi = e / ncols;
+                ci = e - mi * ncols;
+                pos = mpos[mi];
+                src = msrc[mi];
+            } else {
+                const int e2 = e - nm * ncols, k = e2 / na;
+                ci = e2 - k * na;
+                pos = k;
+                src = rtop[k];
+            }
+            const int64_t col = ci < na ? a0 + ci : b0 + (ci - na);
+            val[q] = F.p[(c0 + src) * F.rs + col * F.cs];
+            dst[q] = (c0 + pos) * F.rs + col * F.cs;
+        }
+    }
+    double x[W];
+    if (t < nb) {
+        const int64_t col = b0 + t;
 #pragma unroll
-            for (int q = 0; q < r; ++q) acc = fma(-lrow[q * F.cs], x[q], acc);
-            x[r] = acc;
-            col[(k0 + r) * F.rs] = acc;
+        for (int k = 0; k < W; ++k)
+            if (k < w) {
+                double a = F.p[(c0 + rtop[k]) * F.rs + col * F.cs];
+#pragma unroll
+                for (int q = 0; q < k; ++q) a = fma(-l11[k][q], x[q], a);
+                x[k] = a;
+            }
+    }
+    __syncthreads();   // every read of the other columns is done
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (dst[q] >= 0) F.p[dst[q]] = val[q];
+    if (t < nb) {
+        const int64_t col = b0 + t;
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+            if (k < w) F.p[(c0 + k) * F.rs + col * F.cs] = x[k];
+    }
+}
+
+// k_dlu_rows: the row interchanges ipiv[j0 .. j0 + kn) of one block (kn <=
+// 64) for the columns left of it ([0, j0)) and right of it ([jend, n)), and
+// for the right columns the unit lower solve U12 = L11^{-1} (P A12). One
+// 256-thread workgroup per 64-column chunk (left chunks first). Wave 0
+// composes the interchange sequence into a table of (position, source row)
+// entries: the kn block rows, then every distinct pivot row below the block
+// (at most kn). The chunk's source rows are gathered into LDS before any
+// write, so the composed permutation is applied in one pass. The solve
+// gives each column a quad of lanes, lane g of the quad owning rows g + 4 i
+// in registers; x_q reaches the quad by a DPP quad broadcast, so no barrier
+// is needed inside the substitution.
+constexpr int DR_LD = 65;
+
+template <int G>
+__device__ __forceinline__ double quad_bcast(double x) {
+    constexpr int ctrl = G | (G << 2) | (G << 4) | (G << 6);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), ctrl, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int Q>
+__device__ __forceinline__ void dr_load(double (&l)[16], const double *L, int g) {
+#pragma unroll
+    for (int i = Q >> 2; i < 16; ++i) l[i] = L[Q * 64 + g + 4 * i];
+}
+
+template <int Q>
+__device__ __forceinline__ void dr_step(double (&x)[16], const double (&l)[16], int g) {
+    // x_Q lives in lane Q % 4 of the quad, slot Q / 4
+    const double xq = quad_bcast<Q & 3>(x[Q >> 2]);
+    {
+        constexpr int i = Q >> 2;
+        if (g > (Q & 3)) x[i] = fma(-l[i], xq, x[i]);
+    }
+#pragma unroll
+    for (int i = (Q >> 2) + 1; i < 16; ++i) x[i] = fma(-l[i], xq, x[i]);
+}
+
+// steps Q .. kn - 1; lc holds column Q of L, column Q + 1 is loaded while
+// step Q runs
+template <int Q>
+__device__ __forceinline__ void dr_solve(double (&x)[16], double (&lc)[16], const double *L, int g, int kn) {
+    if constexpr (Q < 64) {
+        if (Q < kn) {
+            double ln[16];
+            if constexpr (Q + 1 < 64) dr_load<Q + 1>(ln, L, g);
+            dr_step<Q>(x, lc, g);
+            if constexpr (Q + 1 < 64) dr_solve<Q + 1>(x, ln, L, g, kn);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dlu_rows(DV F, int64_t n, int64_t j0, int kn, const int64_t *ipiv, int left_chunks) {
+    __shared__ double T[64 * DR_LD];   // T[k * DR_LD + c]: block row j0 + k as it is now
+    __shared__ double X[64 * DR_LD];   // X[r * DR_LD + c]: moved row xpos[r] as it is now
+    __shared__ double L[64 * 64];      // L[q * 64 + k] = L11(k, q), zero outside the block
+    __shared__ int sp[64], tslot[64], xslot[64], xpos[64], nx;
+    const int t = threadIdx.x, lane = t & 63;
+    const int64_t jend = j0 + kn;
+    const bool right = (int)blockIdx.x >= left_chunks;
+    const int64_t cbeg = right ? jend + 64 * (int64_t)(blockIdx.x - left_chunks) : 64 * (int64_t)blockIdx.x;
+    const int64_t clim = right ? n : j0;
+    const int nc = (int)((clim - cbeg) < 64 ? (clim - cbeg) : 64);
+
+    // ---- the pivot rows below the block, each once (wave 0)
+    int p = -1, slot = 0;
+    if (t < 64) {
+        p = lane < kn ? (int)(ipiv[j0 + lane] - 1) : -1;     // global 0-based pivot row of step lane
+        sp[lane] = p;
+    }
+    __syncthreads();
+    if (t < 64) {
+        const bool extra = lane < kn && p >= (int)jend;
+        int first = lane;                                     // first step with the same pivot row
+#pragma unroll
+        for (int q = 0; q < 64; ++q) {
+            const int pq = sp[q];
+            first = (q < first && pq == p) ? q : first;
+        }
+        const bool isfirst = extra && first == lane;
+        const unsigned long long fm = __ballot(isfirst);
+        const int rank = __popcll(fm & ((1ull << lane) - 1ull));
+        const int rfirst = __shfl(rank, first);
+        slot = lane < kn ? (extra ? 64 + rfirst : p - (int)j0) : lane;   // table slot of step lane's pivot row
+        if (isfirst) xpos[rank] = p;
+        if (lane == 0) nx = __popcll(fm);
+    }
+    __syncthreads();
+    const int nxr = nx;
+    // ---- gather the chunk's rows by position (block rows, moved rows) and
+    // L11; the loads are in flight while wave 0 composes the interchanges
+    const int gk = t & 63, gc = t >> 6;
+    double g1[16], g2[16], g3[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = gc + 4 * q;
+        g1[q] = (gk < kn && c < nc) ? F.p[(j0 + gk) * F.rs + (cbeg + c) * F.cs] : 0.0;
+        g2[q] = (gk < nxr && c < nc) ? F.p[(int64_t)xpos[gk] * F.rs + (cbeg + c) * F.cs] : 0.0;
+        g3[q] = (right && gk < kn && c < kn) ? F.p[(j0 + gk) * F.rs + (j0 + c) * F.cs] : 0.0;
+    }
+    if (t < 64) {
+        // table entries: slot e < 64 is block row j0 + e, slot 64 + r is
+        // moved row xpos[r]; sa / sb hold the slot whose current row ends
+        // at each entry after the interchange sequence
+        int sa = lane, sb = 64 + lane;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {
+            if (k < kn) {
+                const int s = __builtin_amdgcn_readlane(slot, k);
+                const int sl = s & 63;
+                const bool hb = s >= 64;
+                const int a = __builtin_amdgcn_readlane(sa, k);
+                const int b = hb ? __builtin_amdgcn_readlane(sb, sl) : __builtin_amdgcn_readlane(sa, sl);
+                sa = lane == k ? b : sa;
+                sa = (!hb && lane == sl) ? a : sa;
+                sb = (hb && lane == sl) ? a : sb;
+            }
+        }
+        tslot[lane] = sa;
+        xslot[lane] = sb;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        T[gk * DR_LD + gc + 4 * q] = g1[q];
+        X[gk * DR_LD + gc + 4 * q] = g2[q];
+        L[(gc + 4 * q) * 64 + gk] = g3[q];
+    }
+    __syncthreads();
+    // the value at entry e after the interchanges: the row in slot tslot / xslot
+    auto row_of = [&](int sl, int c) -> double { return sl < 64 ? T[sl * DR_LD + c] : X[(sl - 64) * DR_LD + c]; };
+    if (right) {
+        const int c = t >> 2, g = t & 3;
+        double x[16], l0[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = g + 4 * i < kn ? row_of(tslot[g + 4 * i], c) : 0.0;
+        dr_load<0>(l0, L, g);
+        dr_solve<0>(x, l0, L, g, kn);
+        // the moved rows are read from X before any of them is overwritten
+        double mv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int cc = gc + 4 * q;
+            mv[q] = gk < nxr ? row_of(xslot[gk], cc) : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) T[(g + 4 * i) * DR_LD + c] = x[i];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int cc = gc + 4 * q;
+            if (gk < kn && cc < nc) F.p[(j0 + gk) * F.rs + (cbeg + cc) * F.cs] = T[gk * DR_LD + cc];
+            if (gk < nxr && cc < nc && xslot[gk] != 64 + gk) F.p[(int64_t)xpos[gk] * F.rs + (cbeg + cc) * F.cs] = mv[q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int cc = gc + 4 * q;
+            if (gk < kn && cc < nc && tslot[gk] != gk) F.p[(j0 + gk) * F.rs + (cbeg + cc) * F.cs] = row_of(tslot[gk], cc);
+            if (gk < nxr && cc < nc && xslot[gk] != 64 + gk) F.p[(int64_t)xpos[gk] * F.rs + (cbeg + cc) * F.cs] = row_of(xslot[gk], cc);
         }
     }
 }
@@ -451,28 +759,27 @@ int dlu_leaf_width(int64_t m) {
     return m <= 32768 ? 1 : 0;
 }
 
-hipError_t dlu_leaf(hipStream_t s, const DV &F, int64_t m, int64_t c0, int w, int64_t *ipiv, Status *st) {
+hipError_t dlu_leaf(hipStream_t s, const DV &F, int64_t m, int64_t c0, int w, int64_t *ipiv, Status *st, int64_t a0, int na, int64_t b0,
+                    int nb) {
     // the instance is chosen by the rows left (mr <= m, so its width is at
     // least dlu_leaf_width(m))
     const int64_t mr = m - c0;
-    if (w < 1 || mr < 1 || w > dlu_leaf_width(mr)) return hipErrorInvalidValue;
+    if (w < 1 || mr < w || w > dlu_leaf_width(mr) || na < 0 || nb < 0 || na + nb + w > 64) return hipErrorInvalidValue;
     const dim3 g(1), b(LU_THREADS);
-    if (mr <= 1024) hipLaunchKernelGGL((k_dlu_leaf<1, 16>), g, b, 0, s, F, m, c0, w, ipiv, st);
-    else if (mr <= 2048) hipLaunchKernelGGL((k_dlu_leaf<2, 16>), g, b, 0, s, F, m, c0, w, ipiv, st);
-    else if (mr <= 4096) hipLaunchKernelGGL((k_dlu_leaf<4, 8>), g, b, 0, s, F, m, c0, w, ipiv, st);
-    else if (mr <= 8192) hipLaunchKernelGGL((k_dlu_leaf<8, 4>), g, b, 0, s, F, m, c0, w, ipiv, st);
-    else if (mr <= 16384) hipLaunchKernelGGL((k_dlu_leaf<16, 2>), g, b, 0, s, F, m, c0, w, ipiv, st);
-    else hipLaunchKernelGGL((k_dlu_leaf<32, 1>), g, b, 0, s, F, m, c0, w, ipiv, st);
+    if (mr <= 1024) hipLaunchKernelGGL((k_dlu_leaf<2, 16>), g, b, 0, s, F, m, c0, w, ipiv, st, a0, na, b0, nb);
+    else if (mr <= 2048) hipLaunchKernelGGL((k_dlu_leaf<4, 16>), g, b, 0, s, F, m, c0, w, ipiv, st, a0, na, b0, nb);
+    else if (mr <= 4096) hipLaunchKernelGGL((k_dlu_leaf<8, 8>), g, b, 0, s, F, m, c0, w, ipiv, st, a0, na, b0, nb);
+    else if (mr <= 8192) hipLaunchKernelGGL((k_dlu_leaf<16, 4>), g, b, 0, s, F, m, c0, w, ipiv, st, a0, na, b0, nb);
+    else if (mr <= 16384) hipLaunchKernelGGL((k_dlu_leaf<32, 2>), g, b, 0, s, F, m, c0, w, ipiv, st, a0, na, b0, nb);
+    else hipLaunchKernelGGL((k_dlu_leaf<64, 1>), g, b, 0, s, F, m, c0, w, ipiv, st, a0, na, b0, nb);
     return hipGetLastError();
 }
 
-hipError_t dlu_swap_solve(hipStream_t s, const DV &F, int64_t k0, int kn, const int64_t *ipiv, int64_t a0, int64_t na, int64_t b0,
-                          int64_t nb) {
-    if (na + nb == 0 || kn == 0) return hipSuccess;
-    if (kn < 0 || kn > 64) return hipErrorInvalidValue;
-    const dim3 g((unsigned)cdiv(na + nb, 256)), b(256);
-    if (kn <= 16) hipLaunchKernelGGL(k_dlu_swap_solve<16>, g, b, 0, s, F, k0, kn, ipiv, a0, na, b0, nb);
-    else hipLaunchKernelGGL(k_dlu_swap_solve<64>, g, b, 0, s, F, k0, kn, ipiv, a0, na, b0, nb);
+hipError_t dlu_rows(hipStream_t s, const DV &F, int64_t n, int64_t j0, int kn, const int64_t *ipiv) {
+    if (kn < 1 || kn > 64 || j0 + kn > n) return hipErrorInvalidValue;
+    const int64_t left = cdiv(j0, 64), rightc = cdiv(n - j0 - kn, 64);
+    if (left + rightc == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dlu_rows, dim3((unsigned)(left + rightc)), dim3(256), 0, s, F, n, j0, kn, ipiv, (int)left);
     return hipGetLastError();
 }
 

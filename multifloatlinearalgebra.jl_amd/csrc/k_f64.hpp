@@ -32,14 +32,16 @@ hipError_t dcopy(hipStream_t s, int64_t m, int64_t n, const DV &dst, const DV &s
 hipError_t drow_abs_sum(hipStream_t s, int64_t m, int64_t n, const DV &A, int uplo, double *out, double *part);
 int drow_abs_chunks();
 // Binary64 LU building blocks. dlu_leaf factors columns [c0, c0 + w) over
-// rows [c0, m) with partial pivoting (w <= dlu_leaf_width(m); 0 = m too
-// large); dlu_swap_solve applies ipiv[k0 .. k0 + kn) (kn <= 64) to the
-// columns [a0, a0 + na) and [b0, b0 + nb) and solves the unit lower triangle
-// F(k0 : k0 + kn, k0 : k0 + kn) on the rows [k0, k0 + kn) of range b.
+// rows [c0, m) with partial pivoting (w <= dlu_leaf_width(m - c0); 0 = too
+// many rows), applies the leaf's row interchanges to the columns [a0, a0 +
+// na) and [b0, b0 + nb) (na + nb + w <= 64) and solves the unit lower
+// triangle of the leaf on the top w rows of the second range. dlu_rows
+// applies a block's interchanges ipiv[j0 .. j0 + kn) (kn <= 64) to the
+// columns left and right of it and solves L11 on the right columns.
 int dlu_leaf_width(int64_t m);
-hipError_t dlu_leaf(hipStream_t s, const DV &F, int64_t m, int64_t c0, int w, int64_t *ipiv, Status *st);
-hipError_t dlu_swap_solve(hipStream_t s, const DV &F, int64_t k0, int kn, const int64_t *ipiv, int64_t a0, int64_t na, int64_t b0,
-                          int64_t nb);
+hipError_t dlu_leaf(hipStream_t s, const DV &F, int64_t m, int64_t c0, int w, int64_t *ipiv, Status *st, int64_t a0, int na, int64_t b0,
+                    int nb);
+hipError_t dlu_rows(hipStream_t s, const DV &F, int64_t n, int64_t j0, int kn, const int64_t *ipiv);
 hipError_t dgather_rows(hipStream_t s, int64_t m, int64_t ncols, const DV &dst, const DV &src, const int64_t *perm);
 
 }  // namespace mfk

@@ -97,7 +97,7 @@ def test_dgemm_triangular_operands_ignore_the_other_triangle(tri_a, tri_b):
     assert np.abs(_np(C)[0].T - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
 
 
-@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 200, 300])
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 129, 130, 200, 300, 1000])
 def test_dpotrf_inverse(n):
     from mfla import api as M
 
@@ -147,7 +147,7 @@ def _lu_check(A0, F, ipiv, pivots_like_lapack):
         assert np.array_equal(ipiv - 1, piv)
 
 
-@pytest.mark.parametrize("n", [1, 5, 8, 17, 64, 65, 130, 300, 1100, 2100])
+@pytest.mark.parametrize("n", [1, 5, 8, 17, 64, 65, 129, 130, 200, 300, 1100, 2100, 4100])
 def test_dgetrf_blocked_lu(n):
     from mfla import api as M
 
