@@ -1215,15 +1215,16 @@ struct RefineWs {
     double *F, *Inv, *InvU, *Y, *D, *R, *P, *rows, *part, *stats;
     int64_t *perm, *ipiv;
     int64_t ksplit;
-    // Ozaki residual: digit planes of A (S x n x ld int8), of the correction
-    // (T x nrhs x ld int8), exponents, int32 digit-pair sums
+    // Ozaki residual: digit planes of A (S planes of planeA bytes in the
+    // fragment layout), of the correction (T x nrhs x ld int8), exponents,
+    // int32 digit-pair sums
     int8_t *dA, *dD;
     int *eA, *eD;
     int32_t *planes;
-    int64_t ld;
+    int64_t ld, planeA;
     int S, T;
 };
-constexpr int kOzCorrectionDigits = 6;
+constexpr int kOzCorrectionDigits = 6;   // = mfk::oz_narrow_digits()
 
 // Split-k factor for the n x nrhs correction products: enough workgroups to
 // cover the chip, at least 256 reduction steps per slice.
@@ -1260,7 +1261,8 @@ int64_t refine_layout(int N, int kind, int64_t n, int64_t nrhs, double *base, Re
         w.ld = (n + mfk::oz_row_align() - 1) / mfk::oz_row_align() * mfk::oz_row_align();
         w.S = mfk::oz_digits(N);
         w.T = kOzCorrectionDigits;
-        w.dA = (int8_t *)take((w.S * n * w.ld + 7) / 8);
+        w.planeA = (n + 15) / 16 * 16 * w.ld;   // fragment layout: rows padded to 16
+        w.dA = (int8_t *)take((w.S * w.planeA + 7) / 8);
         w.dD = (int8_t *)take((w.T * nrhs * w.ld + 7) / 8);
         w.eA = (int *)take((n + 1) / 2);
         w.eD = (int *)take((nrhs + 1) / 2);
@@ -1460,7 +1462,7 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
     for (double v : hrows) anorm = std::max(anorm, v);
     if (ozaki) {   // A's digit planes, once per solve
         HIPCHK(mfk::oz_exponents(s, n, n, Ahi, kernel_uplo(uplo), w.eA, w.part));
-        HIPCHK(mfk::oz_split(N, s, n, n, Ahi, kernel_uplo(uplo), w.eA, w.S, w.dA, w.ld, n * w.ld));
+        HIPCHK(mfk::oz_split(N, s, n, n, Ahi, kernel_uplo(uplo), w.eA, w.S, w.dA, w.ld, w.planeA, true));
     }
     clk.mark(ST_FACTOR);
 
@@ -1498,7 +1500,7 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
             HIPCHK(mfk::add_mixed(1, N, s, n, nrhs, dvn(X), D));
             clk.mark(ST_UPDATE);
             const int U = w.S + w.T - 2;
-            HIPCHK(mfk::oz_gemm(s, n, nrhs, w.ld, w.dA, w.ld, n * w.ld, w.S, w.dD, w.ld, nrhs * w.ld, w.T, U, w.planes));
+            HIPCHK(mfk::oz_gemm_narrow(s, n, nrhs, w.ld, w.dA, w.planeA, w.S, w.dD, w.ld, nrhs * w.ld, w.planes));
             HIPCHK(mfk::oz_combine(N, s, n, nrhs, U, w.planes, w.eA, w.eD, R, nullptr, nullptr, 0, 1));
             clk.mark(ST_RESIDUAL);
         } else {

@@ -119,14 +119,25 @@ __device__ __forceinline__ void acc_digits(const Acc256 &a, int S, int8_t *d, in
     }
 }
 
-// dig[s * plane + i * ld + k] for X(i, k), s = 0..S-1; the padding columns
-// k in [K, ld) get zero digits.
-template <int N>
+// Byte offset of digit (i, k) inside one digit plane. Row layout: i * ld +
+// k. Fragment layout (FRAG): 16-row blocks, each a sequence of 64-deep k
+// steps of 1 KB laid out exactly as the lanes of a wave read one MFMA
+// 16x16x64 fragment (lane l = 16 * (k % 64 / 16) + i % 16 takes 16
+// consecutive k), so a wave's fragment load is one contiguous 1 KB block.
+template <bool FRAG>
+__device__ __forceinline__ int64_t oz_off(int64_t i, int64_t k, int64_t ld) {
+    if (!FRAG) return i * ld + k;
+    return ((i >> 4) * (ld >> 6) + (k >> 6)) * 1024 + ((((k & 63) >> 4) << 4) + (i & 15)) * 16 + (k & 15);
+}
+
+// dig[s * plane + oz_off(i, k)] for X(i, k), s = 0..S-1; the padding
+// columns k in [K, ld) get zero digits.
+template <int N, bool FRAG>
 __global__ void k_oz_split(int64_t m, int64_t K, DV X, int uplo, const int *e, int S, int8_t *dig, int64_t ld, int64_t plane) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= m * ld) return;
     const int64_t k = idx % ld, i = idx / ld;        // k fastest: coalesced digit stores
-    int8_t *d = dig + i * ld + k;
+    int8_t *d = dig + oz_off<FRAG>(i, k, ld);
     const int ei = e[i];
     Acc256 a{{0, 0, 0, 0}};
     if (ei != OZ_ZERO_ROW && k < K) {
@@ -185,6 +196,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 // the current one is multiplied from LDS. LDS rows are padded to 144 bytes
 // so the 16 rows a lane group reads fall on distinct banks.
 constexpr int OZ_KS = 128, OZ_LDW = 144;
+constexpr int OZ_NARROW_TB = 6;   // B digits of the narrow kernel (binary64 corrections)
 
 template <int WM, int WN>
 __global__ __launch_bounds__(256) void k_oz_gemm(int64_t m, int64_t n, int64_t K, const int8_t *dA, int64_t lda, int64_t planeA,
@@ -265,6 +277,67 @@ __global__ __launch_bounds__(256) void k_oz_gemm(int64_t m, int64_t n, int64_t K
             }
 }
 
+// Narrow products (the refinement's A * D: few columns, TB = 6 digits of a
+// binary64 correction): every wave owns one 16 x 16 output block and keeps
+// the partial sums of all SMAX + TB - 1 planes in registers, so each digit
+// plane of A is streamed once per 64-column group. A's digits are in the
+// fragment layout (one contiguous 1 KB load per wave and 64-deep k step);
+// the TB digit rows of the wave's 16 columns are few and stay in L2. Per k
+// step the wave loads its TB B fragments once and multiplies every A digit
+// s into the planes s .. s + TB - 1. A workgroup is 16 rows x 64 columns
+// (four waves side by side); grid.y runs over 64-column groups.
+template <int SA, int TB>
+__global__ __launch_bounds__(256) void k_oz_gemm_narrow(int64_t m, int64_t n, int64_t K, const int8_t *dA, int64_t planeA,
+                                                        const int8_t *dB, int64_t ldb, int64_t planeB, int32_t *out) {
+    constexpr int NP = SA + TB - 1;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x * 16, j0 = (int64_t)blockIdx.y * 64 + (int64_t)w * 16;
+    const int64_t ia = i0 + (lane & 15), jb = j0 + (lane & 15);
+    const bool rowok = ia < m, colok = jb < n;
+    // the fragment layout pads the rows to 16, so every A load is in bounds
+    const int8_t *pa = dA + (int64_t)blockIdx.x * (K >> 6) * 1024 + lane * 16;
+    const int8_t *pb = dB + (colok ? jb : 0) * ldb + (lane >> 4) * 16;
+    v4i acc[NP];
+#pragma unroll
+    for (int u = 0; u < NP; ++u) acc[u] = v4i{0, 0, 0, 0};
+    // all A fragments of a k step in registers; each slot is reloaded for
+    // the next step right after its MFMAs are issued
+    v4i fa[SA], fb[TB];
+#pragma unroll
+    for (int s = 0; s < SA; ++s) fa[s] = *(const v4i *)(pa + s * planeA);
+#pragma unroll
+    for (int q = 0; q < TB; ++q) fb[q] = *(const v4i *)(pb + q * planeB);
+    const v4i zero{0, 0, 0, 0};
+    for (int64_t k0 = 0; k0 < K; k0 += 64) {
+        const bool more = k0 + 64 < K;
+        const int8_t *pn = pa + ((k0 + 64) >> 6) * 1024;
+        v4i gb[TB];
+#pragma unroll
+        for (int q = 0; q < TB; ++q) gb[q] = colok ? fb[q] : zero;
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < TB; ++q) fb[q] = *(const v4i *)(pb + q * planeB + k0 + 64);
+        }
+#pragma unroll
+        for (int s = 0; s < SA; ++s) {
+            const v4i a = fa[s];
+#pragma unroll
+            for (int q = 0; q < TB; ++q) acc[s + q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gb[q], a, acc[s + q], 0, 0, 0);
+            if (more) fa[s] = *(const v4i *)(pn + s * planeA);
+        }
+    }
+    // item v of lane l: row (lane % 16), column 4 (lane / 16) + v of the block
+#pragma unroll
+    for (int u = 0; u < NP; ++u) {
+        int32_t *o = out + (int64_t)u * m * n;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t j = j0 + 4 * (lane >> 4) + v;
+            if (rowok && j < n) o[j * m + ia] = acc[u][v];
+        }
+    }
+}
+
 // -------------------------------------------------------------- combine
 // acc = sum_u C_u 2^(ea_i + eb_j - 7(u + 2)), u ascending, in N limbs; then
 // mode 0: C = alpha*acc (beta == 0) or alpha*acc + beta*C; mode 1: C = C - acc.
@@ -314,10 +387,34 @@ hipError_t oz_exponents(hipStream_t s, int64_t m, int64_t K, const DV &X, int up
 }
 
 hipError_t oz_split(int N, hipStream_t s, int64_t m, int64_t K, const DV &X, int uplo, const int *e, int S, int8_t *dig, int64_t ld,
-                    int64_t plane) {
+                    int64_t plane, bool frag) {
     if (m * ld == 0) return hipSuccess;
     if (S < 1 || 7 * S > 250 || ld % 64 != 0 || ld < K) return hipErrorInvalidValue;
-    OZ_DISPATCH(N, hipLaunchKernelGGL(k_oz_split<NN>, dim3((unsigned)cdiv(m * ld, 256)), dim3(256), 0, s, m, K, X, uplo, e, S, dig, ld, plane));
+    if (frag && plane < cdiv(m, 16) * 16 * ld) return hipErrorInvalidValue;
+    if (frag) {
+        OZ_DISPATCH(N, hipLaunchKernelGGL((k_oz_split<NN, true>), dim3((unsigned)cdiv(m * ld, 256)), dim3(256), 0, s, m, K, X, uplo, e, S, dig,
+                                          ld, plane));
+    } else {
+        OZ_DISPATCH(N, hipLaunchKernelGGL((k_oz_split<NN, false>), dim3((unsigned)cdiv(m * ld, 256)), dim3(256), 0, s, m, K, X, uplo, e, S,
+                                          dig, ld, plane));
+    }
+    return hipGetLastError();
+}
+
+int oz_narrow_digits() { return OZ_NARROW_TB; }
+
+hipError_t oz_gemm_narrow(hipStream_t s, int64_t m, int64_t n, int64_t K, const int8_t *dA, int64_t planeA, int SA, const int8_t *dB,
+                          int64_t ldb, int64_t planeB, int32_t *out) {
+    if (m == 0 || n == 0) return hipSuccess;
+    if (K > 8192 || K % OZ_KS != 0 || ldb % 16 != 0 || planeA < cdiv(m, 16) * 16 * K) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)cdiv(m, 16), (unsigned)cdiv(n, 64)), block(256);
+    switch (SA) {   // the digit counts of oz_digits(N)
+        case 34: hipLaunchKernelGGL((k_oz_gemm_narrow<34, OZ_NARROW_TB>), grid, block, 0, s, m, n, K, dA, planeA, dB, ldb, planeB, out); break;
+        case 27: hipLaunchKernelGGL((k_oz_gemm_narrow<27, OZ_NARROW_TB>), grid, block, 0, s, m, n, K, dA, planeA, dB, ldb, planeB, out); break;
+        case 19: hipLaunchKernelGGL((k_oz_gemm_narrow<19, OZ_NARROW_TB>), grid, block, 0, s, m, n, K, dA, planeA, dB, ldb, planeB, out); break;
+        case 11: hipLaunchKernelGGL((k_oz_gemm_narrow<11, OZ_NARROW_TB>), grid, block, 0, s, m, n, K, dA, planeA, dB, ldb, planeB, out); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

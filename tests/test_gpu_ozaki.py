@@ -114,3 +114,23 @@ def test_refinement_with_ozaki_residuals(oracle, kind, uplo, n, N):
         Ah[:, il[1], il[0]] = Ah[:, il[0], il[1]]
     be = exact_backward_error(oracle, Ah, X.cpu().numpy(), Bh)
     assert be.max() <= 2.0 ** (-52 * N + 8), (be.max(), hist)
+
+
+@pytest.mark.parametrize("kind,n,nrhs", [("cholesky", 1000, 64), ("lu", 777, 33)])
+def test_refinement_with_ozaki_residuals_many_rhs(oracle, kind, n, nrhs):
+    """Ragged row count (not a multiple of the 16-row output blocks) and up to
+    64 right-hand sides through the narrow digit product."""
+    from mfla import refine as R
+    from test_gpu_refine import exact_backward_error
+
+    N = 4
+    A = R.spd_kappa(n, 1e8, N=N) if kind == "cholesky" else R.general_kappa(n, 1e8, N=N)
+    rng = np.random.default_rng(29)
+    Bh = np.zeros((N, nrhs, n))
+    Bh[0] = rng.standard_normal((nrhs, n))
+    for l in range(1, N):
+        Bh[l] = Bh[l - 1] * 2.0**-53 * rng.uniform(-1, 1, (nrhs, n))
+    X, hist = R.refine_solve(A, cu(Bh), kind, residual="ozaki")
+    assert 2 <= len(hist) <= 12, hist
+    be = exact_backward_error(oracle, A.cpu().numpy(), X.cpu().numpy(), Bh)
+    assert be.max() <= 2.0 ** (-52 * N + 8), (be.max(), hist)
