@@ -332,6 +332,14 @@ __global__ void k_dgather_rows(int64_t m, int64_t ncols, DV dst, DV src, const i
 // barrier and all writes after it.
 constexpr int LU_THREADS = 512, LU_WAVES = LU_THREADS / 64;
 
+// Workgroup barrier that orders LDS only: global loads issued before it stay
+// in flight (a full __syncthreads() waits for them).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int CTRL>
 __device__ __forceinline__ double lu_dpp_d(double x) {
     const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
@@ -383,13 +391,19 @@ __global__ __launch_bounds__(LU_THREADS) void k_dlu_leaf(DV F, int64_t m, int64_
     const int mr = (int)(m - c0);   // <= 32768
     double v[RPT][W];
     int rid[RPT];
+    bool rowok[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
         const int r = t + LU_THREADS * i;
         rid[i] = r;
-#pragma unroll
-        for (int j = 0; j < W; ++j) v[i][j] = (r < mr && j < w) ? F.p[(c0 + r) * F.rs + (c0 + j) * F.cs] : 0.0;
+        rowok[i] = r < mr;
     }
+    // column by column, so the first column's loads complete first and its
+    // step overlaps the arrival of the later columns
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) v[i][j] = (rowok[i] && j < w) ? F.p[(c0 + t + LU_THREADS * i) * F.rs + (c0 + j) * F.cs] : 0.0;
     if (t == 0) nmoved = 0;
     int piv[W];   // local pivot row of each column (uniform); stored after the loop so that no
                   // global store is outstanding at the column barriers
@@ -403,7 +417,8 @@ __global__ __launch_bounds__(LU_THREADS) void k_dlu_leaf(DV F, int64_t m, int64_
             for (int i = 0; i < RPT; ++i) {            // rows ascending: first maximum kept
                 const int r = t + LU_THREADS * i;
                 const double a = fabs(v[i][j]);
-                const bool take = r < mr && r >= j && a > best;
+                // rows of slot i >= 1 are >= 512 > j: only slot 0 compares with j
+                const bool take = (i == 0 ? (rowok[0] && t >= j) : rowok[i]) && a > best;
                 best = take ? a : best;
                 bi = take ? r : bi;
             }
@@ -421,7 +436,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_dlu_leaf(DV F, int64_t m, int64_
                 for (int jj = 0; jj < W; ++jj) jrow[par][jj] = v[0][jj];
                 jrid[par] = rid[0];
             }
-            __syncthreads();
+            lds_barrier();   // the slots are LDS; the panel's later columns may still be arriving
             // the wave winners: one LDS read per lane of row 0, a row reduction
             {
                 const double cb = lane < LU_WAVES ? cval[par][lane] : -1.0;
@@ -433,34 +448,40 @@ __global__ __launch_bounds__(LU_THREADS) void k_dlu_leaf(DV F, int64_t m, int64_
             if (best > 0.0) {                                // uniform
                 const double *prow = cand[par][ww];
                 if (bi != j) {
-                    if (t == j) {
+                    if (t == j) {                            // wave 0 (j < 16)
 #pragma unroll
                         for (int jj = 0; jj < W; ++jj) v[0][jj] = prow[jj];
                         rid[0] = crid[par][ww];
                     }
+                    if (wave == ww) {                        // uniform: only the pivot row's wave
 #pragma unroll
-                    for (int i = 0; i < RPT; ++i)
-                        if (t + LU_THREADS * i == bi) {
+                        for (int i = 0; i < RPT; ++i)
+                            if (t + LU_THREADS * i == bi) {
 #pragma unroll
-                            for (int jj = 0; jj < W; ++jj) v[i][jj] = jrow[par][jj];
-                            rid[i] = jrid[par];
-                        }
+                                for (int jj = 0; jj < W; ++jj) v[i][jj] = jrow[par][jj];
+                                rid[i] = jrid[par];
+                            }
+                    }
                 }
                 piv[j] = bi;
                 const double inv = 1.0 / prow[j];
+                // multipliers and rank-1 update of the rows below the pivot:
+                // slot 0 holds rows t <= j (the pivot row and rows above it),
+                // which keep their values; slots >= 1 are all below it (rows
+                // past the panel are padding: never compared or stored)
+                double l[RPT];
+                const bool below0 = t > j;
+#pragma unroll
+                for (int i = 0; i < RPT; ++i) l[i] = v[i][j] * inv;
+                v[0][j] = below0 ? l[0] : v[0][j];
+#pragma unroll
+                for (int i = 1; i < RPT; ++i) v[i][j] = l[i];
 #pragma unroll
                 for (int jj = j + 1; jj < W; ++jj) {
                     const double u = prow[jj];
+                    v[0][jj] = below0 ? fma(-l[0], u, v[0][jj]) : v[0][jj];
 #pragma unroll
-                    for (int i = 0; i < RPT; ++i) {
-                        const int r = t + LU_THREADS * i;
-                        if (r < mr && r > j) v[i][jj] = fma(-(v[i][j] * inv), u, v[i][jj]);
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < RPT; ++i) {
-                    const int r = t + LU_THREADS * i;
-                    if (r < mr && r > j) v[i][j] *= inv;
+                    for (int i = 1; i < RPT; ++i) v[i][jj] = fma(-l[i], u, v[i][jj]);
                 }
             } else {                                         // zero column: no interchange
                 piv[j] = j;
