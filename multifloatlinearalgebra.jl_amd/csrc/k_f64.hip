@@ -4,15 +4,14 @@
 //
 //  * k_dgemm: C = alpha * op(A) * op(B) + beta * C on strided views, 64x64
 //    tile per 256-thread workgroup, each wave a 32x32 block of 2x2 MFMA
-//    16x16x4 f64 accumulators. No LDS and no barriers in the k loop: every
-//    lane loads its own operand values straight from global memory (L1/L2
-//    serve the reuse between the waves of a tile) one 16-deep k chunk
-//    ahead of the MFMAs that consume them, and C is prefetched before the
-//    loop. Optional triangular structure of A / B (the k range of each tile
-//    is clipped and the zero triangle is never read), a lower-triangular
-//    output mask, a batch index (grid z) with element strides, and a split-k
-//    mode that writes partial sums for k_dgemm_reduce (fixed summation
-//    order, so the result is deterministic).
+//    16x16x4 f64 accumulators. Operands pass through double-buffered LDS
+//    stages 32 deep; the next stage is fetched into registers while the
+//    current one is multiplied.
+//    Optional triangular structure of A / B (the k range of each tile is
+//    clipped and the zero triangle's values are never used), a
+//    lower-triangular output mask, a batch index (grid z) with element
+//    strides, and a split-k mode that writes partial sums for k_dgemm_reduce
+//    (fixed summation order, so the result is deterministic).
 //  * k_dpotrf_diag: Cholesky of one nb x nb (nb <= 64) diagonal block in LDS,
 //    plus the inverse of the resulting triangle (needed by the panel update
 //    L21 = A21 * inv(L11)^T and by the correction solve).
@@ -25,7 +24,7 @@ namespace mfk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int DG_T = 64, DG_BK = 16;
+constexpr int DG_T = 64;
 
 // v_mfma_f64_16x16x4_f64 operand layout (lane l of the wave), as measured
 // on gfx950 with integer-valued operands:
@@ -37,39 +36,68 @@ constexpr int DG_T = 64, DG_BK = 16;
 // consecutive elements of a column-major C column. Inside a 16-deep chunk
 // at k0, MFMA step s gives slot q = l / 16 the index k = k0 + 4 q + s, so
 // each lane reads four consecutive k of its row of A / column of B.
-struct DgOps {
-    double a[2][4], b[2][4];
+constexpr int DG_KC = 32, DG_LDK = DG_T + 1;   // k depth of one LDS stage; padded LDS rows
+
+// Operand staging: thread t brings 8 elements of A (64 rows x 32 k) and 8
+// of B (32 k x 64 columns) per stage. The lane index runs along whichever
+// operand dimension has unit stride, so the loads coalesce for both
+// column-major views and transposed ones. Addresses are clamped into the
+// matrix and masked elements (outside the range, or in a zero triangle that
+// is never read) are replaced by zeros after the load, so no load is
+// conditional.
+struct DgStage {
+    double a[8], b[8];
 };
 
-// Operands of one chunk: pa[h] / pb[h] address A(ib + 16 h, k) / B(k, jb + 16 h)
-// at this lane's first k; indices are 32-bit (m, n, k < 2^31).
-__device__ __forceinline__ void dg_load(const DGemmArgs &g, const double *const (&pa)[2], const double *const (&pb)[2], int ib, int jb,
-                                        int k, int khi, DgOps &o) {
+template <typename IT>
+__device__ __forceinline__ void dg_fetch(const DGemmArgs &g, const double *Ap, const double *Bp, int64_t i0, int64_t j0, int64_t k0,
+                                         int64_t khi, int t, DgStage &st) {
+    const bool a_rows = g.A.rs == 1, b_cols = g.B.cs == 1;
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int q = 0; q < 8; ++q) {
+        const int e = t + 256 * q;                                          // 0 .. 2047 = 64 x 32
+        const int ii = a_rows ? (e & 63) : (e >> 5), kk = a_rows ? (e >> 6) : (e & 31);
+        const int64_t i = i0 + ii, k = k0 + kk;
+        bool ok = i < g.m && k < khi;
+        if (g.a_tri == TRI_LOWER) ok = ok && k <= i;
+        if (g.a_tri == TRI_UPPER) ok = ok && k >= i;
+        const IT ic = (IT)(i < g.m ? i : g.m - 1), kc = (IT)(k < g.k ? k : g.k - 1);
+        const double v = Ap[ic * (IT)g.A.rs + kc * (IT)g.A.cs];
+        st.a[q] = ok ? v : 0.0;
+    }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int kk = k + s, i = ib + 16 * h, j = jb + 16 * h;
-            bool oa = i < (int)g.m && kk < khi, ob = j < (int)g.n && kk < khi;
-            if (g.a_tri == TRI_LOWER) oa = oa && kk <= i;
-            if (g.a_tri == TRI_UPPER) oa = oa && kk >= i;
-            if (g.b_tri == TRI_LOWER) ob = ob && kk >= j;
-            if (g.b_tri == TRI_UPPER) ob = ob && kk <= j;
-            o.a[h][s] = oa ? pa[h][s * g.A.cs] : 0.0;
-            o.b[h][s] = ob ? pb[h][s * g.B.rs] : 0.0;
-        }
+    for (int q = 0; q < 8; ++q) {
+        const int e = t + 256 * q;
+        const int jj = b_cols ? (e & 63) : (e >> 5), kk = b_cols ? (e >> 6) : (e & 31);
+        const int64_t j = j0 + jj, k = k0 + kk;
+        bool ok = j < g.n && k < khi;
+        if (g.b_tri == TRI_LOWER) ok = ok && k >= j;
+        if (g.b_tri == TRI_UPPER) ok = ok && k <= j;
+        const IT jc = (IT)(j < g.n ? j : g.n - 1), kc = (IT)(k < g.k ? k : g.k - 1);
+        const double v = Bp[kc * (IT)g.B.rs + jc * (IT)g.B.cs];
+        st.b[q] = ok ? v : 0.0;
+    }
 }
 
-__device__ __forceinline__ void dg_mma(const DgOps &o, d4 (&acc)[2][2]) {
+__device__ __forceinline__ void dg_store(const DGemmArgs &g, int t, const DgStage &st, double (*As)[DG_LDK], double (*Bs)[DG_LDK]) {
+    const bool a_rows = g.A.rs == 1, b_cols = g.B.cs == 1;
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.b[bj][s], o.a[bi][s], acc[bi][bj], 0, 0, 0);
+    for (int q = 0; q < 8; ++q) {
+        const int e = t + 256 * q;
+        const int ii = a_rows ? (e & 63) : (e >> 5), kk = a_rows ? (e >> 6) : (e & 31);
+        As[kk][ii] = st.a[q];
+        const int jj = b_cols ? (e & 63) : (e >> 5), k2 = b_cols ? (e >> 6) : (e & 31);
+        Bs[k2][jj] = st.b[q];
+    }
 }
 
+// Inside a stage, MFMA step s of wave (wy, wx) takes k = 4 s + l / 16 for
+// lane l from LDS. IT: the integer type of element offsets inside A and B
+// (int when the host has checked that every offset fits).
+template <typename IT>
 __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
+    __shared__ double As[2][DG_KC][DG_LDK];
+    __shared__ double Bs[2][DG_KC][DG_LDK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wy = w & 1, wx = w >> 1;
     const int64_t tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
@@ -89,29 +117,14 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
     const int ks = blockIdx.y;
     if (g.ksplit > 1) {
         const int64_t span = khi > klo ? khi - klo : 0;
-        const int64_t chunk = ((span + g.ksplit - 1) / g.ksplit + DG_BK - 1) / DG_BK * DG_BK;
+        const int64_t chunk = ((span + g.ksplit - 1) / g.ksplit + DG_KC - 1) / DG_KC * DG_KC;
         const int64_t a = klo + ks * chunk, b = a + chunk;
         klo = a;
         khi = b < khi ? b : khi;
     }
 
-    const int64_t ib = i0 + wy * 32 + (lane & 15), jb = j0 + wx * 32 + (lane & 15);
+    const int64_t ib = i0 + wy * 32 + (lane & 15);
     const int64_t jc = j0 + wx * 32 + (lane >> 4);   // C column of item v of block bj: jc + 16 bj + 4 v
-    const int kq = 4 * (lane >> 4);
-
-    // C values for the epilogue, fetched before the k loop
-    double cv[2][2][4];
-    const bool readc = !g.beta_zero && g.ksplit == 1;
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t i = ib + 16 * bi, j = jc + 16 * bj + 4 * v;
-                const bool ok = readc && i < g.m && j < g.n && !(g.c_lower && i < j);
-                cv[bi][bj][v] = ok ? Cp[i * g.C.rs + j * g.C.cs] : 0.0;
-            }
 
     d4 acc[2][2];
 #pragma unroll
@@ -119,39 +132,50 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
 
-    // two operand sets: the next chunk is in flight while the current one
-    // feeds the MFMAs
-    DgOps o0, o1;
-    // per-lane operand pointers at the chunk being loaded; a chunk advances
-    // them by 16 k
-    const double *pa[2], *pb[2];
-    {
-        const int64_t k = klo + kq;
-        pa[0] = Ap + ib * g.A.rs + k * g.A.cs;
-        pa[1] = pa[0] + 16 * g.A.rs;
-        pb[0] = Bp + jb * g.B.cs + k * g.B.rs;
-        pb[1] = pb[0] + 16 * g.B.cs;
+    const int64_t nst = khi > klo ? (khi - klo + DG_KC - 1) / DG_KC : 0;
+    DgStage st;
+    if (nst > 0) {
+        dg_fetch<IT>(g, Ap, Bp, i0, j0, klo, khi, t, st);
+        dg_store(g, t, st, As[0], Bs[0]);
     }
-    const int64_t sa = DG_BK * g.A.cs, sb = DG_BK * g.B.rs;
-    const int ib32 = (int)ib, jb32 = (int)jb, khi32 = (int)khi;
-    int kn = (int)klo + kq;
-    auto advance = [&]() {
-        pa[0] += sa; pa[1] += sa; pb[0] += sb; pb[1] += sb;
-        kn += DG_BK;
-    };
-    if (klo < khi) { dg_load(g, pa, pb, ib32, jb32, kn, khi32, o0); advance(); }
-    for (int64_t k0 = klo; k0 < khi; k0 += 2 * DG_BK) {
-        const bool more = k0 + DG_BK < khi;
-        if (more) { dg_load(g, pa, pb, ib32, jb32, kn, khi32, o1); advance(); }
-        dg_mma(o0, acc);
-        if (!more) break;
-        if (k0 + 2 * DG_BK < khi) { dg_load(g, pa, pb, ib32, jb32, kn, khi32, o0); advance(); }
-        dg_mma(o1, acc);
-    }
-    // C may alias A (the in-place panel product L21 = A21 inv(L11)^T): every
-    // wave has read its operands before any wave of the tile writes.
     __syncthreads();
-
+    for (int64_t s = 0; s < nst; ++s) {
+        const int cur = (int)(s & 1);
+        if (s + 1 < nst) dg_fetch<IT>(g, Ap, Bp, i0, j0, klo + (s + 1) * DG_KC, khi, t, st);
+#pragma unroll
+        for (int k4 = 0; k4 < DG_KC / 4; ++k4) {
+            const int kr = k4 * 4 + (lane >> 4);
+            double a[2], b[2];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi) a[bi] = As[cur][kr][wy * 32 + bi * 16 + (lane & 15)];
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) b[bj] = Bs[cur][kr][wx * 32 + bj * 16 + (lane & 15)];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[bj], a[bi], acc[bi][bj], 0, 0, 0);
+        }
+        if (s + 1 < nst) dg_store(g, t, st, As[cur ^ 1], Bs[cur ^ 1]);
+        __syncthreads();
+    }
+    // (C may alias A -- the in-place panel product L21 = A21 inv(L11)^T: all
+    // operands of the tile were read before the last barrier above.)
+    // Epilogue: the 16 C values are loaded together (clamped addresses, so
+    // no load waits on a branch), then the masked stores.
+    double cv[2][2][4];
+    const bool readc = !g.beta_zero && g.ksplit == 1;
+    if (readc) {
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t i = ib + 16 * bi, j = jc + 16 * bj + 4 * v;
+                    const int64_t icl = i < g.m ? i : g.m - 1, jcl = j < g.n ? j : g.n - 1;
+                    cv[bi][bj][v] = Cp[icl * g.C.rs + jcl * g.C.cs];
+                }
+    }
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -159,14 +183,13 @@ __global__ __launch_bounds__(256) void k_dgemm(DGemmArgs g, int tiles_m) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int64_t i = ib + 16 * bi, j = jc + 16 * bj + 4 * v;
-                if (i >= g.m || j >= g.n) continue;
-                if (g.c_lower && i < j) continue;
+                const bool ok = i < g.m && j < g.n && !(g.c_lower && i < j);
                 const double r = acc[bi][bj][v];
                 if (g.ksplit > 1) {
-                    g.P[((int64_t)ks * g.n + j) * g.m + i] = r;
+                    if (ok) g.P[((int64_t)ks * g.n + j) * g.m + i] = r;
                 } else {
-                    double *c = Cp + i * g.C.rs + j * g.C.cs;
-                    *c = g.beta_zero ? g.alpha * r : g.alpha * r + g.beta * cv[bi][bj][v];
+                    const double o = g.beta_zero ? g.alpha * r : g.alpha * r + g.beta * cv[bi][bj][v];
+                    if (ok) Cp[i * g.C.rs + j * g.C.cs] = o;
                 }
             }
 }
@@ -186,78 +209,161 @@ __global__ void k_dgemm_reduce(DGemmArgs g) {
 // ------------------------------------------------ diagonal block Cholesky
 // Factor the nb x nb block at (j0, j0) of F (lower triangle read), write L
 // back to F's lower triangle and inv(L) to Inv's lower triangle at (j0, j0).
-// The block is padded to 64 x 64 with an identity. Factorization: right-
-// looking with deferred scaling (a_rq -= a_rc a_qc / d_c, one barrier per
-// column), then L = column c / sqrt(d_c). Inverse: recursive doubling over
-// block sizes 1, 2, 4, .., 32, each level two small triangular products
-// (W = L21 X11 kept in the strict upper triangle, X21 = -X22 W). A
-// non-positive or non-finite pivot at block column c records info =
-// j0 + c + 1 and stops the factorization.
-constexpr int DP_LD = 65, DP_THREADS = 1024;
+// The block is padded to 64 x 64 with an identity and held in LDS by one
+// 256-thread workgroup. Factorization in four 16-column panels, three
+// barriers each: wave 0 factors the 16 x 16 diagonal block in registers
+// (lane i holds row i; the pivot and multipliers reach the other lanes by
+// lane reads), one thread per row solves the rows below against it, and the
+// workgroup applies the rank-16 update to the trailing lower triangle.
+// Inverse: wave 0 inverts the four diagonal 16 x 16 blocks at once (lane
+// 16 p + j computes column j of block p), then the off-diagonal blocks by
+// block distance d = 1, 2, 3: W = sum_k L(a, k) X(k, b) (kept in the unused
+// upper triangle), X(a, b) = -X(a, a) W. A non-positive or non-finite pivot
+// at block column c records info = j0 + c + 1 and stops the factorization.
+constexpr int DP_LD = 65, DP_THREADS = 256;
+
+__device__ __forceinline__ double dp_bcast(double x, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l), __builtin_amdgcn_readlane(__double2loint(x), l));
+}
 
 __global__ __launch_bounds__(DP_THREADS) void k_dpotrf_diag(DV F, DV Inv, int64_t j0, int nb, Status *st) {
     if (st->info != 0) return;
-    __shared__ double L[64 * DP_LD];   // column-major: L[q * DP_LD + r]
-    __shared__ double X[64 * DP_LD];   // row-major:    X[r * DP_LD + c]
-    __shared__ double sd[64];
-    const int t = threadIdx.x;
-    for (int e = t; e < 64 * 64; e += DP_THREADS) {
-        const int r = e & 63, q = e >> 6;
-        double v = (r == q) ? 1.0 : 0.0;
-        if (r < nb && q < nb && r >= q) v = F.p[(j0 + r) * F.rs + (j0 + q) * F.cs];
-        L[q * DP_LD + r] = v;
-    }
-    __syncthreads();
-    for (int c = 0; c < nb; ++c) {
-        const double d = L[c * DP_LD + c];
-        if (!(d > 0.0) || !__builtin_isfinite(d)) {      // uniform: every thread reads the same pivot
-            if (t == 0) st->info = j0 + c + 1;
-            return;
+    __shared__ double A[64 * DP_LD];   // row-major A[r * DP_LD + q]: the block, L after the factorization
+    __shared__ double X[64 * DP_LD];   // row-major inverse (the upper triangle holds temporaries)
+    __shared__ double idg[64];         // 1 / L(c, c)
+    __shared__ int fail;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    {
+        double g[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {                       // clamped addresses: every load is in bounds
+            const int e = t + DP_THREADS * k, r = e & 63, q = e >> 6;
+            const int rr = r < nb ? r : nb - 1, qq = q < nb ? q : nb - 1;
+            g[k] = F.p[(j0 + rr) * F.rs + (j0 + qq) * F.cs];
         }
-        const double inv_d = 1.0 / d;
-        const int w = nb - c - 1;
-        for (int e = t; e < w * w; e += DP_THREADS) {
-            const int r = c + 1 + e % w, q = c + 1 + e / w;
-            if (r >= q) L[q * DP_LD + r] -= L[c * DP_LD + r] * L[c * DP_LD + q] * inv_d;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = t + DP_THREADS * k, r = e & 63, q = e >> 6;
+            const bool in = r < nb && q < nb && r >= q;
+            A[r * DP_LD + q] = in ? g[k] : (r == q ? 1.0 : 0.0);
+        }
+    }
+    if (t == 0) fail = 0;
+    __syncthreads();
+
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int c0 = 16 * p;
+        if (w == 0) {
+            // the 16 x 16 diagonal block: lane i (< 16) holds row c0 + i
+            const int i = lane & 15;
+            double b[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) b[q] = A[(c0 + i) * DP_LD + c0 + q];
+            int bad = -1;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (bad < 0) {                                // uniform
+                    const double d = dp_bcast(b[k], k);
+                    if (!(d > 0.0) || !__builtin_isfinite(d)) {
+                        bad = k;
+                    } else {
+                        const double sd = __builtin_sqrt(d), is = 1.0 / sd;
+                        b[k] = i == k ? sd : (i > k ? b[k] * is : b[k]);
+#pragma unroll
+                        for (int q = k + 1; q < 16; ++q) {
+                            const double lq = dp_bcast(b[k], q);
+                            b[q] = i >= q ? fma(-b[k], lq, b[q]) : b[q];
+                        }
+                    }
+                }
+            }
+            if (lane < 16) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    if (q <= i) A[(c0 + i) * DP_LD + c0 + q] = b[q];
+                    if (q == i) idg[c0 + i] = 1.0 / b[q];
+                }
+            }
+            if (bad >= 0 && lane == 0) {
+                fail = 1;
+                st->info = j0 + c0 + bad + 1;                 // padding pivots are 1: c0 + bad < nb
+            }
+        }
+        __syncthreads();
+        if (fail) return;                                     // uniform
+        const int R = 48 - c0;                                // rows below the panel
+        if (t < R) {
+            const int r = c0 + 16 + t;
+            double x[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) x[k] = A[r * DP_LD + c0 + k];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                double a = x[k];
+#pragma unroll
+                for (int q = 0; q < k; ++q) a = fma(-x[q], A[(c0 + k) * DP_LD + c0 + q], a);
+                x[k] = a * idg[c0 + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) A[r * DP_LD + c0 + k] = x[k];
+        }
+        __syncthreads();
+        for (int e = t; e < R * R; e += DP_THREADS) {
+            const int i = c0 + 16 + e % R, j = c0 + 16 + e / R;
+            if (i >= j) {
+                double acc = A[i * DP_LD + j];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc = fma(-A[i * DP_LD + c0 + k], A[j * DP_LD + c0 + k], acc);
+                A[i * DP_LD + j] = acc;
+            }
         }
         __syncthreads();
     }
-    if (t < 64) sd[t] = __builtin_sqrt(L[t * DP_LD + t]);
-    __syncthreads();
-    for (int e = t; e < 64 * 64; e += DP_THREADS) {
-        const int r = e & 63, q = e >> 6;
-        if (r > q) L[q * DP_LD + r] /= sd[q];
+
+    // ---- inverse: the diagonal 16 x 16 blocks (wave 0, lane 16 p + j)
+    if (w == 0) {
+        const int c0 = 16 * (lane >> 4), j = lane & 15;
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            double a = i == j ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < i; ++k) a = fma(-A[(c0 + i) * DP_LD + c0 + k], x[k], a);
+            x[i] = i >= j ? a * idg[c0 + i] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) if (i >= j) X[(c0 + i) * DP_LD + c0 + j] = x[i];
     }
     __syncthreads();
-    if (t < 64) {
-        L[t * DP_LD + t] = sd[t];
-        X[t * DP_LD + t] = 1.0 / sd[t];
-    }
-    __syncthreads();
-    for (int e = t; e < 64 * 64; e += DP_THREADS) {
-        const int r = e & 63, q = e >> 6;
-        if (r < nb && q < nb && r >= q) F.p[(j0 + r) * F.rs + (j0 + q) * F.cs] = L[q * DP_LD + r];
-    }
-    for (int s = 1; s < 64; s *= 2) {
-        const int pairs = 64 / (2 * s);
-        for (int e = t; e < pairs * s * s; e += DP_THREADS) {       // W = L21 X11 (X11 lower: k >= j)
-            const int p = e / (s * s), i = (e / s) % s, j = e % s, b = p * 2 * s;
+    // off-diagonal blocks (a, b), a = b + d
+#pragma unroll
+    for (int d = 1; d < 4; ++d) {
+        const int nblk = 4 - d;
+        for (int e = t; e < 256 * nblk; e += DP_THREADS) {   // W(i, jj) = sum_k L(16a + i, k) X(k, 16b + jj), k in [16b, 16a)
+            const int bb = e >> 8, i = (e >> 4) & 15, jj = e & 15, aa = bb + d;
             double acc = 0.0;
-            for (int k = j; k < s; ++k) acc += L[(b + k) * DP_LD + b + s + i] * X[(b + k) * DP_LD + b + j];
-            X[(b + j) * DP_LD + b + s + i] = acc;
+            for (int k = 16 * bb + jj; k < 16 * aa; ++k) acc = fma(A[(16 * aa + i) * DP_LD + k], X[k * DP_LD + 16 * bb + jj], acc);
+            X[(16 * bb + jj) * DP_LD + 16 * aa + i] = acc;   // upper-triangle temporary
         }
         __syncthreads();
-        for (int e = t; e < pairs * s * s; e += DP_THREADS) {       // X21 = -X22 W (X22 lower: k <= i)
-            const int p = e / (s * s), i = (e / s) % s, j = e % s, b = p * 2 * s;
+        for (int e = t; e < 256 * nblk; e += DP_THREADS) {   // X(16a + i, 16b + jj) = -sum_m X(16a + i, 16a + m) W(m, jj)
+            const int bb = e >> 8, i = (e >> 4) & 15, jj = e & 15, aa = bb + d;
             double acc = 0.0;
-            for (int k = 0; k <= i; ++k) acc += X[(b + s + i) * DP_LD + b + s + k] * X[(b + j) * DP_LD + b + s + k];
-            X[(b + s + i) * DP_LD + b + j] = -acc;
+#pragma unroll
+            for (int m = 0; m < 16; ++m)
+                if (m <= i) acc = fma(X[(16 * aa + i) * DP_LD + 16 * aa + m], X[(16 * bb + jj) * DP_LD + 16 * aa + m], acc);
+            X[(16 * aa + i) * DP_LD + 16 * bb + jj] = -acc;
         }
         __syncthreads();
     }
-    for (int e = t; e < 64 * 64; e += DP_THREADS) {
-        const int r = e & 63, q = e >> 6;
-        if (r < nb && q < nb && r >= q) Inv.p[(j0 + r) * Inv.rs + (j0 + q) * Inv.cs] = X[r * DP_LD + q];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = t + DP_THREADS * k, r = e & 63, q = e >> 6;
+        if (r < nb && q < nb && r >= q) {
+            F.p[(j0 + r) * F.rs + (j0 + q) * F.cs] = A[r * DP_LD + q];
+            Inv.p[(j0 + r) * Inv.rs + (j0 + q) * Inv.cs] = X[r * DP_LD + q];
+        }
     }
 }
 
@@ -742,7 +848,11 @@ hipError_t dgemm(hipStream_t s, DGemmArgs g, int64_t batch) {
     if (g.ksplit > 1 && (g.P == nullptr || batch != 1)) return hipErrorInvalidValue;
     const int64_t tiles_m = cdiv(g.m, DG_T), tiles_n = cdiv(g.n, DG_T);
     const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)g.ksplit, (unsigned)batch);
-    hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, g, (int)tiles_m);
+    // element offsets inside A and B fit in 32 bits?
+    auto span = [](int64_t r, int64_t c, const DV &v) { return (r > 0 ? r - 1 : 0) * (v.rs < 0 ? -v.rs : v.rs) + (c > 0 ? c - 1 : 0) * (v.cs < 0 ? -v.cs : v.cs); };
+    const bool off32 = span(g.m, g.k, g.A) < INT32_MAX && span(g.k, g.n, g.B) < INT32_MAX;
+    if (off32) hipLaunchKernelGGL(k_dgemm<int>, grid, dim3(256), 0, s, g, (int)tiles_m);
+    else hipLaunchKernelGGL(k_dgemm<int64_t>, grid, dim3(256), 0, s, g, (int)tiles_m);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || g.ksplit == 1) return e;
     hipLaunchKernelGGL(k_dgemm_reduce, dim3((unsigned)cdiv(g.m * g.n, 256)), dim3(256), 0, s, g);
