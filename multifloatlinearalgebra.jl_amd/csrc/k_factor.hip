@@ -140,23 +140,113 @@ __global__ __launch_bounds__(256) void k_chol_panel(DV A, int64_t first, int64_t
 // swap the rows within the panel columns [c0, c1) (MFLA swaps whole rows;
 // the other columns get the same interchanges from k_lu_laswp once the panel
 // is done, before anything reads them, so every element ends identical).
-// One workgroup.
+// One 256-thread workgroup. Each thread scans its rows in ascending order
+// (chunks of 8 rows whose loads are issued together) keeping the first
+// strict maximum of |a|; the waves then reduce lexicographically -- for
+// each limb the maximum over the lanes still tied on the higher limbs (DPP
+// row rotations and lane reads), then the smallest row among the lanes left
+// -- which is MFLA's first strict maximum for finite values. A NaN in row k
+// keeps the pivot at k, as MFLA's scan does.
+template <int CTRL>
+__device__ __forceinline__ double lp_dpp(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lp_rl(double x, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l), __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+__device__ __forceinline__ double lp_wave_max(double x) {
+    x = fmax(x, lp_dpp<0xB1>(x));
+    x = fmax(x, lp_dpp<0x4E>(x));
+    x = fmax(x, lp_dpp<0x124>(x));
+    x = fmax(x, lp_dpp<0x128>(x));
+    return fmax(fmax(lp_rl(x, 0), lp_rl(x, 16)), fmax(lp_rl(x, 32), lp_rl(x, 48)));
+}
+__device__ __forceinline__ int lp_wave_min(int i) {
+    i = min(i, __builtin_amdgcn_update_dpp(0, i, 0xB1, 0xf, 0xf, false));
+    i = min(i, __builtin_amdgcn_update_dpp(0, i, 0x4E, 0xf, 0xf, false));
+    i = min(i, __builtin_amdgcn_update_dpp(0, i, 0x124, 0xf, 0xf, false));
+    i = min(i, __builtin_amdgcn_update_dpp(0, i, 0x128, 0xf, 0xf, false));
+    return min(min(__builtin_amdgcn_readlane(i, 0), __builtin_amdgcn_readlane(i, 16)),
+               min(__builtin_amdgcn_readlane(i, 32), __builtin_amdgcn_readlane(i, 48)));
+}
+
+// lexicographic wave argmax: every lane ends with the wave's best value and
+// its smallest row
 template <int N>
-__global__ __launch_bounds__(1024) void k_lu_pivot_swap(DV A, int64_t k, int64_t m, int64_t c0, int64_t c1, int64_t *ipiv, Status *st) {
-    if (st->info != 0) return;
-    __shared__ double sv_buf[1024 * N];
-    __shared__ int64_t si_buf[1024];
-    mf::V<N> best = mf::zero<N>();
-    int64_t bi = -1;
-    for (int64_t r = k + threadIdx.x; r < m; r += blockDim.x) {
-        mf::V<N> c = mf::vabs(ld<N>(A, r, k));
-        if (bi < 0 || mf::gt(c, best)) { best = c; bi = r; }
+__device__ __forceinline__ void lp_wave_argmax(mf::V<N> &best, int &bi) {
+    bool cand = true;
+    mf::V<N> m;
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+        m.x[l] = lp_wave_max(cand ? best.x[l] : -INFINITY);
+        cand = cand && best.x[l] == m.x[l];
     }
-    wg_argmax<N>(best, bi, sv_buf, si_buf);
-    const int64_t piv = bi;
-    if (threadIdx.x == 0) { ipiv[k] = piv + 1; st->aux = piv; }
-    if (piv != k)
-        for (int64_t j = c0 + threadIdx.x; j < c1; j += blockDim.x) swap_el<N>(A, k, j, piv, j);
+    bi = lp_wave_min(cand ? bi : INT32_MAX);
+    best = m;
+}
+
+constexpr int LP_THREADS = 256, LP_CHUNK = 8;
+
+template <int N>
+__global__ __launch_bounds__(LP_THREADS) void k_lu_pivot_swap(DV A, int64_t k, int64_t m, int64_t c0, int64_t c1, int64_t *ipiv, Status *st) {
+    __shared__ double wv[LP_THREADS / 64][N];
+    __shared__ int wi[LP_THREADS / 64];
+    __shared__ int nan_kk;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t info = st->info;                 // checked after the first loads are in flight
+    const int rows = (int)(m - k);
+    mf::V<N> best = mf::zero<N>();
+    best.x[0] = -1.0;                              // below every |a|; a NaN never replaces it
+    int bi = INT32_MAX;
+    for (int base = 0; base < rows; base += LP_THREADS * LP_CHUNK) {
+        mf::V<N> v[LP_CHUNK];
+#pragma unroll
+        for (int i = 0; i < LP_CHUNK; ++i) {       // clamped rows: every load in bounds
+            const int r = base + t + LP_THREADS * i;
+            v[i] = ld<N>(A, k + (r < rows ? r : rows - 1), k);
+        }
+        if (base == 0 && t == 0) nan_kk = __builtin_isnan(v[0].x[0]) ? 1 : 0;
+#pragma unroll
+        for (int i = 0; i < LP_CHUNK; ++i) {
+            const int r = base + t + LP_THREADS * i;
+            const mf::V<N> c = mf::vabs(v[i]);
+            if (r < rows && mf::gt(c, best)) { best = c; bi = r; }
+        }
+    }
+    if (info != 0) return;                         // uniform
+    lp_wave_argmax<N>(best, bi);
+    if (lane == 0) {
+#pragma unroll
+        for (int l = 0; l < N; ++l) wv[wave][l] = best.x[l];
+        wi[wave] = bi;
+    }
+    __syncthreads();
+    mf::V<N> b;
+#pragma unroll
+    for (int l = 0; l < N; ++l) b.x[l] = wv[0][l];
+    int ib = wi[0];
+#pragma unroll
+    for (int w = 1; w < LP_THREADS / 64; ++w) {
+        mf::V<N> o;
+#pragma unroll
+        for (int l = 0; l < N; ++l) o.x[l] = wv[w][l];
+        const int io = wi[w];
+        if (mf::gt(o, b) || (mf::eq(o, b) && io < ib)) { b = o; ib = io; }
+    }
+    const int64_t piv = (nan_kk || ib == INT32_MAX) ? k : k + ib;
+    if (t == 0) { ipiv[k] = piv + 1; st->aux = piv; }
+    if (piv != k) {
+        for (int64_t e = t; e < (c1 - c0) * N; e += LP_THREADS) {
+            const int64_t j = c0 + e / N;
+            const int l = (int)(e % N);
+            double *pk = A.p + k * A.rs + j * A.cs + l * A.ls, *pp = A.p + piv * A.rs + j * A.cs + l * A.ls;
+            const double x = *pk, y = *pp;
+            *pk = y;
+            *pp = x;
+        }
+    }
 }
 
 // The panel's interchanges ipiv[first .. first + count) applied in order to
@@ -182,6 +272,13 @@ __global__ void k_lu_laswp(DV A, int64_t first, int64_t count, const int64_t *ip
 }
 
 // lu.jl:104-109 + rank-1 update lu.jl:47-82 restricted to the panel columns.
+// 16 lanes per row: each lane forms the row's multiplier l = a(r, k) *
+// inv(a(k, k)) and applies the msubs of columns k + 1 + (lane % 16) + 16 q
+// (one for the default 16-column panels); lane 0 of
+// the row stores l (the other lanes of its wave loaded a(r, k) in the same
+// instruction, before that store).
+constexpr int LS_LANES = 16;
+
 template <int N>
 __global__ __launch_bounds__(256) void k_lu_scale_update(DV A, int64_t k, int64_t last, int64_t m, Status *st) {
     if (st->info != 0) return;
@@ -191,12 +288,13 @@ __global__ __launch_bounds__(256) void k_lu_scale_update(DV A, int64_t k, int64_
         if (gid == 0) st->info = k + 1;
         return;
     }
-    const int64_t r = k + 1 + gid;
+    const int64_t r = k + 1 + gid / LS_LANES;
+    const int j = (int)(gid % LS_LANES);
     if (r >= m) return;
     const mf::V<N> inv = mf::inv(akk);
-    mf::V<N> l = mf::mul(ld<N>(A, r, k), inv);
-    stv<N>(A, r, k, l);
-    for (int64_t c = k + 1; c <= last; ++c) stv<N>(A, r, c, mf::msub(ld<N>(A, r, c), l, ld<N>(A, k, c)));
+    const mf::V<N> l = mf::mul(ld<N>(A, r, k), inv);
+    for (int64_t c = k + 1 + j; c <= last; c += LS_LANES) stv<N>(A, r, c, mf::msub(ld<N>(A, r, c), l, ld<N>(A, k, c)));
+    if (j == 0) stv<N>(A, r, k, l);
 }
 
 // Sequential row interchanges on right-hand sides (solve.jl:85-113).
@@ -431,7 +529,8 @@ hipError_t chol_panel(int N, hipStream_t s, const DV &A, int64_t first, int64_t 
 }
 
 hipError_t lu_pivot_swap(int N, hipStream_t s, const DV &A, int64_t k, int64_t m, int64_t c0, int64_t c1, int64_t *ipiv, Status *st) {
-    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_pivot_swap<NN>, dim3(1), dim3(1024), 0, s, A, k, m, c0, c1, ipiv, st));
+    if (m - k > INT32_MAX / 2) return hipErrorInvalidValue;
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_pivot_swap<NN>, dim3(1), dim3(LP_THREADS), 0, s, A, k, m, c0, c1, ipiv, st));
     return hipGetLastError();
 }
 
@@ -445,7 +544,7 @@ hipError_t lu_laswp(int N, hipStream_t s, const DV &A, int64_t first, int64_t co
 
 hipError_t lu_scale_update(int N, hipStream_t s, const DV &A, int64_t k, int64_t last, int64_t m, Status *st) {
     const int64_t rows = m - k - 1;
-    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_scale_update<NN>, dim3(nb(rows, 256)), dim3(256), 0, s, A, k, last, m, st));
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_lu_scale_update<NN>, dim3(nb((rows > 0 ? rows : 1) * LS_LANES, 256)), dim3(256), 0, s, A, k, last, m, st));
     return hipGetLastError();
 }
 
