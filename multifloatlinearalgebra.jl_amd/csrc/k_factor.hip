@@ -187,10 +187,15 @@ __device__ __forceinline__ void lp_wave_argmax(mf::V<N> &best, int &bi) {
     best = m;
 }
 
-constexpr int LP_THREADS = 256, LP_CHUNK = 8;
+// The panel kernels run while the look-ahead trailing update fills the CUs:
+// they are held to 64 VGPRs (8 waves per SIMD), which fit beside the update's
+// waves, and raise their wave priority so the SIMD issues their
+// instructions first.
+constexpr int LP_THREADS = 256, LP_CHUNK = 4;
 
 template <int N>
-__global__ __launch_bounds__(LP_THREADS) void k_lu_pivot_swap(DV A, int64_t k, int64_t m, int64_t c0, int64_t c1, int64_t *ipiv, Status *st) {
+__global__ __launch_bounds__(LP_THREADS, 8) void k_lu_pivot_swap(DV A, int64_t k, int64_t m, int64_t c0, int64_t c1, int64_t *ipiv, Status *st) {
+    __builtin_amdgcn_s_setprio(3);
     __shared__ double wv[LP_THREADS / 64][N];
     __shared__ int wi[LP_THREADS / 64];
     __shared__ int nan_kk;
@@ -280,7 +285,8 @@ __global__ void k_lu_laswp(DV A, int64_t first, int64_t count, const int64_t *ip
 constexpr int LS_LANES = 16;
 
 template <int N>
-__global__ __launch_bounds__(256) void k_lu_scale_update(DV A, int64_t k, int64_t last, int64_t m, Status *st) {
+__global__ __launch_bounds__(256, 8) void k_lu_scale_update(DV A, int64_t k, int64_t last, int64_t m, Status *st) {
+    __builtin_amdgcn_s_setprio(3);
     if (st->info != 0) return;
     const mf::V<N> akk = ld<N>(A, k, k);
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
