@@ -30,7 +30,8 @@ constexpr int CHOL_LDS = 32;
 constexpr int CHOL_EPT = (CHOL_LDS * (CHOL_LDS + 1) / 2 + 255) / 256;   // elements per thread
 
 template <int N>
-__global__ __launch_bounds__(256) void k_chol_panel_lds(DV A, int64_t first, int64_t last, Status *st) {
+__global__ __launch_bounds__(256, 8) void k_chol_panel_lds(DV A, int64_t first, int64_t last, Status *st) {
+    __builtin_amdgcn_s_setprio(3);   // runs beside the look-ahead SYRK: 64 VGPRs, issued first
     if (st->info != 0) return;
     __shared__ double P[CHOL_LDS * CHOL_LDS * N];     // P[(q * 32 + r) * N + l] = L(first + r, first + q)
     __shared__ double piv_s[N];
@@ -260,6 +261,7 @@ __global__ __launch_bounds__(LP_THREADS, 8) void k_lu_pivot_swap(DV A, int64_t k
 // up to that column, and the unused ipiv entries are the identity.
 template <int N>
 __global__ void k_lu_laswp(DV A, int64_t first, int64_t count, const int64_t *ipiv, int64_t a0, int64_t na, int64_t b0, int64_t nb) {
+    __builtin_amdgcn_s_setprio(3);
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (na + nb) * N) return;
     const int64_t c = e / N;
