@@ -27,13 +27,16 @@ constexpr int CHOL_LDS = 32;
 // division instead of its whole chain; each element still applies its terms
 // in ascending k, then the square root (diagonal) or the division by the
 // pivot (below it), exactly MFLA's sequence.
-constexpr int CHOL_EPT = (CHOL_LDS * (CHOL_LDS + 1) / 2 + 255) / 256;   // elements per thread
+// PB: the largest panel width of the instance (16: the default 16-column
+// panels, whose 8 KB of LDS fit beside the look-ahead update's workgroups).
+template <int PB> constexpr int chol_ept() { return (PB * (PB + 1) / 2 + 255) / 256; }   // elements per thread
 
-template <int N>
+template <int N, int PB>
 __global__ __launch_bounds__(256, 8) void k_chol_panel_lds(DV A, int64_t first, int64_t last, Status *st) {
     __builtin_amdgcn_s_setprio(3);   // runs beside the look-ahead SYRK: 64 VGPRs, issued first
     if (st->info != 0) return;
-    __shared__ double P[CHOL_LDS * CHOL_LDS * N];     // P[(q * 32 + r) * N + l] = L(first + r, first + q)
+    constexpr int CHOL_EPT = chol_ept<PB>();
+    __shared__ double P[PB * PB * N];     // P[(q * PB + r) * N + l] = L(first + r, first + q)
     __shared__ double piv_s[N];
     __shared__ int fail_s;
     const int nb = (int)(last - first + 1);
@@ -58,7 +61,7 @@ __global__ __launch_bounds__(256, 8) void k_chol_panel_lds(DV A, int64_t first, 
     auto get = [&](int r, int q) {
         mf::V<N> x;
 #pragma unroll
-        for (int l = 0; l < N; ++l) x.x[l] = P[(q * CHOL_LDS + r) * N + l];
+        for (int l = 0; l < N; ++l) x.x[l] = P[(q * PB + r) * N + l];
         return x;
     };
     for (int k = 0; k < nb; ++k) {
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(256, 8) void k_chol_panel_lds(DV A, int64_t first, 
                     const mf::V<N> p = mf::sqrt(v[i]);
                     stv<N>(A, first + k, first + k, p);
 #pragma unroll
-                    for (int l = 0; l < N; ++l) { piv_s[l] = p.x[l]; P[(k * CHOL_LDS + k) * N + l] = p.x[l]; }
+                    for (int l = 0; l < N; ++l) { piv_s[l] = p.x[l]; P[(k * PB + k) * N + l] = p.x[l]; }
                 }
             }
         }
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(256, 8) void k_chol_panel_lds(DV A, int64_t first, 
                 const mf::V<N> q = mf::div(v[i], p);
                 stv<N>(A, first + er[i], first + k, q);
 #pragma unroll
-                for (int l = 0; l < N; ++l) P[(k * CHOL_LDS + er[i]) * N + l] = q.x[l];
+                for (int l = 0; l < N; ++l) P[(k * PB + er[i]) * N + l] = q.x[l];
             }
         }
         __syncthreads();
@@ -528,8 +531,10 @@ static inline unsigned nb(int64_t work, int b) { return (unsigned)cdiv(work > 0 
 template <int N> static S<N> mk(const double *v) { S<N> s; for (int l = 0; l < N; ++l) s.x[l] = v[l]; return s; }
 
 hipError_t chol_panel(int N, hipStream_t s, const DV &A, int64_t first, int64_t last, Status *st) {
-    if (last - first + 1 <= CHOL_LDS) {
-        MF_DISPATCH(N, hipLaunchKernelGGL(k_chol_panel_lds<NN>, dim3(1), dim3(256), 0, s, A, first, last, st));
+    if (last - first + 1 <= 16) {
+        MF_DISPATCH(N, hipLaunchKernelGGL((k_chol_panel_lds<NN, 16>), dim3(1), dim3(256), 0, s, A, first, last, st));
+    } else if (last - first + 1 <= CHOL_LDS) {
+        MF_DISPATCH(N, hipLaunchKernelGGL((k_chol_panel_lds<NN, CHOL_LDS>), dim3(1), dim3(256), 0, s, A, first, last, st));
     } else {
         MF_DISPATCH(N, hipLaunchKernelGGL(k_chol_panel<NN>, dim3(1), dim3(256), 0, s, A, first, last, st));
     }
