@@ -151,8 +151,11 @@ __device__ void wg_backward(const DV &X, const DV &CA, const DV &D, int64_t n, i
     else wg_systolic<N>(X, CA, D, n, nrhs, false, unit, xs);
 }
 
+// Up to three limbs the kernel is held to 128 VGPRs: four systems per CU
+// hide each other's per-column barriers and latencies better than two
+// unspilled ones (10,000 x3 systems: 122 -> 100 ms, with a small spill).
 template <int N>
-__global__ __launch_bounds__(BT) void k_batched(const BatchEntry *E, int64_t count, double *Aall, double *Ball, double *Dall,
+__global__ __launch_bounds__(BT, N <= 3 ? 4 : 2) void k_batched(const BatchEntry *E, int64_t count, double *Aall, double *Ball, double *Dall,
                                                int64_t *Pall, uint8_t *Kall, int64_t *info, double *origmax, unsigned int *counter,
                                                S<N> alpha_s, int64_t chol_block, int exact_backward) {
     __shared__ double sv_buf[BT * N];
