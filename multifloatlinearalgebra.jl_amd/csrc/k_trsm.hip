@@ -67,9 +67,14 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
 // every lane of the group forms the same quotient x_s (so no lane waits for
 // another), then the later unknowns subtract their term. Per element the
 // operations and their order are those of k_trsm_diag.
+//
+// With look-ahead these solves run beside the side stream's tile updates:
+// held to 64 VGPRs (8 waves per SIMD) they fit next to the update's waves,
+// and their raised wave priority lets the SIMD issue them first.
 template <int N, bool FWD, int G>
-__global__ __launch_bounds__(256) void k_trsm_diag_wave(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap,
-                                                        const Status *st, int64_t lim) {
+__global__ __launch_bounds__(256, 8) void k_trsm_diag_wave(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap,
+                                                           const Status *st, int64_t lim) {
+    __builtin_amdgcn_s_setprio(3);
     if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
     constexpr int RPW = 64 / G;
     const int lane = threadIdx.x & 63, grp = lane / G, pl = lane % G;
