@@ -105,7 +105,7 @@ typedef struct mfla_config {
     int64_t gemm_panel_columns;
     int64_t gemm_micro_columns;
     int32_t trsm_order;            /* MFLA_TRSM_ORDER_*; auto = exact for n <= 256 */
-    int32_t gemm_tile;             /* -1 auto, 0: 16x16, 1: 32x32, 2: 32x64, 3: 64x64 (1-2 limbs; else 32x64) output tiles; values outside 0..3 mean 16x16 */
+    int32_t gemm_tile;             /* -1 auto (16x16 for lower-triangle outputs), 0: 16x16, 1: 32x32, 2: 32x64, 3: 64x64 (1-2 limbs; else 32x64) output tiles; values outside 0..3 mean 16x16 */
 } mfla_config;
 
 typedef struct mfla_handle_s *mfla_handle;
