@@ -217,3 +217,51 @@ def test_native_refinement_errors():
         R.refine_solve(A, B[:, :, :10].contiguous(), "cholesky")
     with pytest.raises(M.ArgumentError):
         R.refine_solve(A[:1].contiguous(), B[:1].contiguous(), "cholesky")
+
+
+@pytest.mark.parametrize("bad", [70, 200, 299])
+def test_dgetrf_zero_column_in_later_blocks(bad):
+    """A zero column in a later 64-column block (and in the last, partial one):
+    info names it, its pivot is the identity, and the factors of the columns
+    before it still reproduce P A."""
+    from mfla import api as M
+
+    n = 300
+    rng = np.random.default_rng(bad)
+    A0 = rng.standard_normal((n, n))
+    A0[:, bad] = 0.0
+    A = torch.from_numpy(A0.T.copy()[None]).cuda()
+    ipiv = torch.zeros(n, dtype=torch.int64, device="cuda")
+    assert M.dgetrf_(A, ipiv) == bad + 1
+    piv = ipiv.cpu().numpy()
+    assert piv[bad] == bad + 1
+    assert all(k + 1 <= p <= n for k, p in enumerate(piv))
+    F = _np(A)[0].T
+    PA = A0.copy()
+    for k in range(n):
+        p = piv[k] - 1
+        PA[[k, p]] = PA[[p, k]]
+    Lf = np.tril(F, -1) + np.eye(n)
+    U = np.triu(F)
+    c = bad                                       # columns before the zero one are fully factored
+    assert np.abs(PA[:, :c] - (Lf @ U)[:, :c]).max() <= 1e-13 * n
+
+
+@pytest.mark.parametrize("bad", [3, 40, 104, 250])
+def test_dpotrf_inverse_failing_pivot_in_later_panels(bad):
+    """Failing pivots in each 16-column panel of a 64 x 64 diagonal block and
+    in a later block report info = column + 1."""
+    from mfla import api as M
+
+    n = 300
+    rng = np.random.default_rng(bad)
+    G = rng.standard_normal((n, n))
+    S = G @ G.T / n + np.eye(n)
+    S[bad, bad] = -5.0
+    A = torch.from_numpy(S.T.copy()[None]).cuda()
+    Inv = torch.zeros_like(A)
+    info = M.dpotrf_inverse_(A, Inv)
+    # every leading block before `bad` is SPD and the Schur complement pivot
+    # at `bad` is -5 minus a nonnegative term
+    np.linalg.cholesky(S[:bad, :bad])
+    assert info == bad + 1
