@@ -812,7 +812,7 @@ int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *p
         int64_t pfirst = 0;
         while (pfirst < n) {
             const int64_t reqlast = std::min(pfirst + b, n) - 1;
-            HIPCHK(mfk::ldlt_panel(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, st));
+            HIPCHK(mfk::ldlt_panel(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, st));
             int64_t plast = 0;
             const int64_t r = read_info(h, &plast);
             if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");

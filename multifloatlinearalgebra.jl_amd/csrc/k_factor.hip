@@ -336,39 +336,246 @@ __global__ __launch_bounds__(512) void k_ldlt_unblocked(DV A, int64_t n, DV dsub
 
 // Lazy blocked panel (ldlt.jl:414-593). Writes info to st->info and the
 // (possibly extended) last panel column to st->aux.
+//
+// A Schur entry (i1, i2) -> (row, col) = (max, min) is taken against the
+// pivots q = pfirst..k-1 of this panel in MFLA's order: a 1x1 pivot
+// subtracts A(row, q) * (A(q, q) * A(col, q)), a 2x2 pivot subtracts
+// A(row, q) * c1 + A(row, q + 1) * c2 with c1 = A(q, q) * A(col, q) +
+// dsub(q) * A(col, q + 1), c2 = dsub(q) * A(col, q) + A(q + 1, q + 1) *
+// A(col, q + 1).
+//
+// When the panel (with a possible 2x2 extension) is at most W = the default
+// width + 1 columns wide, every evaluation issues all of its panel loads at
+// once (clamped indices, no branch on a loaded value) and takes the pivot
+// kinds, the pivots' diagonals and dsub from LDS; a column pass (fixed col)
+// takes the row-independent factors (the parenthesized products above) from
+// LDS, and the row pass (fixed row) takes A(row, q) from LDS. Each value is
+// formed by the same operations as the per-element form, which wider panels
+// use.
+template <int N> constexpr int ldlt_pw() { return N <= 2 ? 17 : (N == 3 ? 13 : 9); }
+
+template <int N>
+struct LazyPanelEntries {
+    static constexpr int W = ldlt_pw<N>();
+    DV A, dsub;
+    const uint8_t *blocks;
+    int64_t pfirst, kk;
+    bool hoist;
+    int *kind;          // [W]: blocks[q] (1: 1x1 pivot, 2: first column of a 2x2, 0: its second)
+    double *dqq, *dsq;  // [W * N]: A(q, q), dsub(q)
+    double *coef;       // [2][W * N]: column factors of prep(col, set)
+    double *rowv;       // [W * N]: A(row, q) of prow(row)
+    double *er, *ec;    // [W * N]: A(row, q), A(col, q) of one uniform entry()
+    double *keepbuf;    // device [2 * N * n]: the column pass's values, limb l of row r at l * n + r
+    int64_t n;
+    mf::V<N> akk;       // the last entry(kk, kk) (row kk keeps it until a pivot moves row kk)
+    bool rowk;          // er and ec hold A(kk, q) (staged by setup, until an interchange moves row kk or entry() restages)
+
+    // kk is the same in every lane: read it as a scalar so the per-pivot
+    // branches below are scalar branches
+    __device__ int nt() const { return __builtin_amdgcn_readfirstlane((int)(kk - pfirst)); }
+    __device__ int kind_of(int j) const { return __builtin_amdgcn_readfirstlane(kind[j]); }
+
+    // keeps the LDS reads of one pivot's term from being hoisted above the
+    // previous term (all of them live at once would spill)
+    __device__ static void lds_fence() { asm volatile("" ::: "memory"); }
+
+    // the per-element form
+    __device__ mf::V<N> slow(int64_t row, int64_t col) const {
+        mf::V<N> v = ld<N>(A, row, col);
+        int64_t q = pfirst;
+        while (q < kk) {
+            if (blocks[q] == 1) {
+                v = mf::sub(v, mf::mul(ld<N>(A, row, q), mf::mul(ld<N>(A, q, q), ld<N>(A, col, q))));
+                q += 1;
+            } else {
+                const mf::V<N> rf = ld<N>(A, row, q), rs = ld<N>(A, row, q + 1), cf = ld<N>(A, col, q), cs = ld<N>(A, col, q + 1);
+                const mf::V<N> dq = ld<N>(dsub, q, 0);
+                const mf::V<N> c1 = mf::add(mf::mul(ld<N>(A, q, q), cf), mf::mul(dq, cs));
+                const mf::V<N> c2 = mf::add(mf::mul(dq, cf), mf::mul(ld<N>(A, q + 1, q + 1), cs));
+                v = mf::sub(v, mf::add(mf::mul(rf, c1), mf::mul(rs, c2)));
+                q += 2;
+            }
+        }
+        return v;
+    }
+
+    // the panel columns of one row, all loads issued before any use
+    __device__ void load_row(int64_t r, mf::V<N> (&x)[W]) const {
+        const int last = nt() > 0 ? nt() - 1 : 0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) x[j] = ld<N>(A, r, pfirst + (j < last ? j : last));
+    }
+
+    // v minus the pivots' terms; fx(j), fy(j) give A(row, q), A(col, q)
+    template <typename FX, typename FY>
+    __device__ mf::V<N> reduce(mf::V<N> v, FX fx, FY fy) const {
+        const int m = nt();
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            lds_fence();
+            if (j < m) {
+                const int t = kind_of(j);
+                if (t == 1) {
+                    v = mf::sub(v, mf::mul(fx(j), mf::mul(lds_get<N>(&dqq[j * N]), fy(j))));
+                } else if (t == 2 && j + 1 < W) {
+                    const mf::V<N> dq = lds_get<N>(&dsq[j * N]), cf = fy(j), cs = fy(j + 1);
+                    const mf::V<N> c1 = mf::add(mf::mul(lds_get<N>(&dqq[j * N]), cf), mf::mul(dq, cs));
+                    const mf::V<N> c2 = mf::add(mf::mul(dq, cf), mf::mul(lds_get<N>(&dqq[(j + 1) * N]), cs));
+                    v = mf::sub(v, mf::add(mf::mul(fx(j), c1), mf::mul(fx(j + 1), c2)));
+                }
+            }
+        }
+        return v;
+    }
+
+    // v minus the pivots' terms of a prepared column: x = A(row, q) in registers
+    __device__ mf::V<N> reduce_coef(mf::V<N> v, const mf::V<N> (&x)[W], int set) const {
+        const int m = nt();
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            lds_fence();
+            if (j < m) {
+                const int t = kind_of(j);
+                const double *c = &coef[(set * W + j) * N];
+                if (t == 1) v = mf::sub(v, mf::mul(x[j], lds_get<N>(c)));
+                else if (t == 2 && j + 1 < W) v = mf::sub(v, mf::add(mf::mul(x[j], lds_get<N>(c)), mf::mul(x[j + 1], lds_get<N>(c + N))));
+            }
+        }
+        return v;
+    }
+
+    // kinds and diagonals of the pivots before kk (every thread, once per step)
+    __device__ void setup() {
+        __syncthreads();
+        if (hoist && (int)threadIdx.x < nt()) {
+            const int j = threadIdx.x;
+            const int64_t q = pfirst + j;
+            kind[j] = blocks[q];
+            lds_put<N>(&dqq[j * N], ld<N>(A, q, q));
+            lds_put<N>(&dsq[j * N], ld<N>(dsub, q, 0));
+            const mf::V<N> a = ld<N>(A, kk, q);
+            lds_put<N>(&er[j * N], a);
+            lds_put<N>(&ec[j * N], a);
+        }
+        rowk = true;
+        __syncthreads();
+    }
+
+    // one entry, the same for every thread (called by every thread)
+    __device__ mf::V<N> entry(int64_t i1, int64_t i2) {
+        const int64_t row = i1 > i2 ? i1 : i2, col = i1 > i2 ? i2 : i1;
+        if (!hoist) {
+            const mf::V<N> v = slow(row, col);
+            if (row == kk && col == kk) akk = v;
+            return v;
+        }
+        if (!(rowk && row == kk && col == kk)) {
+            __syncthreads();                          // earlier readers of er / ec are done
+            if ((int)threadIdx.x < nt()) {
+                const int j = threadIdx.x;
+                lds_put<N>(&er[j * N], ld<N>(A, row, pfirst + j));
+                lds_put<N>(&ec[j * N], ld<N>(A, col, pfirst + j));
+            }
+            rowk = false;
+            __syncthreads();
+        }
+        const mf::V<N> v = reduce(ld<N>(A, row, col), [&](int j) { return lds_get<N>(&er[j * N]); }, [&](int j) { return lds_get<N>(&ec[j * N]); });
+        if (row == kk && col == kk) akk = v;
+        return v;
+    }
+
+    __device__ void prep(int64_t col, int set) {
+        __syncthreads();                              // earlier readers of coef[set] (and of A) are done
+        if (hoist && (int)threadIdx.x < nt()) {
+            const int j = threadIdx.x;
+            const int64_t q = pfirst + j;
+            double *c = &coef[(set * W + j) * N];
+            const int t = kind[j];
+            const bool staged = rowk && col == kk;    // A(kk, q) in er
+            if (t == 1) {
+                lds_put<N>(c, mf::mul(lds_get<N>(&dqq[j * N]), staged ? lds_get<N>(&er[j * N]) : ld<N>(A, col, q)));
+            } else if (t == 2) {
+                const mf::V<N> cf = staged ? lds_get<N>(&er[j * N]) : ld<N>(A, col, q);
+                const mf::V<N> cs = staged ? lds_get<N>(&er[(j + 1) * N]) : ld<N>(A, col, q + 1);
+                const mf::V<N> dq = lds_get<N>(&dsq[j * N]);
+                lds_put<N>(c, mf::add(mf::mul(lds_get<N>(&dqq[j * N]), cf), mf::mul(dq, cs)));
+                lds_put<N>(c + N, mf::add(mf::mul(dq, cf), mf::mul(lds_get<N>(&dqq[(j + 1) * N]), cs)));
+            }
+        }
+        __syncthreads();
+    }
+    __device__ void prep(int64_t col) { prep(col, 0); }
+
+    // entry(r, col) for r > col after prep(col, set)
+    __device__ mf::V<N> centry(int64_t r, int64_t col, int set = 0) const {
+        if (!hoist) return slow(r, col);
+        mf::V<N> x[W];
+        load_row(r, x);
+        return reduce_coef(ld<N>(A, r, col), x, set);
+    }
+    // entry(r, k) and entry(r, k + 1) for r > k + 1 after prep(k, 0), prep(k + 1, 1)
+    __device__ void centry2(int64_t r, int64_t k, mf::V<N> &f, mf::V<N> &s) const {
+        if (!hoist) { f = slow(r, k); s = slow(r, k + 1); return; }
+        mf::V<N> x[W];
+        load_row(r, x);
+        f = reduce_coef(ld<N>(A, r, k), x, 0);
+        s = reduce_coef(ld<N>(A, r, k + 1), x, 1);
+    }
+
+    // column-pass values, reused by the same thread (same rows) when the
+    // pivot leaves column k in place
+    __device__ void keep(int64_t r, const mf::V<N> &v) const {
+#pragma unroll
+        for (int l = 0; l < N; ++l) keepbuf[l * n + r] = v.x[l];
+    }
+    __device__ mf::V<N> kept(int64_t r, int set = 0) const {
+        mf::V<N> v;
+#pragma unroll
+        for (int l = 0; l < N; ++l) v.x[l] = keepbuf[(set * N + l) * n + r];
+        return v;
+    }
+    __device__ void keep2(int64_t r, const mf::V<N> &a, const mf::V<N> &b) const {
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+            keepbuf[l * n + r] = a.x[l];
+            keepbuf[(N + l) * n + r] = b.x[l];
+        }
+    }
+
+    __device__ void prow(int64_t row) {
+        __syncthreads();
+        if (hoist && (int)threadIdx.x < nt()) lds_put<N>(&rowv[threadIdx.x * N], ld<N>(A, row, pfirst + threadIdx.x));
+        __syncthreads();
+    }
+
+    // entry(row, c) for c < row after prow(row)
+    __device__ mf::V<N> rentry(int64_t row, int64_t c) const {
+        if (!hoist) return slow(row, c);
+        mf::V<N> y[W];
+        load_row(c, y);
+        return reduce(ld<N>(A, row, c), [&](int j) { return lds_get<N>(&rowv[j * N]); }, [&](int j) { return y[j]; });
+    }
+};
+
 template <int N>
 __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfirst, int64_t reqlast, DV dsub, int64_t *pivots,
-                                                    uint8_t *blocks, S<N> alpha_s, Status *st) {
+                                                    uint8_t *blocks, S<N> alpha_s, double *keepbuf, Status *st) {
     if (st->info != 0) return;
+    constexpr int W = ldlt_pw<N>();
     __shared__ double sv_buf[512 * N];
     __shared__ int64_t si_buf[512];
+    __shared__ int kind[W];
+    __shared__ double dqq[W * N], dsq[W * N], coef[2 * W * N], rowv[W * N], er[W * N], ec[W * N];
     const mf::V<N> alpha = sv(alpha_s);
+    LazyPanelEntries<N> e{A, dsub, blocks, pfirst, pfirst, reqlast - pfirst + 2 <= W, kind, dqq, dsq, coef, rowv, er, ec, keepbuf, n};
     int64_t plast = reqlast;
     int64_t k = pfirst;
     int64_t info = 0;
     while (k <= plast) {
-        // entry(i1, i2) against pivots pfirst..k-1 of this panel
-        const int64_t kk = k;
-        auto entry = [&](int64_t i1, int64_t i2) {
-            const int64_t row = i1 > i2 ? i1 : i2, col = i1 > i2 ? i2 : i1;
-            mf::V<N> v = ld<N>(A, row, col);
-            int64_t q = pfirst;
-            while (q < kk) {
-                if (blocks[q] == 1) {
-                    v = mf::sub(v, mf::mul(ld<N>(A, row, q), mf::mul(ld<N>(A, q, q), ld<N>(A, col, q))));
-                    q += 1;
-                } else {
-                    const mf::V<N> rf = ld<N>(A, row, q), rs = ld<N>(A, row, q + 1), cf = ld<N>(A, col, q), cs = ld<N>(A, col, q + 1);
-                    const mf::V<N> dq = ld<N>(dsub, q, 0);
-                    const mf::V<N> c1 = mf::add(mf::mul(ld<N>(A, q, q), cf), mf::mul(dq, cs));
-                    const mf::V<N> c2 = mf::add(mf::mul(dq, cf), mf::mul(ld<N>(A, q + 1, q + 1), cs));
-                    v = mf::sub(v, mf::add(mf::mul(rf, c1), mf::mul(rs, c2)));
-                    q += 2;
-                }
-            }
-            return v;
-        };
-        BKChoice ch = wg_select_bk<N>(n, k, alpha, entry, sv_buf, si_buf);
+        e.kk = k;
+        e.setup();
+        BKChoice ch = wg_select_bk<N>(n, k, alpha, e, sv_buf, si_buf);
         if (ch.bs == 0) { info = k + 1; break; }
         if (ch.bs == 2 && k == plast) {
             if (!(k < n - 1)) { info = k + 1; break; }
@@ -378,34 +585,47 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
             if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 1; }
             __syncthreads();
             wg_sym_swap<N>(A, n, k, ch.pivot, k);
-            const mf::V<N> d = entry(k, k);
+            if (ch.pivot != k) e.rowk = false;       // row k now holds the pivot row
+            const mf::V<N> d = ch.pivot == k ? e.akk : e.entry(k, k);
             if (mf::is_zero(d)) { info = k + 1; break; }
-            __syncthreads();
-            if (threadIdx.x == 0) stv<N>(A, k, k, d);
-            for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(entry(r, k), d));
+            if (ch.pivot == k) {
+                // no interchange: column k is what the column pass saw (each
+                // thread reads back its own rows)
+                __syncthreads();
+                if (threadIdx.x == 0) stv<N>(A, k, k, d);
+                for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(e.kept(r), d));
+            } else {
+                e.prep(k, 0);
+                if (threadIdx.x == 0) stv<N>(A, k, k, d);
+                for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(e.centry(r, k, 0), d));
+            }
             __syncthreads();
             k += 1;
         } else {
             if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 2; blocks[k + 1] = 0; }
             __syncthreads();
             wg_sym_swap<N>(A, n, k + 1, ch.pivot, k);
-            const mf::V<N> d11 = entry(k, k), d21 = entry(k + 1, k), d22 = entry(k + 1, k + 1);
-            __syncthreads();
+            // the interchange of k + 1 and the pivot leaves row k, so d11 is
+            // the entry the pivot choice formed
+            const mf::V<N> d11 = e.akk, d21 = e.entry(k + 1, k), d22 = e.entry(k + 1, k + 1);
+            e.prep(k, 0);
+            e.prep(k + 1, 1);
             if (threadIdx.x == 0) { stv<N>(A, k, k, d11); stv<N>(A, k + 1, k + 1, d22); stv<N>(dsub, k, 0, d21); }
             __syncthreads();
             mf::V<N> s1, s2;
             if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) { info = k + 1; break; }
             int64_t myfail = INT64_MAX;
             for (int64_t r = k + 2 + threadIdx.x; r < n; r += blockDim.x) {
-                bool ok = solve2x2<N>(d11, d21, d22, entry(r, k), entry(r, k + 1), s1, s2);
+                mf::V<N> f, s;
+                e.centry2(r, k, f, s);
+                bool ok = solve2x2<N>(d11, d21, d22, f, s, s1, s2);
                 if (!(ok && mf::is_finite(s1) && mf::is_finite(s2)) && r < myfail) myfail = r;
+                e.keep2(r, s1, s2);                   // read back below by this thread (rows before the first failure)
             }
             const int64_t firstfail = wg_min_i64(myfail, si_buf);
             for (int64_t r = k + 2 + threadIdx.x; r < n && r < firstfail; r += blockDim.x) {
-                mf::V<N> f = entry(r, k), s = entry(r, k + 1);
-                solve2x2<N>(d11, d21, d22, f, s, s1, s2);
-                stv<N>(A, r, k, s1);
-                stv<N>(A, r, k + 1, s2);
+                stv<N>(A, r, k, e.kept(r, 0));
+                stv<N>(A, r, k + 1, e.kept(r, 1));
             }
             __syncthreads();
             if (firstfail != INT64_MAX) { info = k + 1; break; }
@@ -574,9 +794,9 @@ hipError_t ldlt_unblocked(int N, hipStream_t s, const DV &A, int64_t n, const DV
 }
 
 hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
-                      uint8_t *blocks, const double *alpha, Status *st) {
+                      uint8_t *blocks, const double *alpha, double *keepbuf, Status *st) {
     MF_DISPATCH(N, hipLaunchKernelGGL(k_ldlt_panel<NN>, dim3(1), dim3(512), 0, s, A, n, pfirst, reqlast, dsub, pivots, blocks,
-                                      mk<NN>(alpha), st));
+                                      mk<NN>(alpha), keepbuf, st));
     return hipGetLastError();
 }
 

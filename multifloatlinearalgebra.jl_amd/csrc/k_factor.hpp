@@ -17,7 +17,7 @@ hipError_t apply_ipiv(int N, hipStream_t s, const DV &B, int64_t count, int64_t 
 hipError_t ldlt_unblocked(int N, hipStream_t s, const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks,
                           const double *alpha, Status *st);
 hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
-                      uint8_t *blocks, const double *alpha, Status *st);
+                      uint8_t *blocks, const double *alpha, double *keepbuf, Status *st);
 hipError_t ldlt_weighted(int N, hipStream_t s, const DV &A, int64_t pfirst, int64_t plast, int64_t tfirst, int64_t tcount,
                          const DV &dsub, const uint8_t *blocks, const DV &W, const Status *st);
 hipError_t mirror_lower(int N, hipStream_t s, const DV &A, int64_t n, const Status *st);

@@ -128,17 +128,25 @@ __device__ void wg_sym_swap(const DV &A, int64_t n, int64_t f, int64_t s, int64_
 }
 
 // Shared pieces of the Bunch-Kaufman pivot choice (ldlt.jl:183-219 and the
-// lazy variant :451-508). Entry(i1, i2) evaluates the current Schur entry.
+// lazy variant :451-508). The entry source E gives the current Schur entry
+// E.entry(i1, i2); E.prep(col) (called by every thread) readies
+// E.centry(r, col) = entry(r, col) for r > col, and E.prow(row) readies
+// E.rentry(row, c) = entry(row, c) for c < row; E.keep(r, v) sees each
+// column-pass value. The lazy panel forms the factors those passes share
+// once there; the right-looking body reads A.
 struct BKChoice { int bs; int64_t pivot; };
 
-template <int N, typename Entry>
-__device__ BKChoice wg_select_bk(int64_t n, int64_t k, const mf::V<N> &alpha, Entry entry, double *sv_buf, int64_t *si_buf) {
-    const mf::V<N> absakk = mf::vabs(entry(k, k));
+template <int N, typename E>
+__device__ BKChoice wg_select_bk(int64_t n, int64_t k, const mf::V<N> &alpha, E &e, double *sv_buf, int64_t *si_buf) {
+    const mf::V<N> absakk = mf::vabs(e.entry(k, k));
     if (k == n - 1) return BKChoice{mf::is_zero(absakk) ? 0 : 1, k};
     mf::V<N> best = mf::zero<N>();
     int64_t bi = -1;
+    e.prep(k);
     for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) {
-        mf::V<N> c = mf::vabs(entry(r, k));
+        const mf::V<N> v = e.centry(r, k);
+        e.keep(r, v);
+        mf::V<N> c = mf::vabs(v);
         if (bi < 0 || mf::gt(c, best)) { best = c; bi = r; }
     }
     wg_argmax<N>(best, bi, sv_buf, si_buf);
@@ -147,13 +155,28 @@ __device__ BKChoice wg_select_bk(int64_t n, int64_t k, const mf::V<N> &alpha, En
     if (mf::eq(mf::vmax(absakk, colmax), mf::zero<N>())) return BKChoice{0, k};
     if (mf::ge(absakk, mf::mul(alpha, colmax))) return BKChoice{1, k};
     mf::V<N> rm = mf::zero<N>();
-    for (int64_t c = k + threadIdx.x; c < imax; c += blockDim.x) rm = mf::vmax(rm, mf::vabs(entry(imax, c)));
-    for (int64_t r = imax + 1 + threadIdx.x; r < n; r += blockDim.x) rm = mf::vmax(rm, mf::vabs(entry(r, imax)));
+    e.prow(imax);
+    for (int64_t c = k + threadIdx.x; c < imax; c += blockDim.x) rm = mf::vmax(rm, mf::vabs(e.rentry(imax, c)));
+    e.prep(imax);
+    for (int64_t r = imax + 1 + threadIdx.x; r < n; r += blockDim.x) rm = mf::vmax(rm, mf::vabs(e.centry(r, imax)));
     const mf::V<N> rowmax = wg_max<N>(rm, sv_buf);
     if (mf::ge(absakk, mf::mul(mf::mul(alpha, colmax), mf::div(colmax, rowmax)))) return BKChoice{1, k};
-    if (mf::ge(mf::vabs(entry(imax, imax)), mf::mul(alpha, rowmax))) return BKChoice{1, imax};
+    if (mf::ge(mf::vabs(e.entry(imax, imax)), mf::mul(alpha, rowmax))) return BKChoice{1, imax};
     return BKChoice{2, imax};
 }
+
+// Entries of the right-looking factorization: the trailing block is updated
+// in place, so every entry is the stored lower-triangle element.
+template <int N>
+struct StoredEntries {
+    DV A;
+    __device__ mf::V<N> entry(int64_t i, int64_t j) const { return ld<N>(A, i, j); }
+    __device__ void prep(int64_t) const {}
+    __device__ void keep(int64_t, const mf::V<N> &) const {}
+    __device__ mf::V<N> centry(int64_t r, int64_t c) const { return ld<N>(A, r, c); }
+    __device__ void prow(int64_t) const {}
+    __device__ mf::V<N> rentry(int64_t r, int64_t c) const { return ld<N>(A, r, c); }
+};
 
 // (row, col) = (e mod w, e div w), in 32-bit arithmetic when w * w fits.
 __device__ __forceinline__ void sq_index(int64_t e, int64_t w, int64_t &row, int64_t &col) {
@@ -212,14 +235,14 @@ __device__ void wg_mirror_block(const DV &A, int64_t n, int64_t k0) {
 template <int N>
 __device__ void ldlt_unblocked_body(const DV &A, int64_t n, const DV &dsub, int64_t *pivots, uint8_t *blocks, const mf::V<N> &alpha,
                                     int64_t *info_out, double *sv_buf, int64_t *si_buf, double *cbuf = nullptr) {
-    auto entry = [&](int64_t i, int64_t j) { return ld<N>(A, i, j); };
+    StoredEntries<N> entries{A};
     auto fail = [&](int64_t code, int64_t k0) {
         wg_mirror_block<N>(A, n, k0);
         if (threadIdx.x == 0) *info_out = code;
     };
     int64_t k = 0;
     while (k < n) {
-        BKChoice ch = wg_select_bk<N>(n, k, alpha, entry, sv_buf, si_buf);
+        BKChoice ch = wg_select_bk<N>(n, k, alpha, entries, sv_buf, si_buf);
         if (ch.bs == 0) { fail(k + 1, k); return; }
         if (ch.bs == 1) {
             if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 1; }

@@ -13,7 +13,7 @@ import torch
 
 from backends import GpuBackend
 from kat_cases import run_all
-from mfutil import assert_bitwise, rand_mf
+from mfutil import assert_bitwise, rand_mf, renorm
 
 pytestmark = pytest.mark.gpu
 
@@ -189,6 +189,48 @@ def test_ldlt_failure_state_bitwise(M, oracle, N, zero):
     assert np.array_equal(F.pivots.cpu().numpy(), piv_ref) and np.array_equal(F.blocks.cpu().numpy(), blk_ref)
     assert_bitwise(host(F.dsub), dsub_ref, "dsub")
     assert_bitwise(host(Kd), ref, f"failed ldlt x{N} zero column {zero}")
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("n,block", [(150, 0), (260, 3), (200, 70)])
+def test_ldlt_blocked_indefinite_bitwise(M, oracle, N, n, block):
+    """Blocked lazy Bunch-Kaufman on a random symmetric indefinite matrix:
+    interchanges and 2x2 pivots in most panels. Block 70 takes the panel's
+    per-element Schur entries (more panel columns than its LDS factor slots);
+    the others take the per-column factors."""
+    rng = np.random.default_rng(n + 11 * N + block)
+    H = rand_mf(rng, N, n, n)
+    K = renorm(H + np.transpose(H, (0, 2, 1)))
+    ref = K.copy()
+    info_ref, dsub_ref, piv_ref, blk_ref, om_ref = oracle.sytrf(ref, "blocked", block, 1)
+    assert info_ref == 0 and (blk_ref == 2).any() and (piv_ref != np.arange(1, n + 1)).any()
+    Kd = dev(K)
+    F = M.ldlt_(Kd, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_block=block, ldlt_blocked_crossover=1))
+    assert F.info == 0
+    assert np.array_equal(F.pivots.cpu().numpy(), piv_ref), "pivots"
+    assert np.array_equal(F.blocks.cpu().numpy(), blk_ref), "blocks"
+    assert_bitwise(host(F.dsub), dsub_ref, "dsub")
+    assert_bitwise(host(Kd), ref, f"blocked ldlt x{N} n={n} b={block}")
+
+
+@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("zero", [5, 37, 90])
+def test_ldlt_blocked_failure_state_bitwise(M, oracle, N, zero):
+    """A zero row / column stops the blocked factorization inside the panel
+    that reaches it: info, pivots, blocks, dsub and the matrix are MFLA's."""
+    rng = np.random.default_rng(zero + 5 * N)
+    K = kkt(rng, N, 70)
+    K[:, zero, :] = 0.0
+    K[:, :, zero] = 0.0
+    ref = K.copy()
+    info_ref, dsub_ref, piv_ref, blk_ref, _ = oracle.sytrf(ref, "blocked", 8, 1)
+    assert info_ref > 0
+    Kd = dev(K)
+    F = M.ldlt_(Kd, check=False, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_block=8, ldlt_blocked_crossover=1))
+    assert F.info == info_ref
+    assert np.array_equal(F.pivots.cpu().numpy(), piv_ref) and np.array_equal(F.blocks.cpu().numpy(), blk_ref)
+    assert_bitwise(host(F.dsub), dsub_ref, "dsub")
+    assert_bitwise(host(Kd), ref, f"failed blocked ldlt x{N} zero column {zero}")
 
 
 def test_status_codes_and_check(M):
