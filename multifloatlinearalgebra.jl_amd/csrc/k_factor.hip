@@ -661,14 +661,36 @@ __global__ void k_ldlt_weighted(DV A, int64_t pfirst, int64_t plast, int64_t tfi
     }
 }
 
-// ldlt.jl:141-153: A[c, r] = A[r, c] for r > c over an n x n view.
+// ldlt.jl:141-153: A[c, r] = A[r, c] for r > c over an n x n view. One
+// workgroup per 64 x 64 tile of the lower triangle, staged through LDS so
+// that both the column reads and the transposed writes run along the unit
+// stride. A tile reads only lower-triangle elements (and its own diagonal
+// block) and writes only its mirror image, so workgroups never overlap.
+constexpr int MIR_T = 64;
+
 template <int N>
-__global__ void k_mirror_lower(DV A, int64_t n, const Status *st) {
+__global__ __launch_bounds__(256) void k_mirror_lower(DV A, int64_t n, const Status *st) {
     if (st && st->info != 0) return;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n * n) return;
-    const int64_t r = e % n, c = e / n;
-    if (r > c) stv<N>(A, c, r, ld<N>(A, r, c));
+    const int64_t ti = blockIdx.y, tj = blockIdx.x;   // tile row, tile column
+    if (tj > ti) return;
+    __shared__ double tile[MIR_T][MIR_T + 1];
+    const int64_t r0 = ti * MIR_T, c0 = tj * MIR_T;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int l = 0; l < N; ++l) {
+        double *p = A.p + l * A.ls;
+#pragma unroll 4
+        for (int i = ty; i < MIR_T; i += 4) {
+            const int64_t r = r0 + tx, c = c0 + i;
+            if (r < n && c < n) tile[tx][i] = p[r * A.rs + c * A.cs];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int i = ty; i < MIR_T; i += 4) {
+            const int64_t c = c0 + tx, r = r0 + i;
+            if (r < n && c < n && r > c) p[c * A.rs + r * A.cs] = tile[i][tx];
+        }
+        __syncthreads();
+    }
 }
 
 // solve.jl:207-241: LDLT interchanges on right-hand sides, one thread per column.
@@ -810,7 +832,8 @@ hipError_t ldlt_weighted(int N, hipStream_t s, const DV &A, int64_t pfirst, int6
 
 hipError_t mirror_lower(int N, hipStream_t s, const DV &A, int64_t n, const Status *st) {
     if (n <= 1) return hipSuccess;
-    MF_DISPATCH(N, hipLaunchKernelGGL(k_mirror_lower<NN>, dim3(nb(n * n, 256)), dim3(256), 0, s, A, n, st));
+    const unsigned t = (unsigned)((n + MIR_T - 1) / MIR_T);
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_mirror_lower<NN>, dim3(t, t), dim3(256), 0, s, A, n, st));
     return hipGetLastError();
 }
 
