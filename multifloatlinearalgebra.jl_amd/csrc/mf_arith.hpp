@@ -232,12 +232,20 @@ template <> struct Mac<3, NET_FUSED3> {
 };
 
 // --------------------------------------------------------- comparisons
+// Lexicographic over the limbs: the first limb where x < y or x > y decides
+// (a NaN limb decides nothing and passes to the next). Written without early
+// returns: the per-limb early-return form, used as the condition of a
+// running argmax update in a loop, was compiled (ROCm 7.2, gfx950, N >= 2)
+// into code that stopped updating the leading limb of the running value.
 template <int N> MF_HD bool lt(const V<N> &x, const V<N> &y) {
+    bool less = false, open = true;
+#pragma unroll
     for (int i = 0; i < N; ++i) {
-        if (x.x[i] < y.x[i]) return true;
-        if (x.x[i] > y.x[i]) return false;
+        const bool l = x.x[i] < y.x[i], g = x.x[i] > y.x[i];
+        less = less || (open && l);
+        open = open && !l && !g;
     }
-    return false;
+    return less;
 }
 template <int N> MF_HD bool gt(const V<N> &x, const V<N> &y) { return lt(y, x); }
 template <int N> MF_HD bool eq(const V<N> &x, const V<N> &y) {

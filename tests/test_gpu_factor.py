@@ -192,17 +192,18 @@ def test_ldlt_failure_state_bitwise(M, oracle, N, zero):
 
 
 @pytest.mark.parametrize("N", [1, 2, 3, 4])
-@pytest.mark.parametrize("n,block", [(150, 0), (260, 3), (200, 70)])
+@pytest.mark.parametrize("n,block", [(150, 0), (260, 3), (200, 70), (1100, 0)])
 def test_ldlt_blocked_indefinite_bitwise(M, oracle, N, n, block):
     """Blocked lazy Bunch-Kaufman on a random symmetric indefinite matrix:
     interchanges and 2x2 pivots in most panels. Block 70 takes the panel's
     per-element Schur entries (more panel columns than its LDS factor slots);
-    the others take the per-column factors."""
+    the others take the per-column factors. n = 1100 gives each of the
+    panel's 512 threads up to three rows of a column."""
     rng = np.random.default_rng(n + 11 * N + block)
     H = rand_mf(rng, N, n, n)
     K = renorm(H + np.transpose(H, (0, 2, 1)))
     ref = K.copy()
-    info_ref, dsub_ref, piv_ref, blk_ref, om_ref = oracle.sytrf(ref, "blocked", block, 1)
+    info_ref, dsub_ref, piv_ref, blk_ref, om_ref = oracle.sytrf(ref, "blocked", block, 1, threads=8)
     assert info_ref == 0 and (blk_ref == 2).any() and (piv_ref != np.arange(1, n + 1)).any()
     Kd = dev(K)
     F = M.ldlt_(Kd, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_block=block, ldlt_blocked_crossover=1))
@@ -211,6 +212,26 @@ def test_ldlt_blocked_indefinite_bitwise(M, oracle, N, n, block):
     assert np.array_equal(F.blocks.cpu().numpy(), blk_ref), "blocks"
     assert_bitwise(host(F.dsub), dsub_ref, "dsub")
     assert_bitwise(host(Kd), ref, f"blocked ldlt x{N} n={n} b={block}")
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_ldlt_blocked_spd_many_rows_bitwise(M, oracle, N):
+    """Blocked LDLT of a diagonally dominant matrix (1x1 pivots, no
+    interchanges: the panel reuses its column-pass values) with up to three
+    rows of each column per panel thread."""
+    n = 1300
+    rng = np.random.default_rng(91 + N)
+    H = rand_mf(rng, N, n, n)
+    K = renorm(H + np.transpose(H, (0, 2, 1)))
+    K[0, np.arange(n), np.arange(n)] += 4.0 * n
+    ref = K.copy()
+    info_ref, dsub_ref, piv_ref, blk_ref, _ = oracle.sytrf(ref, "blocked", 0, 1, threads=8)
+    assert info_ref == 0 and (blk_ref == 1).all() and (piv_ref == np.arange(1, n + 1)).all()
+    Kd = dev(K)
+    F = M.ldlt_(Kd, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_blocked_crossover=1))
+    assert F.info == 0
+    assert np.array_equal(F.pivots.cpu().numpy(), piv_ref) and np.array_equal(F.blocks.cpu().numpy(), blk_ref)
+    assert_bitwise(host(Kd), ref, f"blocked ldlt x{N} n={n} diagonally dominant")
 
 
 @pytest.mark.parametrize("N", [2, 4])
