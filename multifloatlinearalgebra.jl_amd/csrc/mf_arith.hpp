@@ -249,8 +249,10 @@ template <int N> MF_HD bool lt(const V<N> &x, const V<N> &y) {
 }
 template <int N> MF_HD bool gt(const V<N> &x, const V<N> &y) { return lt(y, x); }
 template <int N> MF_HD bool eq(const V<N> &x, const V<N> &y) {
-    for (int i = 0; i < N; ++i) if (!(x.x[i] == y.x[i])) return false;
-    return true;
+    bool same = true;
+#pragma unroll
+    for (int i = 0; i < N; ++i) same = same && x.x[i] == y.x[i];
+    return same;
 }
 template <int N> MF_HD bool ge(const V<N> &x, const V<N> &y) { return gt(x, y) || eq(x, y); }
 template <int N> MF_HD bool is_zero(const V<N> &x) { for (int i = 0; i < N; ++i) if (x.x[i] != 0.0) return false; return true; }
