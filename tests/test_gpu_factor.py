@@ -214,6 +214,25 @@ def test_ldlt_blocked_indefinite_bitwise(M, oracle, N, n, block):
     assert_bitwise(host(Kd), ref, f"blocked ldlt x{N} n={n} b={block}")
 
 
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_ldlt_unblocked_indefinite_many_rows_bitwise(M, oracle, N):
+    """The right-looking route (forced) on a random symmetric indefinite
+    matrix with two rows of a column per thread of its 512."""
+    n = 700
+    rng = np.random.default_rng(17 + N)
+    H = rand_mf(rng, N, n, n)
+    K = renorm(H + np.transpose(H, (0, 2, 1)))
+    ref = K.copy()
+    info_ref, dsub_ref, piv_ref, blk_ref, _ = oracle.sytrf(ref, "unblocked", 0, 1, threads=8)
+    assert info_ref == 0 and (blk_ref == 2).any()
+    Kd = dev(K)
+    F = M.ldlt_(Kd, config=M.KernelConfig(ldlt_strategy="unblocked"))
+    assert F.info == 0
+    assert np.array_equal(F.pivots.cpu().numpy(), piv_ref) and np.array_equal(F.blocks.cpu().numpy(), blk_ref)
+    assert_bitwise(host(F.dsub), dsub_ref, "dsub")
+    assert_bitwise(host(Kd), ref, f"unblocked ldlt x{N} n={n}")
+
+
 @pytest.mark.parametrize("N", [2, 4])
 def test_ldlt_blocked_spd_many_rows_bitwise(M, oracle, N):
     """Blocked LDLT of a diagonally dominant matrix (1x1 pivots, no

@@ -39,8 +39,11 @@ def test_qr_bitwise(oracle, N, m, n):
 
 @pytest.mark.parametrize("N", [2, 3, 4])
 @pytest.mark.parametrize("m,n,route", [(15, 7, "auto"), (7, 15, "auto"), (9, 9, "auto"), (64, 40, "auto"), (40, 64, "unblocked"),
-                                       (300, 120, "auto"), (120, 300, "blocked"), (33, 17, "blocked")])
+                                       (300, 120, "auto"), (120, 300, "blocked"), (33, 17, "blocked"),
+                                       (120, 700, "blocked"), (60, 1300, "unblocked")])
 def test_rrqr_bitwise(oracle, N, m, n, route):
+    """Wide cases give each pivot-search thread several columns (700 over
+    256 threads on the delayed-norm route, 1300 over 1024 on the hybrid)."""
     import mfla as M
 
     rng = np.random.default_rng(5 * m + n + N)
