@@ -119,6 +119,25 @@ __device__ __forceinline__ void acc_digits(const Acc256 &a, int S, int8_t *d, in
     }
 }
 
+// acc_digits with the digit count a compile-time constant: every word index
+// and shift is static (the runtime form selects among the four words per
+// digit).
+template <int S>
+__device__ __forceinline__ void acc_digits_t(const Acc256 &a, int8_t *d, int64_t stride) {
+    int carry = 0;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const int bit = 7 * q;
+        const int wq = bit >> 6, bq = bit & 63;
+        uint64_t chunk = a.w[wq] >> bq;
+        if (bq > 57 && wq + 1 < 4) chunk |= a.w[wq + 1] << (64 - bq);
+        int b = (int)(chunk & 127) + carry;
+        int dig;
+        if (b >= 64) { dig = b - 128; carry = 1; } else { dig = b; carry = 0; }
+        d[(int64_t)(S - 1 - q) * stride] = (int8_t)dig;
+    }
+}
+
 // Byte offset of digit (i, k) inside one digit plane. Row layout: i * ld +
 // k. Fragment layout (FRAG): 16-row blocks, each a sequence of 64-deep k
 // steps of 1 KB laid out exactly as the lanes of a wave read one MFMA
@@ -132,7 +151,7 @@ __device__ __forceinline__ int64_t oz_off(int64_t i, int64_t k, int64_t ld) {
 
 // dig[s * plane + oz_off(i, k)] for X(i, k), s = 0..S-1; the padding
 // columns k in [K, ld) get zero digits.
-template <int N, bool FRAG>
+template <int N, bool FRAG, int SC>
 __global__ void k_oz_split(int64_t m, int64_t K, DV X, int uplo, const int *e, int S, int8_t *dig, int64_t ld, int64_t plane) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= m * ld) return;
@@ -149,10 +168,11 @@ __global__ void k_oz_split(int64_t m, int64_t K, DV X, int uplo, const int *e, i
             int ex;
             const double f = frexp(fabs(x), &ex);                 // |x| = f 2^ex, f in [0.5, 1)
             const uint64_t mag = (uint64_t)ldexp(f, 53);            // 53-bit integer mantissa
-            acc_add_shifted(a, mag, ex - 53 + 7 * S - ei, x < 0.0);
+            acc_add_shifted(a, mag, ex - 53 + 7 * (SC > 0 ? SC : S) - ei, x < 0.0);
         }
     }
-    acc_digits(a, S, d, plane);
+    if constexpr (SC > 0) acc_digits_t<SC>(a, d, plane);
+    else acc_digits(a, S, d, plane);
 }
 
 // binary64 correction D(k, j) (k in [0, K), column j): per-column exponent
@@ -391,13 +411,19 @@ hipError_t oz_split(int N, hipStream_t s, int64_t m, int64_t K, const DV &X, int
     if (m * ld == 0) return hipSuccess;
     if (S < 1 || 7 * S > 250 || ld % 64 != 0 || ld < K) return hipErrorInvalidValue;
     if (frag && plane < cdiv(m, 16) * 16 * ld) return hipErrorInvalidValue;
-    if (frag) {
-        OZ_DISPATCH(N, hipLaunchKernelGGL((k_oz_split<NN, true>), dim3((unsigned)cdiv(m * ld, 256)), dim3(256), 0, s, m, K, X, uplo, e, S, dig,
-                                          ld, plane));
-    } else {
-        OZ_DISPATCH(N, hipLaunchKernelGGL((k_oz_split<NN, false>), dim3((unsigned)cdiv(m * ld, 256)), dim3(256), 0, s, m, K, X, uplo, e, S,
-                                          dig, ld, plane));
+    const dim3 grid((unsigned)cdiv(m * ld, 256));
+#define OZ_SPLIT(F, SCV) OZ_DISPATCH(N, hipLaunchKernelGGL((k_oz_split<NN, F, SCV>), grid, dim3(256), 0, s, m, K, X, uplo, e, S, dig, ld, plane))
+#define OZ_SPLIT_S(F)                                            \
+    switch (S) {                                                 \
+        case 11: OZ_SPLIT(F, 11); break;                         \
+        case 19: OZ_SPLIT(F, 19); break;                         \
+        case 27: OZ_SPLIT(F, 27); break;                         \
+        case 34: OZ_SPLIT(F, 34); break;                         \
+        default: OZ_SPLIT(F, 0); break;                          \
     }
+    if (frag) { OZ_SPLIT_S(true) } else { OZ_SPLIT_S(false) }
+#undef OZ_SPLIT_S
+#undef OZ_SPLIT
     return hipGetLastError();
 }
 
