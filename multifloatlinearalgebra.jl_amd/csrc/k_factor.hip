@@ -369,6 +369,9 @@ struct LazyPanelEntries {
     double *keepbuf;    // device [2 * N * n]: the column pass's values, limb l of row r at l * n + r
     int64_t n;
     mf::V<N> akk;       // the last entry(kk, kk) (row kk keeps it until a pivot moves row kk)
+    // entry(k, k) beside the first column-pass row: a gain where registers
+    // allow (x1 / x2), a loss for x4
+    static constexpr bool kFuseDiag = N <= 2;
     bool rowk;          // er and ec hold A(kk, q) (staged by setup, until an interchange moves row kk or entry() restages)
 
     // kk is the same in every lane: read it as a scalar so the per-pivot
@@ -514,6 +517,44 @@ struct LazyPanelEntries {
         load_row(r, x);
         return reduce_coef(ld<N>(A, r, col), x, set);
     }
+    // entry(kk, kk) at the start of a step (row kk staged in er / ec by
+    // setup): no barrier, any subset of threads
+    __device__ mf::V<N> diag_entry(int64_t k) {
+        if (!hoist) { akk = slow(k, k); return akk; }
+        akk = reduce(ld<N>(A, k, k), [&](int j) { return lds_get<N>(&er[j * N]); }, [&](int j) { return lds_get<N>(&ec[j * N]); });
+        return akk;
+    }
+
+    // centry(r, k, 0) after prep(k, 0) and entry(k, k), their chains in one
+    // loop (same operations as centry and entry each)
+    __device__ void centry_diag(int64_t r, int64_t k, mf::V<N> &v, mf::V<N> &d) {
+        if (!hoist) { v = slow(r, k); d = slow(k, k); akk = d; return; }
+        mf::V<N> x[W];
+        load_row(r, x);
+        v = ld<N>(A, r, k);
+        d = ld<N>(A, k, k);
+        const int m = nt();
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            lds_fence();
+            if (j < m) {
+                const int t = kind_of(j);
+                const double *c = &coef[j * N];
+                if (t == 1) {
+                    v = mf::sub(v, mf::mul(x[j], lds_get<N>(c)));
+                    d = mf::sub(d, mf::mul(lds_get<N>(&er[j * N]), mf::mul(lds_get<N>(&dqq[j * N]), lds_get<N>(&ec[j * N]))));
+                } else if (t == 2 && j + 1 < W) {
+                    v = mf::sub(v, mf::add(mf::mul(x[j], lds_get<N>(c)), mf::mul(x[j + 1], lds_get<N>(c + N))));
+                    const mf::V<N> dq = lds_get<N>(&dsq[j * N]), cf = lds_get<N>(&ec[j * N]), cs = lds_get<N>(&ec[(j + 1) * N]);
+                    const mf::V<N> c1 = mf::add(mf::mul(lds_get<N>(&dqq[j * N]), cf), mf::mul(dq, cs));
+                    const mf::V<N> c2 = mf::add(mf::mul(dq, cf), mf::mul(lds_get<N>(&dqq[(j + 1) * N]), cs));
+                    d = mf::sub(d, mf::add(mf::mul(lds_get<N>(&er[j * N]), c1), mf::mul(lds_get<N>(&er[(j + 1) * N]), c2)));
+                }
+            }
+        }
+        akk = d;
+    }
+
     // entry(r, k) and entry(r, k + 1) for r > k + 1 after prep(k, 0), prep(k + 1, 1)
     __device__ void centry2(int64_t r, int64_t k, mf::V<N> &f, mf::V<N> &s) const {
         if (!hoist) { f = slow(r, k); s = slow(r, k + 1); return; }

@@ -129,25 +129,56 @@ __device__ void wg_sym_swap(const DV &A, int64_t n, int64_t f, int64_t s, int64_
 
 // Shared pieces of the Bunch-Kaufman pivot choice (ldlt.jl:183-219 and the
 // lazy variant :451-508). The entry source E gives the current Schur entry
-// E.entry(i1, i2); E.prep(col) (called by every thread) readies
+// E.entry(i1, i2) (called by every thread); E.diag_entry(k) = entry(k, k)
+// at the start of a step, callable by any subset of threads; E.prep(col)
+// (called by every thread) readies
 // E.centry(r, col) = entry(r, col) for r > col, and E.prow(row) readies
 // E.rentry(row, c) = entry(row, c) for c < row; E.keep(r, v) sees each
-// column-pass value. The lazy panel forms the factors those passes share
+// column-pass value; E.centry_diag(r, k, v, d) gives centry(r, k) and
+// entry(k, k) together. The lazy panel forms the factors those passes share
 // once there; the right-looking body reads A.
 struct BKChoice { int bs; int64_t pivot; };
 
 template <int N, typename E>
 __device__ BKChoice wg_select_bk(int64_t n, int64_t k, const mf::V<N> &alpha, E &e, double *sv_buf, int64_t *si_buf) {
-    const mf::V<N> absakk = mf::vabs(e.entry(k, k));
-    if (k == n - 1) return BKChoice{mf::is_zero(absakk) ? 0 : 1, k};
+    mf::V<N> absakk;
     mf::V<N> best = mf::zero<N>();
     int64_t bi = -1;
-    e.prep(k);
-    for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) {
-        const mf::V<N> v = e.centry(r, k);
-        e.keep(r, v);
-        mf::V<N> c = mf::vabs(v);
-        if (bi < 0 || mf::gt(c, best)) { best = c; bi = r; }
+    if constexpr (!E::kFuseDiag) {
+        absakk = mf::vabs(e.entry(k, k));
+        if (k == n - 1) return BKChoice{mf::is_zero(absakk) ? 0 : 1, k};
+        e.prep(k);
+        for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) {
+            const mf::V<N> v = e.centry(r, k);
+            e.keep(r, v);
+            mf::V<N> c = mf::vabs(v);
+            if (bi < 0 || mf::gt(c, best)) { best = c; bi = r; }
+        }
+    } else {
+        // entry(k, k) formed beside the thread's first row (two independent
+        // chains in one loop); a thread without rows forms it alone
+        if (k == n - 1) {
+            absakk = mf::vabs(e.entry(k, k));
+            return BKChoice{mf::is_zero(absakk) ? 0 : 1, k};
+        }
+        e.prep(k);
+        const int64_t r0 = k + 1 + threadIdx.x;
+        if (r0 < n) {
+            mf::V<N> v, dg;
+            e.centry_diag(r0, k, v, dg);
+            absakk = mf::vabs(dg);
+            e.keep(r0, v);
+            best = mf::vabs(v);
+            bi = r0;
+        } else {
+            absakk = mf::vabs(e.diag_entry(k));
+        }
+        for (int64_t r = r0 + blockDim.x; r < n; r += blockDim.x) {
+            const mf::V<N> v = e.centry(r, k);
+            e.keep(r, v);
+            mf::V<N> c = mf::vabs(v);
+            if (mf::gt(c, best)) { best = c; bi = r; }
+        }
     }
     wg_argmax<N>(best, bi, sv_buf, si_buf);
     const mf::V<N> colmax = best;
@@ -174,6 +205,12 @@ struct StoredEntries {
     __device__ void prep(int64_t) const {}
     __device__ void keep(int64_t, const mf::V<N> &) const {}
     __device__ mf::V<N> centry(int64_t r, int64_t c) const { return ld<N>(A, r, c); }
+    __device__ void centry_diag(int64_t r, int64_t k, mf::V<N> &v, mf::V<N> &d) const {
+        v = ld<N>(A, r, k);
+        d = ld<N>(A, k, k);
+    }
+    __device__ mf::V<N> diag_entry(int64_t k) const { return ld<N>(A, k, k); }
+    static constexpr bool kFuseDiag = false;
     __device__ void prow(int64_t) const {}
     __device__ mf::V<N> rentry(int64_t r, int64_t c) const { return ld<N>(A, r, c); }
 };
