@@ -4,7 +4,8 @@
  *
  * Each entry point replaces one MFLA generic function for device-resident
  * Float64xN (N = 1..4) matrices; the reference function is cited beside it.
- * A Julia wrapper binds these with ccall; the Python
+ * A Julia wrapper binds these with ccall (one `ccall((:mfla_gemm, lib), ...)`
+ * per entry point, matrices passed as mfla_matrix by value); the Python
  * package multifloatlinearalgebra.jl_amd/mfla binds them with ctypes.
  *
  * Data model. A Float64xN matrix is LIMB-PLANAR: N planes of binary64, each a
