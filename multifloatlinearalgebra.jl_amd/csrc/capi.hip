@@ -39,6 +39,11 @@ struct mfla_handle_s {
     std::mutex lock;            // calls on one handle are serialised
     hipStream_t side = nullptr; // second stream of the look-ahead factorization
     hipEvent_t ev_join = nullptr, ev_panel = nullptr, ev_update = nullptr;
+    // pinned host staging for the small device-to-host reads a call waits on
+    // (status, per-iteration column maxima): one DMA transfer each instead of
+    // a copy through the runtime's pageable staging; null = pageable path
+    unsigned char *pinned = nullptr;
+    static constexpr size_t kPinnedBytes = 256 * 1024;
 };
 
 struct mfla_workspace_s {
@@ -119,11 +124,35 @@ void *scratch(mfla_handle h, size_t bytes) {
 }
 
 int64_t read_info(mfla_handle h, int64_t *aux = nullptr) {
-    Status st{};
-    if (hipMemcpyAsync(&st, h->status, sizeof(Status), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return INT64_MIN;
+    Status local{};
+    Status *st = h->pinned ? reinterpret_cast<Status *>(h->pinned) : &local;
+    if (hipMemcpyAsync(st, h->status, sizeof(Status), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return INT64_MIN;
     if (hipStreamSynchronize(h->stream) != hipSuccess) return INT64_MIN;
-    if (aux) *aux = st.aux;
-    return st.info;
+    if (aux) *aux = st->aux;
+    return st->info;
+}
+
+// Device-to-host reads of `count` (pointer, bytes) pairs on stream s, then a
+// wait: through the pinned staging when they fit, else directly.
+hipError_t read_back(mfla_handle h, hipStream_t s, int count, void *const *dst, const void *const *src, const size_t *bytes) {
+    size_t total = 0;
+    for (int i = 0; i < count; ++i) total += (bytes[i] + 255) & ~(size_t)255;
+    const bool staged = h->pinned && total + 256 <= mfla_handle_s::kPinnedBytes;
+    size_t off = 256;                              // the first 256 bytes hold the status copy
+    for (int i = 0; i < count; ++i) {
+        void *to = staged ? h->pinned + off : dst[i];
+        hipError_t e = hipMemcpyAsync(to, src[i], bytes[i], hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        off += (bytes[i] + 255) & ~(size_t)255;
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess || !staged) return e;
+    off = 256;
+    for (int i = 0; i < count; ++i) {
+        std::memcpy(dst[i], h->pinned + off, bytes[i]);
+        off += (bytes[i] + 255) & ~(size_t)255;
+    }
+    return hipSuccess;
 }
 
 const double kOne[4] = {1.0, 0.0, 0.0, 0.0};
@@ -344,6 +373,9 @@ int mfla_handle_create(mfla_handle *out) {
     if (hipGetDevice(&h->device) != hipSuccess) { delete h; return fail(MFLA_ERR_HIP, "no HIP device"); }
     if (hipMalloc(&h->status, sizeof(Status)) != hipSuccess) { delete h; return fail(MFLA_ERR_HIP, "status allocation failed"); }
     if (hipMalloc(&h->flags, 64 * sizeof(int)) != hipSuccess) { (void)hipFree(h->status); delete h; return fail(MFLA_ERR_HIP, "flag allocation failed"); }
+    void *pin = nullptr;
+    if (hipHostMalloc(&pin, mfla_handle_s::kPinnedBytes, hipHostMallocDefault) == hipSuccess) h->pinned = static_cast<unsigned char *>(pin);
+    else (void)hipGetLastError();                  // pageable reads instead
     *out = h;
     return MFLA_OK;
 }
@@ -353,6 +385,7 @@ int mfla_handle_destroy(mfla_handle h) {
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(h->status);
     (void)hipFree(h->flags);
+    if (h->pinned) (void)hipHostFree(h->pinned);
     if (h->scratch) (void)hipFree(h->scratch);
     if (h->side) {
         (void)hipStreamSynchronize(h->side);
@@ -1506,8 +1539,12 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
         } else {
         // precision of A*D: |D| / |X| = 2^-e needs about (53 N - e) / 52 limbs
         HIPCHK(mfk::col_absmax_hi(s, n, nrhs, D, dmax));
-        HIPCHK(hipMemcpyAsync(hd.data(), dmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        {
+            void *dst[1] = {hd.data()};
+            const void *src[1] = {dmax};
+            const size_t by[1] = {sizeof(double) * (size_t)nrhs};
+            HIPCHK(read_back(h, s, 1, dst, src, by));
+        }
         clk.mark(ST_CHECK);
         double rho = 0.0;
         for (int64_t j = 0; j < nrhs; ++j) {
@@ -1526,9 +1563,12 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
         }
         HIPCHK(mfk::col_absmax_hi(s, n, nrhs, Rhi, rmax));
         HIPCHK(mfk::col_absmax_hi(s, n, nrhs, DV{X.data, 1, X.ld, X.limb_stride}, xmax));
-        HIPCHK(hipMemcpyAsync(hr.data(), rmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(hx.data(), xmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        {
+            void *dst[2] = {hr.data(), hx.data()};
+            const void *src[2] = {rmax, xmax};
+            const size_t by[2] = {sizeof(double) * (size_t)nrhs, sizeof(double) * (size_t)nrhs};
+            HIPCHK(read_back(h, s, 2, dst, src, by));
+        }
         clk.mark(ST_CHECK);
         double be = 0.0;
         for (int64_t j = 0; j < nrhs; ++j) {
