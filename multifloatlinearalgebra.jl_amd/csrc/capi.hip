@@ -307,8 +307,12 @@ int original_maximum(mfla_handle h, int N, const mfla_matrix &A, int lower_only,
     if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     HIPCHK(mfk::col_absmax_region(N, h->stream, dvn(A), A.rows, cols, lower_only, DV{buf, 1, cols, cols}));
     std::vector<double> host((size_t)N * cols);
-    HIPCHK(hipMemcpyAsync(host.data(), buf, sizeof(double) * N * cols, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    {
+        void *dst_[1] = {host.data()};
+        const void *src_[1] = {buf};
+        const size_t by_[1] = {sizeof(double) * N * (size_t)cols};
+        HIPCHK(read_back(h, h->stream, 1, dst_, src_, by_));
+    }
     host_max(N, host.data(), cols, out);
     return MFLA_OK;
 }
@@ -1471,8 +1475,12 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
     } else {
         // P b gathers rows: perm = the ipiv row swaps applied to the identity
         std::vector<int64_t> ip(n), perm(n);
-        HIPCHK(hipMemcpyAsync(ip.data(), w.ipiv, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        {
+            void *dst_[1] = {ip.data()};
+            const void *src_[1] = {w.ipiv};
+            const size_t by_[1] = {sizeof(int64_t) * (size_t)n};
+            HIPCHK(read_back(h, s, 1, dst_, src_, by_));
+        }
         for (int64_t i = 0; i < n; ++i) perm[i] = i;
         for (int64_t k = 0; k < n; ++k) std::swap(perm[k], perm[ip[k] - 1]);
         HIPCHK(hipMemcpyAsync(w.perm, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
@@ -1486,11 +1494,14 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
     std::vector<double> hrows(n), hb(nrhs), hx(nrhs, 0.0), hr(nrhs), hd(nrhs);
     HIPCHK(mfk::drow_abs_sum(s, n, n, Ahi, kernel_uplo(uplo), w.rows, w.part));
     HIPCHK(mfk::col_absmax_hi(s, n, nrhs, DV{B.data, 1, B.ld, B.limb_stride}, bmax));
-    HIPCHK(hipMemcpyAsync(hrows.data(), w.rows, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(hb.data(), bmax, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
     HIPCHK(mfk::zero_fill(N, s, n, nrhs, dvn(X)));
     HIPCHK(mfk::copy(N, s, n, nrhs, R, dvn(B)));
-    HIPCHK(hipStreamSynchronize(s));
+    {
+        void *dst_[2] = {hrows.data(), hb.data()};
+        const void *src_[2] = {w.rows, bmax};
+        const size_t by_[2] = {sizeof(double) * (size_t)n, sizeof(double) * (size_t)nrhs};
+        HIPCHK(read_back(h, s, 2, dst_, src_, by_));
+    }
     double anorm = 0.0;
     for (double v : hrows) anorm = std::max(anorm, v);
     if (ozaki) {   // A's digit planes, once per solve
@@ -1665,8 +1676,12 @@ int mfla_logdet(mfla_handle h, int N, mfla_matrix F, int64_t info, double *out) 
     if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     HIPCHK(mfk::diag_logprod(N, h->stream, n, dvn(F), buf));
     double me[5];
-    HIPCHK(hipMemcpyAsync(me, buf, sizeof(double) * (N + 1), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    {
+        void *dst_[1] = {me};
+        const void *src_[1] = {buf};
+        const size_t by_[1] = {sizeof(double) * (N + 1)};
+        HIPCHK(read_back(h, h->stream, 1, dst_, src_, by_));
+    }
     switch (N) {
         case 1: logdet_host<1>(me, out); break;
         case 2: logdet_host<2>(me, out); break;
@@ -1784,8 +1799,12 @@ int mfla_dot(mfla_handle h, int N, mfla_matrix x, mfla_matrix y, double *out) {
     double *buf = (double *)scratch(h, sizeof(double) * N);
     if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     HIPCHK(mfk::dot(N, h->stream, x.rows, x.cols, dvn(x), dvn(y), buf));
-    HIPCHK(hipMemcpyAsync(out, buf, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    {
+        void *dst_[1] = {out};
+        const void *src_[1] = {buf};
+        const size_t by_[1] = {sizeof(double) * N};
+        HIPCHK(read_back(h, h->stream, 1, dst_, src_, by_));
+    }
     return MFLA_OK;
 }
 
@@ -1924,8 +1943,12 @@ int mfla_geqp3(mfla_handle h, int N, mfla_matrix A, mfla_matrix tau, int64_t *pe
             HIPCHK(mfk::qr_norm_update(N, s, m, n, step, a, t, bs, ls + 1, 1, q));
             actual = ls + 1;
             // a cancellation-sensitive norm ends the panel (qr.jl:588-592)
-            HIPCHK(hipMemcpyAsync(&unreliable, q.unreliable, sizeof(int), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
+            {
+                void *dst_[1] = {&unreliable};
+                const void *src_[1] = {q.unreliable};
+                const size_t by_[1] = {sizeof(int)};
+                HIPCHK(read_back(h, s, 1, dst_, src_, by_));
+            }
             if (unreliable) break;
         }
         const int64_t be = bs + actual - 1;
