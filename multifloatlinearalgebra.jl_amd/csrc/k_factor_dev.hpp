@@ -129,14 +129,13 @@ __device__ void wg_sym_swap(const DV &A, int64_t n, int64_t f, int64_t s, int64_
 
 // Shared pieces of the Bunch-Kaufman pivot choice (ldlt.jl:183-219 and the
 // lazy variant :451-508). The entry source E gives the current Schur entry
-// E.entry(i1, i2) (called by every thread); E.diag_entry(k) = entry(k, k)
-// at the start of a step, callable by any subset of threads; E.prep(col)
-// (called by every thread) readies
-// E.centry(r, col) = entry(r, col) for r > col, and E.prow(row) readies
-// E.rentry(row, c) = entry(row, c) for c < row; E.keep(r, v) sees each
-// column-pass value; E.centry_diag(r, k, v, d) gives centry(r, k) and
-// entry(k, k) together. The lazy panel forms the factors those passes share
-// once there; the right-looking body reads A.
+// E.entry(i1, i2) (called by every thread). E.prep(col) (called by every
+// thread) readies E.centry(r, col) = entry(r, col) for r > col; E.prow(row)
+// readies E.rentry(row, c) = entry(row, c) for c < row; E.keep(r, v) sees
+// each column-pass value. With E::kFuseDiag, E.centry_diag(r, k, v, d) gives
+// centry(r, k) and entry(k, k) together and E.diag_entry(k) gives entry(k, k)
+// at the start of a step to any subset of threads. The lazy panel forms the
+// factors those passes share once; the right-looking body reads A.
 struct BKChoice { int bs; int64_t pivot; };
 
 template <int N, typename E>
@@ -205,11 +204,6 @@ struct StoredEntries {
     __device__ void prep(int64_t) const {}
     __device__ void keep(int64_t, const mf::V<N> &) const {}
     __device__ mf::V<N> centry(int64_t r, int64_t c) const { return ld<N>(A, r, c); }
-    __device__ void centry_diag(int64_t r, int64_t k, mf::V<N> &v, mf::V<N> &d) const {
-        v = ld<N>(A, r, k);
-        d = ld<N>(A, k, k);
-    }
-    __device__ mf::V<N> diag_entry(int64_t k) const { return ld<N>(A, k, k); }
     static constexpr bool kFuseDiag = false;
     __device__ void prow(int64_t) const {}
     __device__ mf::V<N> rentry(int64_t r, int64_t c) const { return ld<N>(A, r, c); }
