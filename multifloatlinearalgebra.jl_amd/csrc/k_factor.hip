@@ -555,20 +555,18 @@ struct LazyPanelEntries {
         akk = d;
     }
 
-    // entry(r, k) and entry(r, k + 1) for r > k + 1 after prep(k, 0), prep(k + 1, 1)
-    __device__ void centry2(int64_t r, int64_t k, mf::V<N> &f, mf::V<N> &s) const {
-        if (!hoist) { f = slow(r, k); s = slow(r, k + 1); return; }
-        mf::V<N> x[W];
-        load_row(r, x);
-        f = reduce_coef(ld<N>(A, r, k), x, 0);
-        s = reduce_coef(ld<N>(A, r, k + 1), x, 1);
-    }
-
-    // column-pass values, reused by the same thread (same rows) when the
-    // pivot leaves column k in place
+    // column-pass values (set 0), read back when the pivot leaves column k
+    // in place and, after an interchange with k + 1, by the 2x2 step
     __device__ void keep(int64_t r, const mf::V<N> &v) const {
 #pragma unroll
         for (int l = 0; l < N; ++l) keepbuf[l * n + r] = v.x[l];
+    }
+    // centry(r, imax) of the row-max pass, r > imax, into set 1: after the
+    // interchange of imax with k (1x1) or k + 1 (2x2) it is the new
+    // entry(r, k) or entry(r, k + 1) of rows below imax
+    __device__ void keep_row(int64_t r, const mf::V<N> &v) const {
+#pragma unroll
+        for (int l = 0; l < N; ++l) keepbuf[(N + l) * n + r] = v.x[l];
     }
     __device__ mf::V<N> kept(int64_t r, int set = 0) const {
         mf::V<N> v;
@@ -636,9 +634,12 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
                 if (threadIdx.x == 0) stv<N>(A, k, k, d);
                 for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(e.kept(r), d));
             } else {
+                // rows below the pivot row: the row-max pass formed their
+                // new entry (set 1); the others are formed here
                 e.prep(k, 0);
                 if (threadIdx.x == 0) stv<N>(A, k, k, d);
-                for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, r, k, mf::div(e.centry(r, k, 0), d));
+                for (int64_t r = k + 1 + threadIdx.x; r < n; r += blockDim.x)
+                    stv<N>(A, r, k, mf::div(r > ch.pivot ? e.kept(r, 1) : e.centry(r, k, 0), d));
             }
             __syncthreads();
             k += 1;
@@ -649,16 +650,22 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
             // the interchange of k + 1 and the pivot leaves row k, so d11 is
             // the entry the pivot choice formed
             const mf::V<N> d11 = e.akk, d21 = e.entry(k + 1, k), d22 = e.entry(k + 1, k + 1);
-            e.prep(k, 0);
             e.prep(k + 1, 1);
             if (threadIdx.x == 0) { stv<N>(A, k, k, d11); stv<N>(A, k + 1, k + 1, d22); stv<N>(dsub, k, 0, d21); }
             __syncthreads();
             mf::V<N> s1, s2;
             if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) { info = k + 1; break; }
             int64_t myfail = INT64_MAX;
+            // the interchange of k + 1 and the pivot row p leaves column k:
+            // entry(r, k) is the column pass's value of row r (of row k + 1
+            // for r = p); entry(r, k + 1) of a row below p is the row-max
+            // pass's value (set 1), the others are formed here. Each thread
+            // reads only its own rows' slots (and slot k + 1) before writing
+            // its rows' solutions into them.
+            const int64_t p = ch.pivot;
             for (int64_t r = k + 2 + threadIdx.x; r < n; r += blockDim.x) {
-                mf::V<N> f, s;
-                e.centry2(r, k, f, s);
+                const mf::V<N> f = e.kept(r == p ? k + 1 : r, 0);
+                const mf::V<N> s = r > p ? e.kept(r, 1) : e.centry(r, k + 1, 1);
                 bool ok = solve2x2<N>(d11, d21, d22, f, s, s1, s2);
                 if (!(ok && mf::is_finite(s1) && mf::is_finite(s2)) && r < myfail) myfail = r;
                 e.keep2(r, s1, s2);                   // read back below by this thread (rows before the first failure)

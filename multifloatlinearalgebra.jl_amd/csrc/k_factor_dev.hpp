@@ -132,7 +132,8 @@ __device__ void wg_sym_swap(const DV &A, int64_t n, int64_t f, int64_t s, int64_
 // E.entry(i1, i2) (called by every thread). E.prep(col) (called by every
 // thread) readies E.centry(r, col) = entry(r, col) for r > col; E.prow(row)
 // readies E.rentry(row, c) = entry(row, c) for c < row; E.keep(r, v) sees
-// each column-pass value. With E::kFuseDiag, E.centry_diag(r, k, v, d) gives
+// each column-pass value and E.keep_row(r, v) each centry(r, imax) of the
+// row-max pass. With E::kFuseDiag, E.centry_diag(r, k, v, d) gives
 // centry(r, k) and entry(k, k) together and E.diag_entry(k) gives entry(k, k)
 // at the start of a step to any subset of threads. The lazy panel forms the
 // factors those passes share once; the right-looking body reads A.
@@ -188,7 +189,11 @@ __device__ BKChoice wg_select_bk(int64_t n, int64_t k, const mf::V<N> &alpha, E 
     e.prow(imax);
     for (int64_t c = k + threadIdx.x; c < imax; c += blockDim.x) rm = mf::vmax(rm, mf::vabs(e.rentry(imax, c)));
     e.prep(imax);
-    for (int64_t r = imax + 1 + threadIdx.x; r < n; r += blockDim.x) rm = mf::vmax(rm, mf::vabs(e.centry(r, imax)));
+    for (int64_t r = imax + 1 + threadIdx.x; r < n; r += blockDim.x) {
+        const mf::V<N> v = e.centry(r, imax);
+        e.keep_row(r, v);
+        rm = mf::vmax(rm, mf::vabs(v));
+    }
     const mf::V<N> rowmax = wg_max<N>(rm, sv_buf);
     if (mf::ge(absakk, mf::mul(mf::mul(alpha, colmax), mf::div(colmax, rowmax)))) return BKChoice{1, k};
     if (mf::ge(mf::vabs(e.entry(imax, imax)), mf::mul(alpha, rowmax))) return BKChoice{1, imax};
@@ -203,6 +208,7 @@ struct StoredEntries {
     __device__ mf::V<N> entry(int64_t i, int64_t j) const { return ld<N>(A, i, j); }
     __device__ void prep(int64_t) const {}
     __device__ void keep(int64_t, const mf::V<N> &) const {}
+    __device__ void keep_row(int64_t, const mf::V<N> &) const {}
     __device__ mf::V<N> centry(int64_t r, int64_t c) const { return ld<N>(A, r, c); }
     static constexpr bool kFuseDiag = false;
     __device__ void prow(int64_t) const {}
