@@ -49,6 +49,10 @@ extern "C" {
 #define MFLA_ERR_FAILED_FACTOR (-5)
 #define MFLA_ERR_WORKSPACE (-6)
 #define MFLA_ERR_BOUNDS (-7)      /* BoundsError (a reduction range outside the panel) */
+/* Positive: the call completed and its outputs are valid, but a stated goal
+ * was not met (mfla_refine_solve: the backward-error estimate of the best
+ * iterate, which X holds, is above tol). */
+#define MFLA_NOT_CONVERGED 1
 
 /* side / uplo / trans / diag symbols (MFLA's :left/:right, :lower/:upper,
  * :N/:T, :nonunit/:unit). MFLA_GENERAL is residual!'s uplo=:general. */
@@ -302,6 +306,11 @@ int mfla_syrk_packed(mfla_handle h, int limbs, mfla_matrix packed, mfla_matrix P
  * mfla_refine_workspace_size() doubles. history (host, max_iter entries) gets
  * the estimate after each correction; stage_ms (host, 5 entries, may be NULL)
  * milliseconds spent in factor, solve, update, residual and check.
+ * The loop also stops when an estimate is more than half the previous one;
+ * a correction that made the estimate worse is taken back, so X is always
+ * the best iterate (the smallest entry of history). Returns MFLA_OK when
+ * that iterate's estimate is <= tol, else MFLA_NOT_CONVERGED (X still valid:
+ * e.g. kappa(A) * u_64 >= 1, or tol below the Float64xN residual floor).
  * *info is the binary64 factorization's status (0, -1 non-finite input, k > 0
  * failing pivot); a failed factorization returns MFLA_ERR_FAILED_FACTOR. */
 #define MFLA_REFINE_CHOLESKY 0

@@ -1512,7 +1512,7 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
 
     // ---- refinement loop
     const DV Rhi{w.R, 1, n, 0};
-    double prev = 0.0;
+    double prev = 0.0, best = INFINITY;
     int it = 0;
     while (it < max_iter) {
         // D = F64 \ hi(R)
@@ -1590,12 +1590,25 @@ int mfla_refine_solve(mfla_handle h, int N, int kind_flags, int uplo, mfla_matri
         }
         history[it] = be;
         ++it;
-        if (!std::isfinite(be) || be <= tol || (it > 1 && be > 0.5 * prev)) break;
+        if (!std::isfinite(be)) { best = be; break; }
+        if (it > 1 && be > prev) {
+            // the last correction made the estimate worse: take it back
+            // (X += -D, exact up to one Float64xN rounding), so X is the
+            // best iterate, the one the previous estimate belongs to
+            const double minus_one = -1.0;
+            HIPCHK(mfk::scale(1, s, n, nrhs, D, &minus_one, nullptr));
+            HIPCHK(mfk::add_mixed(1, N, s, n, nrhs, dvn(X), D));
+            HIPCHK(hipStreamSynchronize(s));
+            best = prev;
+            break;
+        }
+        best = be;
+        if (be <= tol || (it > 1 && be > 0.5 * prev)) break;
         prev = be;
     }
     *iterations = it;
     clk.finish(stage_ms);
-    return MFLA_OK;
+    return best <= tol ? MFLA_OK : MFLA_NOT_CONVERGED;
 }
 
 }  // extern "C"

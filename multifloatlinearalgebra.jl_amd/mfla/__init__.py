@@ -5,7 +5,7 @@ operation runs in libmfla_hip.so on the GPU (see include/mfla_hip.h).
 """
 from ._lib import LIB_PATH, load  # noqa: F401
 from .api import (  # noqa: F401
-    ArgumentError, DimensionMismatch, add_mixed_, DomainError, KernelConfig, MFCholesky, MFLDLT, MFLU, MFWorkspace, PosDefException,
+    ArgumentError, ConvergenceError, DimensionMismatch, add_mixed_, DomainError, KernelConfig, MFCholesky, MFLDLT, MFLU, MFWorkspace, PosDefException,
     SingularException, arith, cholesky_, ensure_workspace_capacity_, from_interleaved, gemm_, gemmt_, gemv_, host_arith, ldiv_, ldlt_,
     lu_, normwise_backward_error, refinement_correction_, residual_, residual_mixed_, solve, symv_, synchronize, syrk_, to_interleaved,
     trsm_, trsv_, workspace_capacity, trmm_, mfdot, syrk_packed_, capabilities, BoundsError, dgemm_, dpotrf_inverse_, dgetrf_,

@@ -20,6 +20,7 @@ MFLA_ERR_HIP = -4
 MFLA_ERR_FAILED_FACTOR = -5
 MFLA_ERR_WORKSPACE = -6
 MFLA_ERR_BOUNDS = -7
+MFLA_NOT_CONVERGED = 1
 
 
 class MFMatrix(ctypes.Structure):

@@ -52,6 +52,15 @@ class SingularException(Exception):
         self.info = info
 
 
+class ConvergenceError(Exception):
+    """Iterative refinement stopped above its tolerance. ``X`` is the best
+    iterate (still a valid approximation) and ``history`` the estimates."""
+
+    def __init__(self, msg, X=None, history=None):
+        super().__init__(msg)
+        self.X, self.history = X, history
+
+
 def _raise(rc: int):
     if rc == L.MFLA_OK:
         return

@@ -26,29 +26,21 @@ import torch
 from . import api as M
 
 
-def _eye(N, n):
-    Q = torch.zeros((N, n, n), dtype=torch.float64, device="cuda")
-    Q[0].diagonal().fill_(1.0)
-    return Q
+def haar_orthogonal(N, n, rng):
+    """A Haar-random orthogonal Q: the binary64 Householder QR of a seeded
+    n x n N(0, 1) matrix (LAPACK dgeqrf / dorgqr through numpy: Q is the
+    product of n - 1 non-trivial reflectors), each column signed so that R's
+    diagonal is positive. Returned as Float64xN with zero lower limbs, i.e.
+    the binary64 values exactly (orthogonal to ~1e-15)."""
+    G = rng.standard_normal((n, n))
+    Q, Rr = np.linalg.qr(G)
+    Q *= np.where(np.diagonal(Rr) < 0, -1.0, 1.0)[None, :]
+    out = torch.zeros((N, n, n), dtype=torch.float64, device="cuda")
+    out[0] = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()     # planar [col, row]
+    return out
 
 
-def householder_orthogonal(N, n, reflectors, rng):
-    """Q = H_1 ... H_r, H = I - tau v v^T with v ~ N(0, 1) (FP64 values) and
-    tau = 2 / (v^T v) in Float64xN, formed with gemv!/gemm! in Float64xN."""
-    Q = _eye(N, n)
-    for _ in range(reflectors):
-        v = torch.zeros((N, 1, n), dtype=torch.float64, device="cuda")
-        v[0, 0] = torch.from_numpy(rng.standard_normal(n)).cuda()
-        vt = v.reshape(N, n, 1).contiguous()                       # 1 x n
-        vv = torch.zeros((N, 1, 1), dtype=torch.float64, device="cuda")
-        M.gemm_(vv, vt, v)
-        two = np.zeros((N, 1))
-        two[0, 0] = 2.0
-        tau = M.host_arith("div", two, vv.reshape(N, 1).cpu().numpy())[:, 0]
-        w = torch.zeros((N, 1, n), dtype=torch.float64, device="cuda")
-        M.gemm_(w, Q, v)
-        M.gemm_(Q, w, vt, -tau, 1.0)
-    return Q
+REFLECTORS = "n - 1 (binary64 Householder QR of an n x n Gaussian)"
 
 
 def _scale_columns(Q, sigma):
@@ -66,10 +58,11 @@ def _symmetrize_lower(A):
     return torch.where(lower[None], A, A.transpose(1, 2)).contiguous()
 
 
-def spd_kappa(n, kappa=1e8, reflectors=4, seed=20260812, N=4):
-    """A = Q diag(sigma) Q^T, sigma log-spaced over [1, kappa], exactly symmetric."""
+def spd_kappa(n, kappa=1e8, seed=20260812, N=4):
+    """A = Q diag(sigma) Q^T, sigma log-spaced over [1, kappa], Q Haar-random
+    (haar_orthogonal), formed in Float64xN and made exactly symmetric."""
     rng = np.random.default_rng(seed)
-    Q = householder_orthogonal(N, n, reflectors, rng)
+    Q = haar_orthogonal(N, n, rng)
     sigma = 10.0 ** (np.log10(kappa) * np.arange(n) / max(n - 1, 1))
     W = _scale_columns(Q, sigma)
     A = torch.zeros_like(Q)
@@ -77,11 +70,11 @@ def spd_kappa(n, kappa=1e8, reflectors=4, seed=20260812, N=4):
     return _symmetrize_lower(A)
 
 
-def general_kappa(n, kappa=1e8, reflectors=4, seed=20260813, N=4):
-    """A = U diag(sigma) V^T with independent Householder U, V."""
+def general_kappa(n, kappa=1e8, seed=20260813, N=4):
+    """A = U diag(sigma) V^T with independent Haar-random U, V."""
     rng = np.random.default_rng(seed)
-    U = householder_orthogonal(N, n, reflectors, rng)
-    V = householder_orthogonal(N, n, reflectors, rng)
+    U = haar_orthogonal(N, n, rng)
+    V = haar_orthogonal(N, n, rng)
     sigma = 10.0 ** (np.log10(kappa) * np.arange(n) / max(n - 1, 1))
     W = _scale_columns(U, sigma)
     A = torch.zeros_like(U)
@@ -132,7 +125,7 @@ def _refine_workspace(N, kind, n, nrhs, flag=0):
     return w, need.value
 
 
-def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general", stages=False, residual="auto"):
+def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general", stages=False, residual="auto", check=True):
     """Solve A X = B (A, B Float64xN on the device, N = 2..4) with a binary64
     factor and Float64xN refinement, in the library's native driver
     (mfla_refine_solve). The default tolerance 2^-(52N+8) stops at the
@@ -143,7 +136,10 @@ def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general"
     residual="limbs" sums R -= A*D in as many limbs as |D|/|X| requires;
     residual="ozaki" uses the Ozaki-split INT8 matrix-core product (exact
     digit products recombined in Float64xN; n <= 8192); "auto" picks ozaki
-    when n <= 8192."""
+    when n <= 8192. X is the best iterate; when its estimate is above tol
+    (kappa(A) * u_64 >= 1, or tol below the Float64xN floor) check=True
+    raises ConvergenceError (carrying X and the history), check=False
+    returns it."""
     import ctypes
     from . import _lib as L
     N = M._limbs(A, B)
@@ -164,8 +160,12 @@ def refine_solve(A, B, kind="cholesky", *, tol=None, max_iter=30, uplo="general"
     rc = L.load().mfla_refine_solve(M._handle(), N, _REFINE_KIND[kind] | flag, u, M._mat(A), M._mat(B), M._mat(X),
                                     ctypes.c_void_p(work.data_ptr()), length, float(tol), int(max_iter), ctypes.byref(info),
                                     ctypes.byref(iters), M._sp(hist), M._sp(st) if stages else None)
-    M._raise(rc)
     h = [float(v) for v in hist[: iters.value]]
+    if rc == L.MFLA_NOT_CONVERGED:
+        if check:
+            raise M.ConvergenceError(f"refinement stopped at backward error {min(h):.3e} > tol {tol:.3e}", X, h)
+    else:
+        M._raise(rc)
     if stages:
         return X, h, {k: round(float(v), 3) for k, v in zip(_STAGES, st)}
     return X, h
@@ -217,9 +217,12 @@ def plain_solve(A, B, kind="cholesky", *, block=16):
     return M.solve(F, B, config=M.KernelConfig(trsm_order="blocked"))
 
 
-def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812, residual="auto"):
+def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812, residual="auto", keep_cols=None):
     """Times the refined and the plain Float64xN solve on the same matrix and
-    right-hand sides; reports times and Float64xN backward errors."""
+    right-hand sides; reports times and the Float64xN-evaluated backward
+    errors. keep_cols: right-hand-side columns whose solutions (both
+    solves), right-hand sides and the matrix are returned as host arrays
+    under "_check", for an exact evaluation of the residual by the caller."""
     A = spd_kappa(n, kappa, N=N, seed=seed) if kind == "cholesky" else general_kappa(n, kappa, N=N, seed=seed)
     rng = np.random.default_rng(seed + 1)
     Bh = np.zeros((N, nrhs, n))
@@ -227,7 +230,9 @@ def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812
     for l in range(1, N):
         Bh[l] = Bh[l - 1] * 2.0**-53 * rng.uniform(-1, 1, (nrhs, n))
     B = torch.from_numpy(Bh).cuda()
-    out = {"n": n, "nrhs": nrhs, "kind": kind, "kappa": kappa, "limbs": N, "residual": residual}
+    out = {"n": n, "nrhs": nrhs, "kind": kind, "kappa": kappa, "limbs": N, "residual": residual,
+           "orthogonal_factors": "Haar-random, " + REFLECTORS}
+    keep = {}
     for name, fn in (("refined", lambda: refine_solve(A, B, kind, residual=residual)), ("plain", lambda: (plain_solve(A, B, kind), None))):
         fn()                                   # warm-up: code-object loading, workspace allocation
         torch.cuda.synchronize()
@@ -239,6 +244,8 @@ def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812
         M.residual_(R, A, X, B)
         be = _max_be(M.normwise_backward_error(A, X, B, R))
         out[name] = {"seconds": round(dt, 4), "backward_error": be}
+        if keep_cols is not None:
+            keep[name] = X[:, keep_cols, :].cpu().numpy()
         if hist is not None:
             out[name]["iterations"] = len(hist)
             out[name]["history"] = [float(f"{h:.3e}") for h in hist]
@@ -253,4 +260,9 @@ def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812
     M.solve(F, B, config=M.KernelConfig(trsm_order="blocked"))
     P.mark("solve")
     out["plain"]["stage_ms"] = P.done()
+    if keep_cols is not None:
+        keep["A"] = A.cpu().numpy()
+        keep["B"] = np.ascontiguousarray(Bh[:, keep_cols, :])
+        keep["cols"] = list(keep_cols)
+        out["_check"] = keep
     return out

@@ -9,8 +9,8 @@ network (2, 30, 111, 239 for N = 1..4), and the FP64 instruction issue rate
 39.3 T lane-instructions/s. Level-2 operations also report HBM GB/s against
 8 TB/s with compulsory bytes 8N(mn + m + n) per right-hand side.
 
-Inputs are seeded synthetic Float64xN matrices: hi limb N(0, 1), lower limbs
-hi * 2^-53 * U(-1, 1) (not renormalised; timing does not depend on it).
+Inputs are seeded synthetic Float64xN matrices: hi limb N(0, 1), limb l =
+limb(l-1) * 2^-53 * U(-1, 1), renormalised by two fast_two_sum sweeps.
 SPD matrices are R R^T + n I formed with gemm!; the triangle of trsm! is the
 Cholesky factor of such a matrix.
 """
@@ -26,7 +26,7 @@ from . import api as M
 
 F_N = {1: 2, 2: 30, 3: 111, 4: 239}
 INSTR_N = {1: 2, 2: 27, 3: 108, 4: 229}
-INSTR_X3_FUSED = 99
+F_X3_FUSED, INSTR_X3_FUSED = 102, 99      # mulacc_x3: MFLA's default x3 gemm! route
 PEAK_TF = 78.6
 PEAK_INSTR = 39.3e12
 PEAK_HBM = 8.0e12
@@ -39,6 +39,11 @@ def rand_dev(N, rows, cols, seed=0):
     for l in range(1, N):
         u = torch.rand((cols, rows), generator=g, dtype=torch.float64, device="cuda") * 2 - 1
         x[l] = x[l - 1] * 2.0**-53 * u
+    for _ in range(2):
+        for l in range(N - 1):
+            sm = x[l] + x[l + 1]
+            e = x[l + 1] - (sm - x[l])
+            x[l], x[l + 1] = sm, e
     return x
 
 
@@ -63,10 +68,11 @@ def _time(fn, reps=1, warm=1):
     return a.elapsed_time(b) / reps / 1e3
 
 
-def _compute(name, N, macs, t, instr=None, **extra):
+def _compute(name, N, macs, t, instr=None, flops=None, **extra):
     instr = instr or INSTR_N[N]
+    flops = flops or F_N[N]
     r = {"case": name, "limbs": N, "ms": round(t * 1e3, 3), "macs": macs,
-         "tflops": round(macs * F_N[N] / t / 1e12, 3), "frac": round(macs * F_N[N] / t / 1e12 / PEAK_TF, 4),
+         "tflops": round(macs * flops / t / 1e12, 3), "frac": round(macs * flops / t / 1e12 / PEAK_TF, 4),
          "issue_frac": round(macs * instr / t / PEAK_INSTR, 4)}
     r.update(extra)
     return r
@@ -93,8 +99,9 @@ def case_gemm(N, m, k, n, reps=2, strategy="auto"):
         return {"case": f"gemm {m}x{k}x{n} (ozaki)", "limbs": N, "ms": round(t * 1e3, 3), "macs": m * k * n,
                 "x_equiv_tflops": round(m * k * n * F_N[N] / t / 1e12, 3), "int8_tops": round(ops / t / 1e12, 1),
                 "int8_frac": round(ops / t / 5.0e15, 4)}
-    instr = INSTR_X3_FUSED if N == 3 else None
-    return _compute(f"gemm {m}x{k}x{n}", N, m * k * n, t, instr)
+    fused = N == 3 and strategy in ("auto", "fused")
+    return _compute(f"gemm {m}x{k}x{n}", N, m * k * n, t, INSTR_X3_FUSED if fused else None, F_X3_FUSED if fused else None,
+                    matrix_equiv_gflops=round(2.0 * m * k * n / t / 1e9, 2))
 
 
 def case_syrk(N, n, k, reps=2):
@@ -204,24 +211,45 @@ def case_batched(count=10000, nmin=16, nmax=256, N=3, nrhs=4, seed=3):
 
 
 def case_autodiff(batch=32, n=1024, nrhs=32, N=4):
-    """Config 4: x = S^{-1} B and logdet(S) forward + pullbacks over a batch."""
+    """Config 4: x = S^{-1} B and logdet(S), forward and pullbacks, over a
+    batch of 32 problems (after one warm-up problem); the Cholesky rrule's
+    pullback is timed separately over the same factors.
+    MACs per problem: cholesky! n^3/6, solve n^2 nrhs, solve pullback 2 n^2
+    nrhs (S^{-1} Xbar and Bbar X^T), logdet pullback n^3 (S^{-1} from two
+    triangular solves with n right-hand sides); Cholesky pullback 2 n^3
+    (L^T Lbar as a full product, then two triangular solves)."""
     from . import autodiff as AD
     Ss = [spd_dev(N, n, 100 + i) for i in range(batch)]
     Bs = [rand_dev(N, n, nrhs, 200 + i) for i in range(batch)]
     W = rand_dev(N, n, nrhs, 300)
+    Lbar = rand_dev(N, n, n, 301)
     one = np.zeros(N)
     one[0] = 1.0
+    factors = []
+
+    def problem(S, B):
+        F = M.cholesky_(S.clone())
+        X = M.solve(F, B)
+        AD.logdet(F)
+        AD.solve_adjoint(F, X, W)
+        AD.logdet_adjoint(F, one)
+        return F
+
+    problem(Ss[0], Bs[0])                      # warm-up: code-object loading
 
     def run():
+        factors.clear()
         for S, B in zip(Ss, Bs):
-            F = M.cholesky_(S.clone())
-            X = M.solve(F, B)
-            AD.logdet(F)
-            AD.solve_adjoint(F, X, W)
-            AD.logdet_adjoint(F, one)
+            factors.append(problem(S, B))
     t = _time(run, 1, warm=0)
-    return {"case": f"autodiff batch={batch} n={n} nrhs={nrhs}", "limbs": N, "ms": round(t * 1e3, 1),
-            "ms_per_problem": round(t * 1e3 / batch, 2)}
+    macs = batch * (n**3 // 6 + n * n * nrhs + 2 * n * n * nrhs + n**3)
+    r = _compute(f"autodiff batch={batch} n={n} nrhs={nrhs} (cholesky!, solve, logdet + both pullbacks)", N, macs, t,
+                 ms_per_problem=round(t * 1e3 / batch, 2))
+    AD.cholesky_adjoint(factors[0], Lbar)      # warm-up
+    tc = _time(lambda: [AD.cholesky_adjoint(F, Lbar) for F in factors], 1, warm=0)
+    r["cholesky_rrule"] = {"ms_per_problem": round(tc * 1e3 / batch, 2),
+                           "frac": round(batch * 2 * n**3 * F_N[N] / tc / 1e12 / PEAK_TF, 4)}
+    return r
 
 
 def run(which=("blas3", "factor", "blas2", "refine", "batched", "autodiff"), out=None):

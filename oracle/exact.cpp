@@ -3,7 +3,9 @@
 // binary64 values and exact products of binary64 pairs with no rounding, so
 // sums of products of multi-double limbs are evaluated exactly. It stands in
 // for MFLA's BigFloat-512 checks (test/runtests.jl:10-23) at sizes where
-// mpmath is too slow; it is itself checked against mpmath in tests/.
+// mpmath is too slow. tests/test_oracle_exact.py checks it against exact
+// rational arithmetic (fractions.Fraction): wide exponents, full
+// cancellation, both signs, transposed addressing, range edges.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -35,19 +37,29 @@ struct Acc {
             else { c[idx + i] -= lo; c[idx + i + 1] -= hi; }
         }
     }
+    // |x| = mant * 2^exp from the binary64 fields (subnormals: exponent
+    // field 0 means 2^-1074 units); false for zero, Inf and NaN.
+    static bool split(double x, uint64_t &mant, int &exp) {
+        uint64_t b;
+        std::memcpy(&b, &x, sizeof(b));
+        const int ef = (int)((b >> 52) & 0x7ff);
+        mant = b & ((1ull << 52) - 1);
+        if (ef == 0x7ff) return false;
+        if (ef == 0) { exp = -1074; return mant != 0; }
+        mant |= 1ull << 52;
+        exp = ef - 1075;
+        return true;
+    }
     void add_double(double x) {
-        if (x == 0.0 || !std::isfinite(x)) return;
-        int e; double f = std::frexp(std::fabs(x), &e);
-        uint64_t m = (uint64_t)std::ldexp(f, 53);
-        add_mag((unsigned __int128)m, e - 53, x < 0 ? -1 : 1);
+        uint64_t m; int e;
+        if (!split(x, m, e)) return;
+        add_mag((unsigned __int128)m, e, std::signbit(x) ? -1 : 1);
     }
     void add_prod(double a, double b, int sign) {
-        if (a == 0.0 || b == 0.0 || !std::isfinite(a) || !std::isfinite(b)) return;
-        int ea, eb;
-        double fa = std::frexp(std::fabs(a), &ea), fb = std::frexp(std::fabs(b), &eb);
-        uint64_t ma = (uint64_t)std::ldexp(fa, 53), mb = (uint64_t)std::ldexp(fb, 53);
-        int s = ((a < 0) != (b < 0)) ? -1 : 1;
-        add_mag((unsigned __int128)ma * mb, ea - 53 + eb - 53, s * sign);
+        uint64_t ma, mb; int ea, eb;
+        if (!split(a, ma, ea) || !split(b, mb, eb)) return;
+        const int s = (std::signbit(a) != std::signbit(b)) ? -1 : 1;
+        add_mag((unsigned __int128)ma * mb, ea + eb, s * sign);
     }
     // normalise carries; returns sign of the value (-1, 0, 1) and magnitude as double
     double to_double() {
@@ -84,13 +96,22 @@ extern "C" {
 // For each sampled (row, col): value = sc*C[row,col] + sab * sum_k A[row,k]*B[k,col]
 // evaluated exactly over all limb products, returned rounded to binary64,
 // plus scale = |C| + sum_k |A||B| (binary64, approximate).
-// Planar layout: element (i, j), limb l at p[l*ls + j*ld + i].
+// Planar layout: element (i, j), limb l at p[l*ls + j*ld + i]. Samples are
+// independent: OpenMP threads take them in chunks, one accumulator each.
 int orc_exact_axpb(int N, int64_t k, const double *A, int64_t lda, int64_t lsa, int atrans, const double *B, int64_t ldb,
                    int64_t lsb, const double *C, int64_t ldc, int64_t lsc, int sab, int sc, const int64_t *rows,
                    const int64_t *cols, int64_t count, double *out_val, double *out_scale) {
+    int rc = 0;
+#pragma omp parallel
+    {
     Acc *acc = (Acc *)std::malloc(sizeof(Acc));
-    if (!acc) return -1;
+    if (!acc) {
+#pragma omp atomic write
+        rc = -1;
+    }
+#pragma omp for schedule(dynamic, 16)
     for (int64_t t = 0; t < count; ++t) {
+        if (!acc) continue;
         int64_t i = rows[t], j = cols[t];
         acc->clear();
         double scale = 0.0;
@@ -113,7 +134,8 @@ int orc_exact_axpb(int N, int64_t k, const double *A, int64_t lda, int64_t lsa, 
         out_scale[t] = scale;
     }
     std::free(acc);
-    return 0;
+    }
+    return rc;
 }
 
 // Exact value of sum_i x_i (count binary64 values) minus sum_j y_j, rounded.

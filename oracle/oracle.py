@@ -29,11 +29,19 @@ Parity pinning per function (what checks the restatement itself):
   reflector convention: factors, tau and pivot order agree to float64
   accuracy on well-separated norms) and by exact orthogonality / QR = AP
   residuals; blocked and unblocked routes agree on ranks.
-* gemm / lower_update / trsm / gemv / symv / potrf / getrf / sytrf / solves /
-  residuals / backward error -- pinned by the reference's known-answer tests
-  (test/adversarial.jl, test/runtests.jl) re-created in tests/, by the exact
-  long accumulator (``exact_axpb``, itself checked against mpmath) and by
-  closed forms (exact solves of diagonal systems, Wilkinson growth 2^(n-1)...).
+* exact_axpb / exact_sum_diff (the stand-in for BigFloat-512) -- against
+  fractions.Fraction sums of the same limb products in
+  tests/test_oracle_exact.py (exponents over 2^-400..2^400 and the binary64
+  range edges, full cancellation C = fl(AB), both signs, atrans).
+* gemm / syrk / gemmt / trsm (all 16 variants) -- exact results on integer
+  data (index maps, alpha / beta, beta == 0 not reading C, untouched upper
+  triangle, unit diagonal) and exact-accumulator bounds on multi-limb data
+  (tests/test_oracle_blas.py).
+* potrf / getrf / sytrf / solves / gemv / symv / residuals / backward error --
+  pinned by the reference's known-answer tests (test/adversarial.jl,
+  test/runtests.jl) re-created in tests/, by the exact long accumulator and
+  by closed forms (exact solves of diagonal systems, Wilkinson growth
+  2^(n-1)...).
 """
 from __future__ import annotations
 

@@ -37,15 +37,23 @@ def make_batch(oracle, rng, N, count, nmin, nmax, nrhs=4):
     return systems
 
 
-@pytest.mark.parametrize("N", [3, 4])
-def test_batched_bitwise_and_exact(oracle, N):
-    from mfla.batched import Batch
+def weak_kkt(rng, N, n, nrhs=4):
+    """KKT [H A'; A -1e-8 I] whose H is small against A (H = 1e-3 (G + G'),
+    G ~ N(0, 1)): Bunch-Kaufman pivots with 2x2 blocks and interchanges that
+    reach the constraint rows, i.e. the last third of the matrix."""
+    m = (2 * n) // 3
+    p = n - m
+    H = rand_mf(rng, N, m, m) * 1e-3
+    K = np.zeros((N, n, n))
+    K[:, :m, :m] = H + np.transpose(H, (0, 2, 1))
+    A = rand_mf(rng, N, p, m)
+    K[:, :m, m:] = A
+    K[:, m:, :m] = np.transpose(A, (0, 2, 1))
+    K[0, np.arange(m, n), np.arange(m, n)] = -1e-8
+    return (1, K, rand_mf(rng, N, n, nrhs))
 
-    rng = np.random.default_rng(400 + N)
-    systems = make_batch(oracle, rng, N, 24, 3, 90)
-    b = Batch(systems, N)
-    b.solve(exact_backward=True)
-    res = b.results()
+
+def check_against_oracle(oracle, systems, res):
     for (kind, A, B), r in zip(systems, res):
         n = A.shape[1]
         if kind == 0:
@@ -63,6 +71,40 @@ def test_batched_bitwise_and_exact(oracle, N):
             assert_bitwise(r["A"], F, f"batched ldlt n={n}")
             assert_bitwise(r["original_maximum"], om, "original maximum")
             assert_bitwise(r["X"], oracle.sytrs(F, dsub, piv, blk, B.copy()), f"batched sytrs n={n}")
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_batched_bitwise_and_exact(oracle, N):
+    from mfla.batched import Batch
+
+    rng = np.random.default_rng(400 + N)
+    systems = make_batch(oracle, rng, N, 24, 3, 90)
+    b = Batch(systems, N)
+    b.solve(exact_backward=True)
+    check_against_oracle(oracle, systems, b.results())
+
+
+def test_batched_bitwise_orders_91_to_256(oracle):
+    """The configuration's full order range: n in [91, 256] (n = 256
+    included), so the workgroup's 256 threads all own rows; the weak-H KKT
+    systems take interchanges with rows past 128, i.e. candidates found in
+    waves 2-3 of the argmax."""
+    from mfla.batched import Batch
+
+    N = 3
+    rng = np.random.default_rng(4096)
+    systems = make_batch(oracle, rng, N, 8, 91, 255)
+    systems += make_batch(oracle, rng, N, 2, 256, 256)
+    systems += [weak_kkt(rng, N, n) for n in (256, 227, 160)]
+    b = Batch(systems, N)
+    b.solve()
+    res = b.results()
+    check_against_oracle(oracle, systems, res)
+    for (kind, A, _), r in zip(systems[-3:], res[-3:]):
+        n = A.shape[1]
+        moved = np.abs(r["pivots"]) != np.arange(1, n + 1)
+        assert moved.any() and (np.abs(r["pivots"][moved]) > 128).any(), (n, r["pivots"][:20])
+        assert (r["blocks"] == 2).any()
 
 
 def test_batched_systolic_order_and_graph(oracle):

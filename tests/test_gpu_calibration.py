@@ -9,7 +9,7 @@ import mfla as M
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("N", [2, 3, 4])
 def test_calibrate_gemm(N):
     prof, meas = M.calibrate_gemm(N, sizes=(192, 320), samples=1)
     assert prof.source == "calibrated" and prof.limbs == N

@@ -34,8 +34,9 @@ def test_refined_solve_reaches_x4_backward_error(oracle, kind):
     B = torch.from_numpy(Bh).cuda()
     X, hist = R.refine_solve(A, B, kind)
     assert len(hist) >= 3 and hist[-1] <= 1e-60, hist
-    # the convergence is geometric at ~kappa * u_64 per step
-    assert hist[1] < 1e-6 * hist[0]
+    # the convergence is geometric at ~c kappa * u_64 per step (dense Haar
+    # factors: kappa * u_64 ~ 1e-8 times a modest growth constant)
+    assert hist[1] < 1e-4 * hist[0]
     Ah, Xh = A.cpu().numpy(), X.cpu().numpy()
     be = exact_backward_error(oracle, Ah, Xh, Bh)
     assert be.max() <= 1e-60, be
@@ -217,6 +218,30 @@ def test_native_refinement_errors():
         R.refine_solve(A, B[:, :, :10].contiguous(), "cholesky")
     with pytest.raises(M.ArgumentError):
         R.refine_solve(A[:1].contiguous(), B[:1].contiguous(), "cholesky")
+
+
+def test_refinement_without_convergence_keeps_the_best_iterate():
+    """kappa = 1e17 > 1 / u_64: the binary64 LU factor still exists but the
+    corrections stop contracting. check=True raises ConvergenceError, and with
+    check=False the returned X is the best iterate (a correction that made
+    the estimate worse is taken back)."""
+    from mfla import api as M
+    from mfla import refine as R
+
+    n, nrhs = 256, 2
+    A = R.general_kappa(n, 1e17, N=4)
+    rng = np.random.default_rng(17)
+    Bh = np.zeros((4, nrhs, n))
+    Bh[0] = rng.standard_normal((nrhs, n))
+    B = torch.from_numpy(Bh).cuda()
+    with pytest.raises(M.ConvergenceError) as ei:
+        R.refine_solve(A, B, "lu", residual="limbs")
+    assert ei.value.X is not None and min(ei.value.history) > 2.0 ** -216
+    X, hist = R.refine_solve(A, B, "lu", residual="limbs", check=False)
+    Rr = torch.zeros_like(B)
+    M.residual_(Rr, A, X, B)
+    be = float(np.max(np.abs(M.normwise_backward_error(A, X, B, Rr)[0])))
+    assert be <= 4 * min(hist), (be, hist)
 
 
 @pytest.mark.parametrize("bad", [70, 200, 299])
