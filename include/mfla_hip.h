@@ -111,6 +111,7 @@ typedef struct mfla_config {
     int64_t gemm_micro_columns;
     int32_t trsm_order;            /* MFLA_TRSM_ORDER_*; auto = exact for n <= 256 */
     int32_t gemm_tile;             /* -1 auto (16x16 for lower-triangle outputs), 0: 16x16, 1: 32x32, 2: 32x64, 3: 64x64 (1-2 limbs; else 32x64) output tiles; values outside 0..3 mean 16x16 */
+    int64_t gemm_packed_crossover; /* KernelConfig.gemm_packed_crossover (INT64_MAX by default): only gemm_plan reads it */
 } mfla_config;
 
 typedef struct mfla_handle_s *mfla_handle;
@@ -174,6 +175,62 @@ int mfla_gemmt(mfla_handle h, int limbs, mfla_matrix C, mfla_matrix L, mfla_matr
 /* trsm! (src/kernels/trsm.jl:224-302): op(A) X = alpha B or X op(A) = alpha B, in place */
 int mfla_trsm(mfla_handle h, int limbs, int side, int uplo, int trans, int diag, mfla_matrix B, mfla_matrix A, const double *alpha,
               const mfla_config *config);
+
+/* Route plans. Reason codes of GemmPlan / LDLTPlan (MFLA's symbols
+ * :forced_direct ... :auto_fused_direct, :forced_unblocked ...). */
+#define MFLA_PLAN_FORCED_DIRECT 1
+#define MFLA_PLAN_FORCED_PACKED 2
+#define MFLA_PLAN_FORCED_FUSED 3
+#define MFLA_PLAN_FORCED_OZAKI 4
+#define MFLA_PLAN_AUTO_OUTSIDE_CALIBRATED_SHAPE 5
+#define MFLA_PLAN_AUTO_REDUCTION_TOO_SMALL 6
+#define MFLA_PLAN_AUTO_BELOW_CROSSOVER 7
+#define MFLA_PLAN_AUTO_ABOVE_CROSSOVER 8
+#define MFLA_PLAN_AUTO_FUSED_DIRECT 9
+#define MFLA_PLAN_FORCED_UNBLOCKED 10
+#define MFLA_PLAN_FORCED_BLOCKED 11
+
+/* GemmPlan (src/types.jl:210-217) as gemm_plan(T, m, k, n, config)
+ * (src/kernels/gemm.jl:44-136) resolves it, plus the device launch gemm!
+ * makes for it. strategy is the resolved route (MFLA_GEMM_DIRECT, _PACKED,
+ * _FUSED, or _OZAKI when forced; never _AUTO); reason an MFLA_PLAN_* code.
+ * panel_columns, micro_columns, workers and packed_elements_per_worker are
+ * MFLA's CPU decomposition for config->thread_count (kept so plan-driven
+ * caller code sees MFLA's values); the device computes every route with the
+ * tile kernel (packed == direct per element: the standard network, ascending
+ * k). Device fields: tile (0..3) and its tile_rows x tile_cols outputs per
+ * workgroup; network 1 = mulacc_x3, 0 = the standard network; launches = 2
+ * when the x3 mulacc route has remainder_rows = m mod 4 rows on the standard
+ * network; workgroups over all launches. */
+typedef struct mfla_gemm_plan_t {
+    int32_t strategy;
+    int32_t reason;
+    int64_t panel_columns;
+    int64_t micro_columns;
+    int64_t workers;
+    int64_t packed_elements_per_worker;
+    int32_t tile;
+    int32_t tile_rows;
+    int32_t tile_cols;
+    int32_t network;
+    int64_t launches;
+    int64_t workgroups;
+    int64_t remainder_rows;
+} mfla_gemm_plan_t;
+int mfla_gemm_plan(int limbs, int64_t m, int64_t k, int64_t n, const mfla_config *config, mfla_gemm_plan_t *out);
+
+/* LDLTPlan (src/types.jl:219-223) as ldlt_plan(T, n, config)
+ * (src/factorizations/ldlt.jl:329-350) resolves it: strategy
+ * MFLA_LDLT_UNBLOCKED / _BLOCKED, reason MFLA_PLAN_FORCED_UNBLOCKED,
+ * _FORCED_BLOCKED, _AUTO_BELOW_CROSSOVER or _AUTO_ABOVE_CROSSOVER, and the
+ * panel width (16 / 12 / 8 for 1-2 / 3 / 4 limbs unless config->ldlt_block,
+ * src/factorizations/ldlt_tuning.jl). ldlt! (mfla_sytrf) follows it. */
+typedef struct mfla_ldlt_plan_t {
+    int32_t strategy;
+    int32_t reason;
+    int64_t block_size;
+} mfla_ldlt_plan_t;
+int mfla_ldlt_plan(int limbs, int64_t n, const mfla_config *config, mfla_ldlt_plan_t *out);
 
 /* ------------------------------------------------------------ BLAS 2 */
 /* trsv! (src/kernels/trsv.jl:19-73): x is an n x 1 matrix; same order as trsm! */

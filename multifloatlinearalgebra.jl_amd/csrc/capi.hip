@@ -369,6 +369,7 @@ void mfla_config_default(mfla_config *c) {
     c->gemm_micro_columns = 0;
     c->trsm_order = MFLA_TRSM_ORDER_AUTO;
     c->gemm_tile = -1;
+    c->gemm_packed_crossover = INT64_MAX;
 }
 
 int mfla_handle_create(mfla_handle *out) {
@@ -517,6 +518,82 @@ int mfla_to_interleaved(mfla_handle h, int limbs, double *dst, int64_t dst_ld, m
 }
 
 // ------------------------------------------------------------ BLAS 3
+int mfla_gemm_plan(int N, int64_t m, int64_t k, int64_t n, const mfla_config *config, mfla_gemm_plan_t *out) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (!out) return fail(MFLA_ERR_ARGUMENT, "null plan pointer");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (c.gemm_strategy < 0 || c.gemm_strategy > 4)
+        return fail(MFLA_ERR_ARGUMENT, "gemm_strategy must be :auto, :direct, :packed, :fused, or :ozaki");
+    // gemm_plan (gemm.jl:44-136): MFLA's CPU decomposition
+    mfla_gemm_plan_t p{};
+    const int64_t threads = c.thread_count;
+    const int64_t base = N <= 1 ? 48 : N == 2 ? 32 : N == 3 ? 24 : 16;                       // gemm.jl:1-7
+    const int64_t dpanel = threads >= 8 ? std::max<int64_t>(12, base * 3 / 4) : base;
+    const int64_t panel = std::max<int64_t>(c.gemm_panel_columns > 0 ? c.gemm_panel_columns : dpanel, 1);
+    const int64_t jobs = (std::max<int64_t>(n, 0) + panel - 1) / panel;
+    const int64_t workers = std::max<int64_t>(1, std::min<int64_t>(threads, std::max<int64_t>(jobs, 1)));  // types.jl:249-250
+    p.panel_columns = panel;
+    p.workers = workers;
+    if (c.gemm_strategy == MFLA_GEMM_FUSED) {
+        if (N != 3) return fail(MFLA_ERR_ARGUMENT, "gemm_strategy=:fused is only available for Float64x3");
+        p.strategy = MFLA_GEMM_FUSED; p.reason = MFLA_PLAN_FORCED_FUSED; p.micro_columns = 2;
+    } else {
+        const int64_t req = c.gemm_micro_columns > 0 ? c.gemm_micro_columns : (N <= 3 ? 4 : 2);   // gemm.jl:9-13
+        if (req != 1 && req != 2 && req != 4) return fail(MFLA_ERR_ARGUMENT, "gemm_micro_columns must be 1, 2, or 4");
+        const int64_t mc = std::min<int64_t>(req, panel);
+        p.micro_columns = mc >= 4 ? 4 : mc >= 2 ? 2 : 1;
+        if (c.gemm_strategy == MFLA_GEMM_DIRECT) {
+            p.strategy = MFLA_GEMM_DIRECT; p.reason = MFLA_PLAN_FORCED_DIRECT;
+        } else if (c.gemm_strategy == MFLA_GEMM_PACKED) {
+            p.strategy = MFLA_GEMM_PACKED; p.reason = MFLA_PLAN_FORCED_PACKED;
+            p.packed_elements_per_worker = std::max<int64_t>(k, 0) * panel;
+        } else if (c.gemm_strategy == MFLA_GEMM_OZAKI) {
+            p.strategy = MFLA_GEMM_OZAKI; p.reason = MFLA_PLAN_FORCED_OZAKI;
+        } else {
+            // volume-based crossover on near-square shapes (gemm.jl:110-127, :15-18)
+            const int64_t crossover = std::max<int64_t>(c.gemm_packed_crossover, 1);
+            const double work = (double)std::max<int64_t>(m, 0) * (double)std::max<int64_t>(k, 0) * (double)std::max<int64_t>(n, 0);
+            const int64_t min_red = std::max<int64_t>(32, crossover / 2);
+            const int64_t lo = std::min(m, std::min(k, n)), hi = std::max(m, std::max(k, n));
+            const bool near_square = lo > 0 && hi <= 4 * lo;
+            int reason;
+            if (!near_square) reason = MFLA_PLAN_AUTO_OUTSIDE_CALIBRATED_SHAPE;
+            else if (k < min_red) reason = MFLA_PLAN_AUTO_REDUCTION_TOO_SMALL;
+            else if (work < (double)crossover * (double)crossover * (double)crossover) reason = MFLA_PLAN_AUTO_BELOW_CROSSOVER;
+            else reason = MFLA_PLAN_AUTO_ABOVE_CROSSOVER;
+            const bool packed = reason == MFLA_PLAN_AUTO_ABOVE_CROSSOVER;
+            p.strategy = packed ? MFLA_GEMM_PACKED : (N == 3 ? MFLA_GEMM_FUSED : MFLA_GEMM_DIRECT);
+            p.reason = packed ? reason : (N == 3 ? MFLA_PLAN_AUTO_FUSED_DIRECT : reason);
+            p.packed_elements_per_worker = packed ? std::max<int64_t>(k, 0) * panel : 0;
+        }
+    }
+    // the device launch gemm! makes (gemm_launch): the x3 mulacc route splits
+    // off the m mod 4 remainder rows onto the standard network
+    const bool fused = N == 3 && p.strategy == MFLA_GEMM_FUSED;
+    const int64_t mm = std::max<int64_t>(m, 0), nn = std::max<int64_t>(n, 0);
+    const int64_t main_rows = fused ? (mm / 4) * 4 : mm;
+    int tr = 0, tc = 0;
+    p.network = fused ? 1 : 0;
+    if (p.strategy == MFLA_GEMM_OZAKI) {
+        p.tile = -1; p.launches = 0; p.workgroups = 0;
+    } else {
+        p.tile = mfk::tile_geometry(N, main_rows, nn, false, c.gemm_tile, &tr, &tc);
+        p.tile_rows = tr; p.tile_cols = tc;
+        p.launches = (main_rows > 0 && nn > 0) ? 1 : 0;
+        p.workgroups = mfk::cdiv(main_rows, tr) * mfk::cdiv(nn, tc);
+        p.remainder_rows = mm - main_rows;
+        if (p.remainder_rows > 0 && nn > 0) {
+            int rr = 0, rc = 0;
+            mfk::tile_geometry(N, p.remainder_rows, nn, false, c.gemm_tile, &rr, &rc);
+            p.launches += 1;
+            p.workgroups += mfk::cdiv(p.remainder_rows, rr) * mfk::cdiv(nn, rc);
+        }
+    }
+    *out = p;
+    return MFLA_OK;
+}
+
 int mfla_gemm(mfla_handle h, int N, mfla_matrix C, mfla_matrix A, mfla_matrix B, const double *alpha, const double *beta,
               const mfla_config *config) {
     mfla_config tmp;
@@ -803,6 +880,30 @@ int mfla_getrf(mfla_handle h, int N, mfla_matrix A, int64_t *ipiv, double *origm
     return MFLA_OK;
 }
 
+// ldlt_plan (ldlt.jl:329-350); panel widths 16 / 12 / 8 (ldlt_tuning.jl:82-88)
+static int ldlt_plan_core(int N, int64_t n, const mfla_config &c, mfla_ldlt_plan_t *p) {
+    if (c.ldlt_strategy < 0 || c.ldlt_strategy > 2) return fail(MFLA_ERR_ARGUMENT, "ldlt_strategy must be :auto, :unblocked, or :blocked");
+    const int64_t b = std::max<int64_t>(c.ldlt_block > 0 ? c.ldlt_block : (N <= 2 ? 16 : (N == 3 ? 12 : 8)), 1);
+    if (c.ldlt_strategy == MFLA_LDLT_UNBLOCKED) {
+        *p = {MFLA_LDLT_UNBLOCKED, MFLA_PLAN_FORCED_UNBLOCKED, 1};
+    } else if (c.ldlt_strategy == MFLA_LDLT_BLOCKED) {
+        *p = {MFLA_LDLT_BLOCKED, MFLA_PLAN_FORCED_BLOCKED, std::max<int64_t>(b, 2)};
+    } else if (b > 1 && n >= std::max<int64_t>(c.ldlt_blocked_crossover, 1)) {
+        *p = {MFLA_LDLT_BLOCKED, MFLA_PLAN_AUTO_ABOVE_CROSSOVER, b};
+    } else {
+        *p = {MFLA_LDLT_UNBLOCKED, MFLA_PLAN_AUTO_BELOW_CROSSOVER, 1};
+    }
+    return MFLA_OK;
+}
+
+int mfla_ldlt_plan(int N, int64_t n, const mfla_config *config, mfla_ldlt_plan_t *out) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    if (!out) return fail(MFLA_ERR_ARGUMENT, "null plan pointer");
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    return ldlt_plan_core(N, n, c, out);
+}
+
 int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *pivots, uint8_t *blocks, double *origmax, int64_t *info,
                const mfla_config *config, mfla_workspace ws) {
     mfla_config tmp;
@@ -812,14 +913,10 @@ int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *p
     if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
     if (check_desc(A, "A") || check_desc(dsub, "dsub")) return MFLA_ERR_ARGUMENT;
     if (dsub.rows != n || dsub.cols != 1) return fail(MFLA_ERR_DIMENSION, "ldlt! dsub length differs");
-    if (c.ldlt_strategy < 0 || c.ldlt_strategy > 2) return fail(MFLA_ERR_ARGUMENT, "ldlt_strategy must be :auto, :unblocked, or :blocked");
-    // ldlt_plan (ldlt.jl:329-350), block widths ldlt_tuning.jl:82-88
-    int64_t b = c.ldlt_block > 0 ? c.ldlt_block : (N <= 2 ? 16 : (N == 3 ? 12 : 8));
-    b = std::max<int64_t>(b, 1);
-    bool blocked;
-    if (c.ldlt_strategy == MFLA_LDLT_UNBLOCKED) blocked = false;
-    else if (c.ldlt_strategy == MFLA_LDLT_BLOCKED) { blocked = true; b = std::max<int64_t>(b, 2); }
-    else blocked = b > 1 && n >= std::max<int64_t>(c.ldlt_blocked_crossover, 1);
+    mfla_ldlt_plan_t plan;
+    if (ldlt_plan_core(N, n, c, &plan) != MFLA_OK) return MFLA_ERR_ARGUMENT;
+    const bool blocked = plan.strategy == MFLA_LDLT_BLOCKED;
+    const int64_t b = plan.block_size;
 
     std::lock_guard<std::mutex> g(h->lock);
     hipStream_t s = h->stream;

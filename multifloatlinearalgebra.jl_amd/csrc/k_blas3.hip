@@ -176,6 +176,16 @@ static int auto_tile(int N, int64_t rows, int64_t n, bool lower) {
     return (cdiv(rows, 32) * cdiv(n, 32) >= 512) ? 1 : 0;
 }
 
+int tile_geometry(int N, int64_t rows, int64_t n, bool lower, int requested, int *tile_rows, int *tile_cols) {
+    int t = requested >= 0 ? requested : auto_tile(N, rows, n, lower);
+    if (t < 0 || t > 3) t = 0;                 // pick(): values outside 0..3 run the 16x16 tile
+    if (t == 3 && N > 2) t = 2;                // 64x64 exists for 1-2 limbs only
+    static const int tm[4] = {1, 2, 2, 4}, tn[4] = {1, 2, 4, 4};
+    *tile_rows = 16 * tm[t];
+    *tile_cols = 16 * tn[t];
+    return t;
+}
+
 template <int N>
 hipError_t tile_launch(hipStream_t s, const TileArgs &t, const DV &A, const DV &B, const DV &C, const double *alpha,
                        const double *beta, const Status *st) {

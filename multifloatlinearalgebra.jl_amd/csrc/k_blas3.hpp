@@ -26,4 +26,9 @@ struct TileArgs {
 hipError_t tile_launch_n(int N, hipStream_t s, const TileArgs &t, const DV &A, const DV &B, const DV &C,
                          const double *alpha, const double *beta, const Status *st);
 
+// The output tile a launch over rows x n outputs takes (requested < 0: the
+// automatic choice): its index 0..3 as the launcher resolves it, and its
+// rows and columns per 256-thread workgroup.
+int tile_geometry(int N, int64_t rows, int64_t n, bool lower, int requested, int *tile_rows, int *tile_cols);
+
 }  // namespace mfk

@@ -48,7 +48,30 @@ class MFConfig(ctypes.Structure):
         ("gemm_micro_columns", ctypes.c_int64),
         ("trsm_order", ctypes.c_int32),
         ("gemm_tile", ctypes.c_int32),
+        ("gemm_packed_crossover", ctypes.c_int64),
     ]
+
+
+class MFGemmPlan(ctypes.Structure):
+    _fields_ = [
+        ("strategy", ctypes.c_int32),
+        ("reason", ctypes.c_int32),
+        ("panel_columns", ctypes.c_int64),
+        ("micro_columns", ctypes.c_int64),
+        ("workers", ctypes.c_int64),
+        ("packed_elements_per_worker", ctypes.c_int64),
+        ("tile", ctypes.c_int32),
+        ("tile_rows", ctypes.c_int32),
+        ("tile_cols", ctypes.c_int32),
+        ("network", ctypes.c_int32),
+        ("launches", ctypes.c_int64),
+        ("workgroups", ctypes.c_int64),
+        ("remainder_rows", ctypes.c_int64),
+    ]
+
+
+class MFLDLTPlan(ctypes.Structure):
+    _fields_ = [("strategy", ctypes.c_int32), ("reason", ctypes.c_int32), ("block_size", ctypes.c_int64)]
 
 
 # Every symbol include/mfla_hip.h declares, with its ctypes prototype.
@@ -78,6 +101,8 @@ PROTOTYPES = {
     "mfla_from_interleaved": (_I, [_P, _I, _M, _P, _I64]),
     "mfla_to_interleaved": (_I, [_P, _I, _P, _I64, _M]),
     "mfla_gemm": (_I, [_P, _I, _M, _M, _M, _DP, _DP, _CP]),
+    "mfla_gemm_plan": (_I, [_I, _I64, _I64, _I64, _CP, ctypes.POINTER(MFGemmPlan)]),
+    "mfla_ldlt_plan": (_I, [_I, _I64, _CP, ctypes.POINTER(MFLDLTPlan)]),
     "mfla_syrk": (_I, [_P, _I, _I, _M, _M, _DP, _DP, _CP]),
     "mfla_gemmt": (_I, [_P, _I, _M, _M, _M, _DP, _DP, _CP]),
     "mfla_trsm": (_I, [_P, _I, _I, _I, _I, _I, _M, _M, _DP, _CP]),

@@ -145,11 +145,71 @@ class KernelConfig:
         c.gemm_micro_columns = self.gemm_micro_columns
         c.trsm_order = _ORDER[self.trsm_order]
         c.gemm_tile = self.gemm_tile
+        c.gemm_packed_crossover = self.gemm_packed_crossover
         return c
 
 
 def _cfg(config):
     return ctypes.byref((config or KernelConfig())._c())
+
+
+# ------------------------------------------------------------------- plans
+_PLAN_REASON = {1: "forced_direct", 2: "forced_packed", 3: "forced_fused", 4: "forced_ozaki",
+                5: "auto_outside_calibrated_shape", 6: "auto_reduction_too_small", 7: "auto_below_crossover",
+                8: "auto_above_crossover", 9: "auto_fused_direct", 10: "forced_unblocked", 11: "forced_blocked"}
+_GEMM_NAME = {v: k for k, v in _GEMM.items()}
+_LDLT_NAME = {v: k for k, v in _LDLT.items()}
+
+
+@dataclass(frozen=True)
+class GemmPlan:
+    """GemmPlan (src/types.jl:210-217): MFLA's route and CPU decomposition,
+    plus the device launch gemm! makes for it (tile index and shape, network
+    "mulacc_x3" or "standard", launches, workgroups, x3 remainder rows)."""
+
+    strategy: str
+    reason: str
+    panel_columns: int
+    micro_columns: int
+    workers: int
+    packed_elements_per_worker: int
+    tile: int
+    tile_rows: int
+    tile_cols: int
+    network: str
+    launches: int
+    workgroups: int
+    remainder_rows: int
+
+
+@dataclass(frozen=True)
+class LDLTPlan:
+    """LDLTPlan (src/types.jl:219-223)."""
+
+    strategy: str
+    reason: str
+    block_size: int
+
+
+def gemm_plan(limbs: int, m: int, k: int, n: int, config: KernelConfig | None = None) -> GemmPlan:
+    """gemm_plan(T, m, k, n, config) (src/kernels/gemm.jl:44-136) for
+    Float64x{limbs}: the same strategy / reason resolution (volume-based
+    packed crossover on near-square shapes, fused mulacc_x3 for x3 on :auto)
+    and the device launch that gemm! runs for it."""
+    p = L.MFGemmPlan()
+    _raise(L.load().mfla_gemm_plan(int(limbs), int(m), int(k), int(n), _cfg(config), ctypes.byref(p)))
+    return GemmPlan(_GEMM_NAME[p.strategy], _PLAN_REASON[p.reason], p.panel_columns, p.micro_columns, p.workers,
+                    p.packed_elements_per_worker, p.tile, p.tile_rows, p.tile_cols, "mulacc_x3" if p.network else "standard",
+                    p.launches, p.workgroups, p.remainder_rows)
+
+
+def ldlt_plan(limbs: int, n: int, config: KernelConfig | None = None) -> LDLTPlan:
+    """ldlt_plan(T, n, config) (src/factorizations/ldlt.jl:329-350): the
+    unblocked route below the crossover (512), the lazy blocked panel of
+    width 16 / 12 / 8 above it; ldlt! follows this plan."""
+    p = L.MFLDLTPlan()
+    _raise(L.load().mfla_ldlt_plan(int(limbs), int(n), _cfg(config), ctypes.byref(p)))
+    return LDLTPlan(_LDLT_NAME[p.strategy], _PLAN_REASON[p.reason], p.block_size)
 
 
 # ------------------------------------------------------------------ arrays
