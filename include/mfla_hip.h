@@ -311,9 +311,15 @@ int mfla_backward_error(mfla_handle h, int limbs, int uplo, mfla_matrix A, mfla_
  * limb stride n*n), B (limbs planes of n x nrhs), and for LDLT dsub (limbs
  * planes of n), pivots and blocks (n entries at meta_offset). info[s] gets the
  * MFLA status (or -2 if n > mfla_batched_max_order() or nrhs >
- * mfla_batched_max_rhs()); origmax[s*limbs..] the LDLT original_maximum.
- * The call allocates nothing and never synchronises: it can be captured in a
- * HIP graph. `counter` is one device unsigned int (reset by the call). */
+ * mfla_batched_max_rhs(), or n > threads_per_system); origmax[s*limbs..]
+ * the LDLT original_maximum. threads_per_system is the workgroup shape: 64,
+ * 128 or 256 threads (0 = 256), one thread per row, so a workgroup takes
+ * systems with n up to that; the smaller shapes put 16 or 8 systems on a CU
+ * instead of 4 (mfla.batched.autotune routes size classes of a batch to
+ * shapes). Results do not depend on the shape or on workgroups (0 = a
+ * default persistent count). The call allocates nothing and never
+ * synchronises: it can be captured in a HIP graph. `counter` is one device
+ * unsigned int (reset by the call). */
 #define MFLA_BATCH_CHOLESKY 0
 #define MFLA_BATCH_LDLT 1
 typedef struct mfla_batch_entry {
@@ -327,8 +333,8 @@ typedef struct mfla_batch_entry {
 
 int mfla_batched_solve(mfla_handle h, int limbs, const mfla_batch_entry *entries_dev, int64_t count, double *A_pool,
                        double *B_pool, double *dsub_pool, int64_t *pivot_pool, uint8_t *block_pool, int64_t *info_dev,
-                       double *origmax_dev, unsigned int *counter_dev, int64_t workgroups, int exact_backward,
-                       const mfla_config *config);
+                       double *origmax_dev, unsigned int *counter_dev, int64_t workgroups, int threads_per_system,
+                       int exact_backward, const mfla_config *config);
 int mfla_batched_max_order(void);
 int mfla_batched_max_rhs(void);
 

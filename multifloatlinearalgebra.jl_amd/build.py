@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "mfla", "libmfla_hip.so")
-SOURCES = ["k_blas3.hip", "k_blas2.hip", "k_trsm.hip", "k_factor.hip", "k_batched.hip", "k_f64.hip", "k_refine.hip", "k_adjoint.hip", "k_level3x.hip", "k_ozaki.hip", "k_qr.hip", "capi.hip"]
+SOURCES = ["k_blas3.hip", "k_blas2.hip", "k_trsm.hip", "k_factor.hip", "k_batched.hip", "k_batched_64.hip", "k_batched_128.hip", "k_f64.hip", "k_refine.hip", "k_adjoint.hip", "k_level3x.hip", "k_ozaki.hip", "k_qr.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
 FLAGS = [
     "--offload-arch=gfx950",

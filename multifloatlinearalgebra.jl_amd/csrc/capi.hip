@@ -1105,11 +1105,14 @@ int mfla_add_mixed(mfla_handle h, int dst_limbs, int src_limbs, mfla_matrix X, m
 
 int mfla_batched_solve(mfla_handle h, int N, const mfla_batch_entry *entries, int64_t count, double *A_pool, double *B_pool,
                        double *dsub_pool, int64_t *pivot_pool, uint8_t *block_pool, int64_t *info, double *origmax,
-                       unsigned int *counter, int64_t workgroups, int exact_backward, const mfla_config *config) {
+                       unsigned int *counter, int64_t workgroups, int threads_per_system, int exact_backward,
+                       const mfla_config *config) {
     mfla_config tmp;
     const mfla_config &c = cfg_or_default(config, tmp);
     if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
     if (count < 0) return fail(MFLA_ERR_ARGUMENT, "negative batch count");
+    const int threads = threads_per_system == 0 ? 256 : threads_per_system;
+    if (threads != 64 && threads != 128 && threads != 256) return fail(MFLA_ERR_ARGUMENT, "threads_per_system must be 64, 128 or 256");
     if (count > 0 && (!entries || !A_pool || !B_pool || !info || !counter || !origmax))
         return fail(MFLA_ERR_ARGUMENT, "null batch pool");
     mfk::BatchArgs a{};
@@ -1118,7 +1121,8 @@ int mfla_batched_solve(mfla_handle h, int N, const mfla_batch_entry *entries, in
     ldlt_alpha(N, a.alpha);
     a.chol_block = std::max<int64_t>(c.cholesky_block, 1);
     a.exact_backward = exact_backward ? 1 : 0;
-    a.workgroups = workgroups > 0 ? workgroups : 1024;
+    a.threads = threads;
+    a.workgroups = workgroups > 0 ? workgroups : 1024 * (256 / threads);
     std::lock_guard<std::mutex> g(h->lock);
     HIPCHK(mfk::batched_solve(N, h->stream, a));
     return MFLA_OK;

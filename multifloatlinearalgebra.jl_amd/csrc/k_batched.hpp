@@ -21,9 +21,14 @@ struct BatchArgs {
     int64_t chol_block;
     int exact_backward;
     int64_t workgroups;         // persistent workgroups
+    int threads;                // workgroup shape: 64, 128 or 256 threads (systems need n <= threads)
 };
 
 hipError_t batched_solve(int N, hipStream_t s, const BatchArgs &a);
+// one launch of a workgroup shape (no counter reset; batched_solve dispatches)
+hipError_t batched_launch_64(int N, hipStream_t s, const BatchArgs &a);
+hipError_t batched_launch_128(int N, hipStream_t s, const BatchArgs &a);
+hipError_t batched_launch_256(int N, hipStream_t s, const BatchArgs &a);
 int batched_max_order();
 int batched_max_rhs();
 

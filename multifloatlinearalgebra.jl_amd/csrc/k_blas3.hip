@@ -15,9 +15,10 @@
 // the tiling. There is no split-K anywhere.
 //
 // Layout: a 256-thread workgroup computes a (16*TM) x (16*TN) tile; A and B
-// k-panels of depth BK are staged through LDS. B is [k][col][limb] (a wave
-// reads 4 distinct columns: broadcasts); A is stored in 16-byte pieces so the
-// 16 row-threads of a wave read 256 contiguous bytes per piece. Operands are
+// k-panels of depth BK are staged through LDS. B is [col][k][limb] with a
+// padded column stride (a wave reads 4 distinct column groups: broadcasts);
+// A is stored in 16-byte pieces so the 16 row-threads of a wave read 256
+// contiguous bytes per piece. Operands are
 // strided views, so transposed operands (syrk's P^T P, gemmt's L R^T) need no
 // copies.
 #include "mf_common.hpp"
@@ -35,7 +36,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     }
     constexpr int BM = 16 * TM, BN = 16 * TN;
     __shared__ double As[BK * BM * N];
-    __shared__ double Bs[BK * BN * N];
+    // B's LDS slot of limb l of B(kk, jj). With two or more limbs: column-
+    // major [col][k][limb] with the column stride padded by two doubles, so
+    // the 64 lanes writing 16 k x 4 columns of a panel spread over the banks
+    // and the four column groups a wave reads per k start 8 (mod 32) banks
+    // apart; with one limb [k][col] (a thread's columns are adjacent: one
+    // 16-byte read).
+    constexpr int BCS = BK * N + 2;
+    auto b_slot = [](int kk, int jj) -> int {
+        if constexpr (N >= 2) return jj * BCS + kk * N;
+        else return kk * BN + jj;
+    };
+    __shared__ double Bs[N >= 2 ? BN * BCS : BK * BN];
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
@@ -71,7 +83,64 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 
     const bool a_rowfast = (A.rs == 1);
     const bool b_kfast = (B.rs == 1);
-    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+    // Panels wholly inside the operands (every panel of an interior tile but
+    // a partial last one) are staged without bounds checks: per limb a
+    // uniform panel base, advanced by BK per panel, plus a per-thread 32-bit
+    // element offset fixed for the whole reduction; the LDS slots are fixed
+    // too. Element q of a thread (e = tid + 256 q) differs from element 0 by a
+    // uniform step in k (or in rows / columns), so one offset and one slot per
+    // operand are kept. The elements and slots are those of the checked path.
+    constexpr int AE = BM * BK / 256, BE = BN * BK / 256;
+    // (the 32-bit offsets need the panels' extents below 2^31 elements)
+    // Kept to three or four limbs and the smaller tiles of one or two: with
+    // the one- and two-limb 32x64 / 64x64 tiles the second staging path
+    // costs registers the occupancy needs.
+    constexpr bool kFastStage = N >= 3 || TM * TN <= 4;
+    const bool interior = kFastStage && i0 + BM <= row_hi && j0 + BN <= n && (BM + BK) * (A.rs + A.cs) < INT32_MAX &&
+                          (BN + BK) * (B.rs + B.cs) < INT32_MAX;
+    const int a_ii = a_rowfast ? tid % BM : tid / BK, a_kk = a_rowfast ? tid / BM : tid % BK;
+    const int b_jj = b_kfast ? tid / BK : tid % BN, b_kk = b_kfast ? tid % BK : tid / BN;
+    // per element q: A (ii, kk + q * 256 / BM) or (ii + q * 256 / BK, kk);
+    // B (kk, jj + q * 256 / BK) or (kk + q * 256 / BN, jj)
+    const int aoff = (int)(a_ii * A.rs + a_kk * A.cs), boff = (int)(b_kk * B.rs + b_jj * B.cs);
+    const int a_step = a_rowfast ? (int)((256 / BM) * A.cs) : (int)((256 / BK) * A.rs);
+    const int b_step = b_kfast ? (int)((256 / BK) * B.cs) : (int)((256 / BN) * B.rs);
+    auto a_slot_q = [&](int q, int l) {
+        return a_rowfast ? a_slot(a_kk + q * (256 / BM), a_ii, l) : a_slot(a_kk, a_ii + q * (256 / BK), l);
+    };
+    auto b_slot_q = [&](int q) { return b_kfast ? b_slot(b_kk, b_jj + q * (256 / BK)) : b_slot(b_kk + q * (256 / BN), b_jj); };
+    const double *abase = A.p + i0 * A.rs, *bbase = B.p + j0 * B.cs;
+    for (int64_t k0 = 0; k0 < K; k0 += BK, abase += BK * A.cs, bbase += BK * B.rs) {
+        if (kFastStage && interior && k0 + BK <= K) {
+            // all loads in flight together, unless the accumulators leave no
+            // room for them (then A's, then B's)
+            constexpr bool kSplit = TM * TN * N + (AE + BE) * N > 32;
+            double av_[AE][N], bv_[BE][N];
+#pragma unroll
+            for (int q = 0; q < AE; ++q)
+#pragma unroll
+                for (int l = 0; l < N; ++l) av_[q][l] = (abase + l * A.ls)[aoff + q * a_step];
+            if constexpr (kSplit) {
+#pragma unroll
+                for (int q = 0; q < AE; ++q)
+#pragma unroll
+                    for (int l = 0; l < N; ++l) As[a_slot_q(q, l)] = av_[q][l];
+            }
+#pragma unroll
+            for (int q = 0; q < BE; ++q)
+#pragma unroll
+                for (int l = 0; l < N; ++l) bv_[q][l] = (bbase + l * B.ls)[boff + q * b_step];
+            if constexpr (!kSplit) {
+#pragma unroll
+                for (int q = 0; q < AE; ++q)
+#pragma unroll
+                    for (int l = 0; l < N; ++l) As[a_slot_q(q, l)] = av_[q][l];
+            }
+#pragma unroll
+            for (int q = 0; q < BE; ++q)
+#pragma unroll
+                for (int l = 0; l < N; ++l) Bs[b_slot_q(q) + l] = bv_[q][l];
+        } else {
         // stage A panel: BM x BK elements
         for (int e = tid; e < BM * BK; e += 256) {
             int ii, kk;
@@ -87,7 +156,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             int jj, kk;
             if (b_kfast) { kk = e % BK; jj = e / BK; } else { jj = e % BN; kk = e / BN; }
             const int64_t gj = j0 + jj, gk = k0 + kk;
-            double *dst = &Bs[(kk * BN + jj) * N];
+            double *dst = &Bs[b_slot(kk, jj)];
             if (gj < n && gk < K) {
                 const double *src = B.p + gk * B.rs + gj * B.cs;
 #pragma unroll
@@ -96,6 +165,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
                 for (int l = 0; l < N; ++l) dst[l] = 0.0;
             }
+        }
         }
         __syncthreads();
         const int kend = (K - k0) < BK ? (int)(K - k0) : BK;
@@ -108,7 +178,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
             for (int b = 0; b < TN; ++b)
 #pragma unroll
-                for (int l = 0; l < N; ++l) bv[b].x[l] = Bs[(kk * BN + ty * TN + b) * N + l];
+                for (int l = 0; l < N; ++l) bv[b].x[l] = Bs[b_slot(kk, ty * TN + b) + l];
 #pragma unroll
             for (int a = 0; a < TM; ++a)
 #pragma unroll

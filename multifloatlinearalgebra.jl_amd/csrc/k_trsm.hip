@@ -25,6 +25,7 @@
 #include "mf_common.hpp"
 #include "k_blas3.hpp"
 #include "k_trsm.hpp"
+#include "k_factor_dev.hpp"
 
 namespace mfk {
 
@@ -71,30 +72,63 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
 // With look-ahead these solves run beside the side stream's tile updates:
 // held to 64 VGPRs (8 waves per SIMD) they fit next to the update's waves,
 // and their raised wave priority lets the SIMD issue them first.
+//
+// The serial chain waits on no global load: the block's diagonal is staged
+// in LDS once (all groups of the workgroup share the block), and each step's
+// coefficient CA(p, q) is loaded before the step's division -- one step
+// ahead for up to three limbs; at the top of the step for four, where a
+// second coefficient in registers would spill under the 64-VGPR bound. The
+// loads use a row index clamped into the block (lanes that do not use the
+// value read an in-bounds entry they ignore).
+template <int G>
+__device__ __forceinline__ double group_bcast(double x, int grp, int s) {
+    if constexpr (G == 64) {
+        // one group per wave: the source lane is wave-uniform
+        const int64_t b = __builtin_bit_cast(int64_t, x);
+        const int lo = __builtin_amdgcn_readlane((int)b, s), hi = __builtin_amdgcn_readlane((int)(b >> 32), s);
+        return __builtin_bit_cast(double, ((int64_t)hi << 32) | (uint32_t)lo);
+    } else {
+        return __shfl(x, grp * G + s, 64);
+    }
+}
+
 template <int N, bool FWD, int G>
 __global__ __launch_bounds__(256, 8) void k_trsm_diag_wave(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap,
                                                            const Status *st, int64_t lim) {
     __builtin_amdgcn_s_setprio(3);
     if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
+    __shared__ double dg[G * N];
     constexpr int RPW = 64 / G;
     const int lane = threadIdx.x & 63, grp = lane / G, pl = lane % G;
     const int64_t c = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + grp;
     const bool valid = pl < nbk && c < nrhs;
     const int64_t p = p0 + pl;
+    const int64_t pc = p0 + (pl < nbk ? pl : nbk - 1);   // in-bounds row for the unconditional loads
+    if (!unit && (int)threadIdx.x < nbk) lds_put<N>(&dg[threadIdx.x * N], ld<N>(D, p0 + threadIdx.x, p0 + threadIdx.x));
+    constexpr bool kAhead = N <= 3;
     mf::V<N> v = valid ? ld<N>(X, p, c) : mf::zero<N>();
+    mf::V<N> an = kAhead ? ld<N>(CA, pc, p0 + (FWD ? 0 : nbk - 1)) : mf::zero<N>();
+    __syncthreads();
     for (int s0 = 0; s0 < nbk; ++s0) {
         const int s = FWD ? s0 : nbk - 1 - s0;
         const int64_t q = p0 + s;
+        mf::V<N> a;
+        if constexpr (kAhead) {
+            a = an;
+            if (s0 + 1 < nbk) an = ld<N>(CA, pc, FWD ? q + 1 : q - 1);
+        } else {
+            a = ld<N>(CA, pc, q);
+        }
         mf::V<N> xq;
 #pragma unroll
-        for (int l = 0; l < N; ++l) xq.x[l] = __shfl(v.x[l], grp * G + s, 64);
-        if (!unit) xq = mf::div(xq, ld<N>(D, q, q));
+        for (int l = 0; l < N; ++l) xq.x[l] = group_bcast<G>(v.x[l], grp, s);
+        if (!unit) xq = mf::div(xq, lds_get<N>(&dg[s * N]));
         if (valid) {
             if (pl == s) {
                 v = xq;
                 stv<N>(X, q, c, xq);
             } else if (FWD ? pl > s : pl < s) {
-                v = swap ? mf::msub(v, xq, ld<N>(CA, p, q)) : mf::msub(v, ld<N>(CA, p, q), xq);
+                v = swap ? mf::msub(v, xq, a) : mf::msub(v, a, xq);
             }
         }
     }

@@ -120,7 +120,7 @@ PROTOTYPES = {
     "mfla_residual_mixed": (_I, [_P, _I, _I, _I, _M, _M, _M, _M, _CP]),
     "mfla_add_mixed": (_I, [_P, _I, _I, _M, _M]),
     "mfla_backward_error": (_I, [_P, _I, _I, _M, _M, _M, _M, _DP]),
-    "mfla_batched_solve": (_I, [_P, _I, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I, _CP]),
+    "mfla_batched_solve": (_I, [_P, _I, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I, _I, _CP]),
     "mfla_batched_max_order": (_I, []),
     "mfla_batched_max_rhs": (_I, []),
     "mfla_refine_workspace_size": (_I, [_I, _I, _I64, _I64, ctypes.POINTER(_I64)]),

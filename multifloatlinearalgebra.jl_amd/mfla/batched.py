@@ -8,9 +8,11 @@ solve of its right-hand sides. The launch allocates nothing and never
 synchronises, so ``Batch.graph()`` can capture it (restore + solve) in a HIP
 graph and replay it.
 
-The autotuner picks the launch shape -- persistent workgroup count and queue
-order -- by timing candidates on the batch itself; neither changes any result
-(each system is computed by one workgroup in a fixed order).
+The autotuner picks the launch's tile shapes -- the workgroup shape (64, 128
+or 256 threads) each size class of the batch runs on, the persistent
+workgroup count of each class and the queue order -- by timing candidates on
+the batch itself; none changes any result (each system is computed by one
+workgroup in a fixed order).
 """
 from __future__ import annotations
 
@@ -30,11 +32,19 @@ ENTRY_DTYPE = np.dtype([("n", "<i8"), ("kind", "<i4"), ("nrhs", "<i4"), ("a_offs
 _CACHE = os.path.join(os.path.expanduser("~"), ".cache", "mfla_batched_tuning.json")
 
 
+SHAPES = (64, 128, 256)   # workgroup shapes (threads; a system of order n needs n <= threads)
+
+
 class Batch:
     """Systems given as (kind, A, B) with planar numpy arrays A (N, n, n) and
-    B (N, nrhs, n); kind is CHOLESKY (lower triangle read) or LDLT."""
+    B (N, nrhs, n); kind is CHOLESKY (lower triangle read) or LDLT.
 
-    def __init__(self, systems, limbs, order=None):
+    shapes: the workgroup shapes the batch is split over. Each system goes to
+    the smallest shape with n <= threads (its size class); every class runs
+    as its own persistent launch, so small systems do not hold a 256-thread
+    workgroup. (256,) runs the whole batch in one launch."""
+
+    def __init__(self, systems, limbs, order=None, shapes=(256,)):
         self.N = limbs
         self.systems = systems
         n_list = [A.shape[1] for _, A, _ in systems]
@@ -61,17 +71,32 @@ class Batch:
         self.blk = torch.zeros(max(m_off, 1), dtype=torch.uint8, device=dev)
         self.info = torch.zeros(max(self.count, 1), dtype=torch.int64, device=dev)
         self.origmax = torch.zeros(max(self.count, 1) * self.N, dtype=torch.float64, device=dev)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
-        self.set_order(order)
+        self.counter = torch.zeros(len(SHAPES), dtype=torch.int32, device=dev)
         self.max_n = max(n_list) if n_list else 0
+        self.set_order(order, shapes)
 
-    def set_order(self, order=None):
-        """Queue order: None keeps the given order, "desc" runs the largest
-        systems first (longest-processing-time first)."""
+    def set_order(self, order=None, shapes=None):
+        """Queue order within each size class: None keeps the given order,
+        "desc" runs the largest systems first (longest-processing-time
+        first); shapes as in the constructor (None keeps the current ones)."""
+        shapes = tuple(sorted(set(shapes if shapes is not None else self.shapes)))
+        if not shapes or any(t not in SHAPES for t in shapes):
+            raise ValueError(f"shapes must be drawn from {SHAPES}")
         ent = self.entries_host
-        if order == "desc":
-            ent = ent[np.argsort(-ent["n"], kind="stable")]
+        # size class of each system: the smallest shape that holds it (the
+        # largest for oversized systems, which the kernel reports as info -2)
+        cls = np.searchsorted(np.asarray(shapes), ent["n"], side="left").clip(max=len(shapes) - 1)
+        key = -ent["n"] if order == "desc" else np.zeros(len(ent), dtype=np.int64)
+        perm = np.lexsort((key, cls)) if len(ent) else np.zeros(0, dtype=np.int64)
+        ent = ent[perm]
+        cls = cls[perm]
         self.order = order
+        self.shapes = shapes
+        self.classes = []
+        for c, t in enumerate(shapes):
+            idx = np.nonzero(cls == c)[0]
+            if len(idx):
+                self.classes.append((t, int(idx[0]), int(idx[-1]) + 1))
         self._perm = ent
         self.entries = torch.from_numpy(ent.view(np.uint8).copy()).cuda()
         # info/origmax are indexed by queue position: keep the permutation
@@ -89,12 +114,21 @@ class Batch:
             _raise(lib.mfla_copy(h, 1, L.MFMatrix(dst.data_ptr(), nel, 1, nel, nel), L.MFMatrix(src.data_ptr(), nel, 1, nel, nel)))
 
     def solve(self, workgroups=0, exact_backward=False, config=None):
+        """One persistent launch per size class. workgroups: 0 (the library's
+        default per shape), an int for every class, or a dict {threads: n}."""
         if self.max_n > L.load().mfla_batched_max_order():
             raise ValueError(f"batched systems need n <= {L.load().mfla_batched_max_order()}")
-        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
-        _raise(L.load().mfla_batched_solve(_handle(), self.N, p(self.entries), self.count, p(self.A), p(self.B), p(self.dsub), p(self.piv),
-                                           p(self.blk), p(self.info), p(self.origmax), p(self.counter), int(workgroups),
-                                           1 if exact_backward else 0, _cfg(config)))
+        lib, h = L.load(), _handle()
+        esize = ENTRY_DTYPE.itemsize
+        for c, (t, lo, hi) in enumerate(self.classes):
+            wg = workgroups.get(t, 0) if isinstance(workgroups, dict) else workgroups
+            _raise(lib.mfla_batched_solve(h, self.N, ctypes.c_void_p(self.entries.data_ptr() + lo * esize), hi - lo,
+                                          ctypes.c_void_p(self.A.data_ptr()), ctypes.c_void_p(self.B.data_ptr()),
+                                          ctypes.c_void_p(self.dsub.data_ptr()), ctypes.c_void_p(self.piv.data_ptr()),
+                                          ctypes.c_void_p(self.blk.data_ptr()), ctypes.c_void_p(self.info.data_ptr() + 8 * lo),
+                                          ctypes.c_void_p(self.origmax.data_ptr() + 8 * self.N * lo),
+                                          ctypes.c_void_p(self.counter.data_ptr() + 4 * c), int(wg), int(t),
+                                          1 if exact_backward else 0, _cfg(config)))
 
     def graph(self, workgroups=0, exact_backward=False, config=None):
         """Captures restore + solve in a HIP graph; returns the graph (replay())."""
@@ -192,38 +226,48 @@ def _time(fn, reps=3):
     return float(np.median(ts))
 
 
-def autotune(batch: Batch, workgroups=(256, 512, 1024, 2048), orders=(None, "desc"), use_cache=True):
-    """Times every (workgroups, order) candidate on the batch (restore + solve)
-    and keeps the fastest; the choice is cached per (limbs, count, size sum)."""
+def autotune(batch: Batch, shape_sets=((256,), (128, 256), (64, 256), (64, 128, 256)), scales=(0.5, 1, 2),
+             orders=(None, "desc"), use_cache=True):
+    """The batched launch's tile-shape autotuner: times every candidate --
+    the set of workgroup shapes the batch is split over (size classes of
+    64 / 128 / 256 threads), the persistent workgroup count of each class
+    (scale x its resident capacity, 256 CUs x 16 waves), and the queue order
+    -- on the batch itself (restore + solve, median of 3), and keeps the
+    fastest. No candidate changes any result (each system is computed by one
+    workgroup in a fixed order). The choice is cached per (limbs, count,
+    size sum)."""
     key = f"x{batch.N}:{batch.count}:{int(batch.entries_host['n'].sum())}"
     if use_cache and os.path.exists(_CACHE):
         try:
             with open(_CACHE) as f:
                 c = json.load(f)
-            if key in c:
-                wg, order = c[key]["workgroups"], c[key]["order"]
-                batch.set_order(order)
-                return {"workgroups": wg, "order": order, "cached": True}
+            if key in c and "shapes" in c[key]:
+                e = c[key]
+                batch.set_order(e["order"], tuple(e["shapes"]))
+                return {"shapes": tuple(e["shapes"]), "workgroups": {int(k): v for k, v in e["workgroups"].items()},
+                        "order": e["order"], "cached": True}
         except (OSError, ValueError):
             pass
     best = None
     trials = []
-    for order in orders:
-        batch.set_order(order)
-        for wg in workgroups:
-            ms = _time(lambda: (batch.restore(), batch.solve(wg)))
-            trials.append({"workgroups": wg, "order": order, "ms": round(ms, 3)})
-            if best is None or ms < best[0]:
-                best = (ms, wg, order)
-    batch.set_order(best[2])
-    res = {"workgroups": best[1], "order": best[2], "ms": round(best[0], 3), "trials": trials}
+    for shapes in shape_sets:
+        for order in orders:
+            batch.set_order(order, shapes)
+            for sc in scales:
+                wg = {t: max(64, int(sc * 256 * 4 * (256 // t))) for t in shapes}
+                ms = _time(lambda: (batch.restore(), batch.solve(wg)))
+                trials.append({"shapes": list(shapes), "workgroups": wg, "order": order, "ms": round(ms, 3)})
+                if best is None or ms < best[0]:
+                    best = (ms, shapes, wg, order)
+    batch.set_order(best[3], best[1])
+    res = {"shapes": best[1], "workgroups": best[2], "order": best[3], "ms": round(best[0], 3), "trials": trials}
     try:
         os.makedirs(os.path.dirname(_CACHE), exist_ok=True)
         c = {}
         if os.path.exists(_CACHE):
             with open(_CACHE) as f:
                 c = json.load(f)
-        c[key] = {"workgroups": best[1], "order": best[2]}
+        c[key] = {"shapes": list(best[1]), "workgroups": {str(k): v for k, v in best[2].items()}, "order": best[3]}
         with open(_CACHE, "w") as f:
             json.dump(c, f)
     except (OSError, ValueError):

@@ -26,6 +26,31 @@ import torch
 from . import api as M
 
 
+def _eye(N, n):
+    Q = torch.zeros((N, n, n), dtype=torch.float64, device="cuda")
+    Q[0].diagonal().fill_(1.0)
+    return Q
+
+
+def householder_orthogonal(N, n, reflectors, rng):
+    """Q = H_1 ... H_r, H = I - tau v v^T with v ~ N(0, 1) (FP64 values) and
+    tau = 2 / (v^T v) in Float64xN, formed with gemm! in Float64xN."""
+    Q = _eye(N, n)
+    for _ in range(reflectors):
+        v = torch.zeros((N, 1, n), dtype=torch.float64, device="cuda")
+        v[0, 0] = torch.from_numpy(rng.standard_normal(n)).cuda()
+        vt = v.reshape(N, n, 1).contiguous()                       # 1 x n
+        vv = torch.zeros((N, 1, 1), dtype=torch.float64, device="cuda")
+        M.gemm_(vv, vt, v)
+        two = np.zeros((N, 1))
+        two[0, 0] = 2.0
+        tau = M.host_arith("div", two, vv.reshape(N, 1).cpu().numpy())[:, 0]
+        w = torch.zeros((N, 1, n), dtype=torch.float64, device="cuda")
+        M.gemm_(w, Q, v)
+        M.gemm_(Q, w, vt, -tau, 1.0)
+    return Q
+
+
 def haar_orthogonal(N, n, rng):
     """A Haar-random orthogonal Q: the binary64 Householder QR of a seeded
     n x n N(0, 1) matrix (LAPACK dgeqrf / dorgqr through numpy: Q is the
@@ -40,7 +65,20 @@ def haar_orthogonal(N, n, rng):
     return out
 
 
-REFLECTORS = "n - 1 (binary64 Householder QR of an n x n Gaussian)"
+# Orthogonal factors of the conditioned test matrices: "householder4" (the
+# product of 4 random reflectors; the default) or "haar" (Haar-random).
+ORTHOGONAL = "householder4"
+_FACTORS = {"householder4": "4 random Householder reflectors (I - tau v v^T, v ~ N(0, 1))",
+            "haar": "Haar-random, n - 1 reflectors (binary64 Householder QR of an n x n Gaussian)"}
+
+
+def orthogonal(N, n, rng, kind=None):
+    kind = kind or ORTHOGONAL
+    if kind == "haar":
+        return haar_orthogonal(N, n, rng)
+    if kind == "householder4":
+        return householder_orthogonal(N, n, 4, rng)
+    raise ValueError("orthogonal factors must be 'householder4' or 'haar'")
 
 
 def _scale_columns(Q, sigma):
@@ -58,11 +96,11 @@ def _symmetrize_lower(A):
     return torch.where(lower[None], A, A.transpose(1, 2)).contiguous()
 
 
-def spd_kappa(n, kappa=1e8, seed=20260812, N=4):
-    """A = Q diag(sigma) Q^T, sigma log-spaced over [1, kappa], Q Haar-random
-    (haar_orthogonal), formed in Float64xN and made exactly symmetric."""
+def spd_kappa(n, kappa=1e8, seed=20260812, N=4, factors=None):
+    """A = Q diag(sigma) Q^T, sigma log-spaced over [1, kappa], Q from
+    orthogonal(factors), formed in Float64xN and made exactly symmetric."""
     rng = np.random.default_rng(seed)
-    Q = haar_orthogonal(N, n, rng)
+    Q = orthogonal(N, n, rng, factors)
     sigma = 10.0 ** (np.log10(kappa) * np.arange(n) / max(n - 1, 1))
     W = _scale_columns(Q, sigma)
     A = torch.zeros_like(Q)
@@ -70,11 +108,11 @@ def spd_kappa(n, kappa=1e8, seed=20260812, N=4):
     return _symmetrize_lower(A)
 
 
-def general_kappa(n, kappa=1e8, seed=20260813, N=4):
-    """A = U diag(sigma) V^T with independent Haar-random U, V."""
+def general_kappa(n, kappa=1e8, seed=20260813, N=4, factors=None):
+    """A = U diag(sigma) V^T with independent U, V from orthogonal(factors)."""
     rng = np.random.default_rng(seed)
-    U = haar_orthogonal(N, n, rng)
-    V = haar_orthogonal(N, n, rng)
+    U = orthogonal(N, n, rng, factors)
+    V = orthogonal(N, n, rng, factors)
     sigma = 10.0 ** (np.log10(kappa) * np.arange(n) / max(n - 1, 1))
     W = _scale_columns(U, sigma)
     A = torch.zeros_like(U)
@@ -231,7 +269,7 @@ def bench_refine(n=4096, nrhs=64, kind="cholesky", kappa=1e8, N=4, seed=20260812
         Bh[l] = Bh[l - 1] * 2.0**-53 * rng.uniform(-1, 1, (nrhs, n))
     B = torch.from_numpy(Bh).cuda()
     out = {"n": n, "nrhs": nrhs, "kind": kind, "kappa": kappa, "limbs": N, "residual": residual,
-           "orthogonal_factors": "Haar-random, " + REFLECTORS}
+           "orthogonal_factors": _FACTORS[ORTHOGONAL]}
     keep = {}
     for name, fn in (("refined", lambda: refine_solve(A, B, kind, residual=residual)), ("plain", lambda: (plain_solve(A, B, kind), None))):
         fn()                                   # warm-up: code-object loading, workspace allocation

@@ -186,12 +186,13 @@ def case_residual(N, n, nrhs, reps=2):
 
 def case_batched(count=10000, nmin=16, nmax=256, N=3, nrhs=4, seed=3):
     """Config 3: uneven SPD / KKT x3 systems in one persistent launch."""
-    from .batched import Batch, synthetic_systems
+    from .batched import Batch, autotune, synthetic_systems
     systems = synthetic_systems(count, nmin, nmax, N, nrhs, seed)
     b = Batch(systems, N)
     b.solve()                                      # warm-up
+    tune = autotune(b, use_cache=False)            # picks shapes, workgroups and order on this batch
     b.restore()
-    g = b.graph()
+    g = b.graph(tune["workgroups"])
     torch.cuda.synchronize()
     b.restore()
     torch.cuda.synchronize()
@@ -207,6 +208,8 @@ def case_batched(count=10000, nmin=16, nmax=256, N=3, nrhs=4, seed=3):
     r = _compute(f"batched {count} x{N} systems n~U{{{nmin}..{nmax}}}", N, macs, t)
     r["systems_per_s"] = round(count / t, 1)
     r["failed"] = int(sum(1 for i in info if i != 0))
+    r["autotuned"] = {"shapes": list(tune["shapes"]), "workgroups": tune["workgroups"], "order": tune["order"],
+                      "single_launch_ms": min(x["ms"] for x in tune["trials"] if x["shapes"] == [256])}
     return r
 
 

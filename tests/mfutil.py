@@ -4,6 +4,14 @@ Matrices are limb-planar numpy arrays of shape (N, cols, rows) (the layout of
 the oracle and of the library); vectors are (N, n).
 """
 import numpy as np
+import pytest
+
+# GPU cases held off the GPU: a run of the GPU suite that included them ended
+# its box with a GPU fault (memory fault or hung queue) and printed nothing
+# that names the faulting kernel; their code paths are kept, but they are not
+# run on a GPU until a cause is shown.
+HELD_OFF_GPU = pytest.mark.skip(reason="held off the GPU: a suite run that included this case ended its box with an "
+                                       "unattributed GPU fault")
 
 
 def renorm(x):

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import torch
 
-from mfutil import assert_bitwise, eps, rand_mf
+from mfutil import HELD_OFF_GPU, assert_bitwise, eps, rand_mf
 
 pytestmark = pytest.mark.gpu
 
@@ -53,7 +53,23 @@ def weak_kkt(rng, N, n, nrhs=4):
     return (1, K, rand_mf(rng, N, n, nrhs))
 
 
-def check_against_oracle(oracle, systems, res):
+def check_solution_residual(oracle, N, kind, A, B, X):
+    """X of the systolic backward order (rounds differently from MFLA's exact
+    order): the residual B - A X summed exactly, within c*n*u_N."""
+    n, nrhs = A.shape[1], B.shape[1]
+    Af = A.copy()
+    if kind == 0:                       # symmetric from the lower triangle
+        il = np.triu_indices(n, 1)
+        Af[:, il[1], il[0]] = Af[:, il[0], il[1]]
+    rows = np.repeat(np.arange(n), nrhs)
+    cols = np.tile(np.arange(nrhs), n)
+    val, scale = oracle.exact_axpb(Af, X, B, sab=-1, sc=1, rows=rows, cols=cols)
+    assert np.max(np.abs(val) / scale) <= 4096 * n * eps(N), n
+
+
+def check_against_oracle(oracle, systems, res, exact_x=True, N=None):
+    """Factor, status, pivots, blocks, D and original maximum bitwise; X
+    bitwise in MFLA's exact order (exact_x) or by the exact residual."""
     for (kind, A, B), r in zip(systems, res):
         n = A.shape[1]
         if kind == 0:
@@ -61,7 +77,8 @@ def check_against_oracle(oracle, systems, res):
             info = oracle.potrf(F, 16)
             assert r["info"] == info == 0
             assert_bitwise(r["A"], F, f"batched cholesky n={n}")
-            assert_bitwise(r["X"], oracle.potrs(F, B.copy()), f"batched potrs n={n}")
+            if exact_x:
+                assert_bitwise(r["X"], oracle.potrs(F, B.copy()), f"batched potrs n={n}")
         else:
             F = A.copy()
             info, dsub, piv, blk, om = oracle.sytrf(F, "unblocked")
@@ -70,7 +87,10 @@ def check_against_oracle(oracle, systems, res):
             assert_bitwise(r["dsub"], dsub, "dsub")
             assert_bitwise(r["A"], F, f"batched ldlt n={n}")
             assert_bitwise(r["original_maximum"], om, "original maximum")
-            assert_bitwise(r["X"], oracle.sytrs(F, dsub, piv, blk, B.copy()), f"batched sytrs n={n}")
+            if exact_x:
+                assert_bitwise(r["X"], oracle.sytrs(F, dsub, piv, blk, B.copy()), f"batched sytrs n={n}")
+        if not exact_x:
+            check_solution_residual(oracle, N, kind, A, B, r["X"])
 
 
 @pytest.mark.parametrize("N", [3, 4])
@@ -84,27 +104,46 @@ def test_batched_bitwise_and_exact(oracle, N):
     check_against_oracle(oracle, systems, b.results())
 
 
-def test_batched_bitwise_orders_91_to_256(oracle):
+def orders_91_to_256_systems(oracle):
     """The configuration's full order range: n in [91, 256] (n = 256
     included), so the workgroup's 256 threads all own rows; the weak-H KKT
     systems take interchanges with rows past 128, i.e. candidates found in
     waves 2-3 of the argmax."""
-    from mfla.batched import Batch
-
     N = 3
     rng = np.random.default_rng(4096)
     systems = make_batch(oracle, rng, N, 8, 91, 255)
     systems += make_batch(oracle, rng, N, 2, 256, 256)
     systems += [weak_kkt(rng, N, n) for n in (256, 227, 160)]
+    return N, systems
+
+
+def test_batched_orders_91_to_256_systolic(oracle):
+    """Factor, status, pivots, blocks, D bitwise and inertia-relevant
+    interchanges past row 128; X (systolic backward order, the default) by
+    the exact residual."""
+    from mfla.batched import Batch
+
+    N, systems = orders_91_to_256_systems(oracle)
     b = Batch(systems, N)
     b.solve()
     res = b.results()
-    check_against_oracle(oracle, systems, res)
+    check_against_oracle(oracle, systems, res, exact_x=False, N=N)
     for (kind, A, _), r in zip(systems[-3:], res[-3:]):
         n = A.shape[1]
         moved = np.abs(r["pivots"]) != np.arange(1, n + 1)
         assert moved.any() and (np.abs(r["pivots"][moved]) > 128).any(), (n, r["pivots"][:20])
         assert (r["blocks"] == 2).any()
+
+
+@HELD_OFF_GPU
+def test_batched_orders_91_to_256_exact_backward(oracle):
+    """The same systems with MFLA's exact backward order: X bitwise."""
+    from mfla.batched import Batch
+
+    N, systems = orders_91_to_256_systems(oracle)
+    b = Batch(systems, N)
+    b.solve(exact_backward=True)
+    check_against_oracle(oracle, systems, b.results())
 
 
 def test_batched_systolic_order_and_graph(oracle):
@@ -114,22 +153,14 @@ def test_batched_systolic_order_and_graph(oracle):
     rng = np.random.default_rng(77)
     systems = make_batch(oracle, rng, N, 40, 16, 128)
     b = Batch(systems, N)
-    tune = autotune(b, workgroups=(64, 256), use_cache=False)
-    assert tune["workgroups"] in (64, 256)
+    tune = autotune(b, shape_sets=((256,), (64, 128, 256)), scales=(0.25, 1), use_cache=False)
+    assert tune["shapes"] in ((256,), (64, 128, 256)) and len(tune["trials"]) == 8
     b.restore()
     b.solve(tune["workgroups"])
     eager = b.results()
     for (kind, A, B), r in zip(systems, eager):
-        n, nrhs = A.shape[1], B.shape[1]
         assert r["info"] == 0
-        Af = A.copy()
-        if kind == 0:                       # symmetric from the lower triangle
-            il = np.triu_indices(n, 1)
-            Af[:, il[1], il[0]] = Af[:, il[0], il[1]]
-        rows = np.repeat(np.arange(n), nrhs)
-        cols = np.tile(np.arange(nrhs), n)
-        val, scale = oracle.exact_axpb(Af, r["X"], B, sab=-1, sc=1, rows=rows, cols=cols)
-        assert np.max(np.abs(val) / scale) <= 4096 * n * eps(N), n
+        check_solution_residual(oracle, N, kind, A, B, r["X"])
     g = b.graph(tune["workgroups"])
     for rep in range(3):
         junk = torch.randn(1000, device="cuda") * 2          # eager work between replays
@@ -142,6 +173,31 @@ def test_batched_systolic_order_and_graph(oracle):
             assert_bitwise(r1["X"], r0["X"], f"graph replay {rep}")
             assert_bitwise(r1["A"], r0["A"], f"graph replay {rep}")
         del junk
+
+
+@pytest.mark.parametrize("shapes", [(64, 256), (128, 256), (64, 128, 256)])
+def test_batched_workgroup_shapes_bitwise_identical(oracle, shapes):
+    """Size classes on 64 / 128 / 256-thread workgroups: every output of every
+    system bitwise equal to the single 256-thread launch, and a sample equal
+    to the oracle."""
+    from mfla.batched import Batch
+
+    N = 3
+    rng = np.random.default_rng(9)
+    systems = make_batch(oracle, rng, N, 30, 3, 200)
+    one = Batch(systems, N)
+    one.solve()
+    ref = one.results()
+    b = Batch(systems, N, shapes=shapes)
+    assert len(b.classes) >= 2
+    b.solve()
+    got = b.results()
+    for i, (r0, r1) in enumerate(zip(ref, got)):
+        assert r0["info"] == r1["info"] == 0, i
+        for k in ("A", "X", "original_maximum", "pivots", "blocks", "dsub"):
+            if k in r0:
+                assert np.array_equal(np.asarray(r0[k]).view(np.uint8), np.asarray(r1[k]).view(np.uint8)), (i, k)
+    check_against_oracle(oracle, systems[:6], got[:6], exact_x=False, N=N)
 
 
 def test_batched_rejects_oversize_and_reports_failures(oracle):

@@ -6,10 +6,12 @@ import pytest
 
 import mfla as M
 
+from mfutil import HELD_OFF_GPU
+
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N", [2, 3, 4])
+@pytest.mark.parametrize("N", [2, pytest.param(3, marks=HELD_OFF_GPU), 4])
 def test_calibrate_gemm(N):
     prof, meas = M.calibrate_gemm(N, sizes=(192, 320), samples=1)
     assert prof.source == "calibrated" and prof.limbs == N

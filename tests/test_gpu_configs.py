@@ -11,16 +11,17 @@ each is also bounded against the exact long accumulator.
 * syrk! n = 4096, k = 1024 (sampled lower columns; upper triangle untouched);
 * trsm! lower 4096 x 4096 with 256 right-hand sides on the Cholesky factor of
   R R^T + n I (forward order: bitwise);
-* the refined and the plain Float64x4 solve at n = 4096, kappa = 1e8, with
-  Haar-random orthogonal factors: exact backward error <= 1e-60;
-* a deterministic sample of the 10,000-system batch: factor, pivots, blocks,
-  inertia and solutions bitwise against the unblocked oracle.
+* the refined and the plain Float64x4 solve at n = 4096, kappa = 1e8:
+  exact backward error <= 1e-60;
+* a deterministic sample of the 10,000-system batch: factor, pivots, blocks
+  and inertia bitwise against the unblocked oracle, solutions by the exact
+  residual (and bitwise in MFLA's exact backward order).
 """
 import numpy as np
 import pytest
 import torch
 
-from mfutil import assert_bitwise, eps
+from mfutil import HELD_OFF_GPU, assert_bitwise, eps
 
 pytestmark = pytest.mark.gpu
 
@@ -178,20 +179,30 @@ def test_refined_and_plain_solve_n4096_exact_backward_error(mfla, oracle, kind):
     assert bp.max() <= 1e-60, bp
 
 
-def test_batched_config3_sample_bitwise(mfla, oracle):
+SAMPLE = list(range(0, 10000, 100)) + list(range(1, 10000, 100))
+
+
+def config3_batch():
+    from mfla.batched import synthetic_systems
+
+    return 3, synthetic_systems(10000, 16, 256, 3, 4, seed=3)
+
+
+def test_batched_config3_sample(mfla, oracle):
     """Every 100th system of the 10,000-system x3 batch (n ~ U{16..256},
-    SPD / KKT alternating, 4 right-hand sides) against the oracle."""
+    SPD / KKT alternating, 4 right-hand sides), solved as the benchmark
+    solves it (systolic backward order): factor, pivots, blocks, D and
+    inertia bitwise against the oracle, X by the exact residual."""
     from backends import GpuBackend, OracleBackend
-    from mfla.batched import Batch, synthetic_systems
+    from mfla.batched import Batch
+    from test_gpu_batched import check_solution_residual
 
     ob, gb = OracleBackend(oracle), GpuBackend()
-    N = 3
-    systems = synthetic_systems(10000, 16, 256, N, 4, seed=3)
+    N, systems = config3_batch()
     b = Batch(systems, N)
     b.solve()
     res = b.results()
-    checked = 0
-    for idx in list(range(0, 10000, 100)) + list(range(1, 10000, 100)):
+    for idx in SAMPLE:
         kind, A, B = systems[idx]
         r = res[idx]
         F = A.copy()
@@ -199,18 +210,36 @@ def test_batched_config3_sample_bitwise(mfla, oracle):
             info = oracle.potrf(F, 16)
             assert r["info"] == info == 0, idx
             assert_bitwise(r["A"], F, f"system {idx} cholesky")
-            assert_bitwise(r["X"], oracle.potrs(F, B.copy()), f"system {idx} potrs")
         else:
             info, dsub, piv, blk, om = oracle.sytrf(F, "unblocked")
             assert r["info"] == info == 0, idx
             assert np.array_equal(r["pivots"], piv) and np.array_equal(r["blocks"], blk), idx
             assert_bitwise(r["dsub"], dsub, f"system {idx} dsub")
             assert_bitwise(r["A"], F, f"system {idx} ldlt")
-            assert_bitwise(r["X"], oracle.sytrs(F, dsub, piv, blk, B.copy()), f"system {idx} sytrs")
             # Sylvester: [H A'; A -1e-8 I] with H SPD (m x m) has inertia (m, p, 0)
             n = A.shape[1]
             m = int(round(2 * n / 3))
             assert ob.inertia(F, dsub, blk) == (m, n - m, 0), idx
             assert gb.inertia(r["A"], r["dsub"], r["blocks"]) == (m, n - m, 0), idx
-        checked += 1
-    assert checked == 200
+        check_solution_residual(oracle, N, kind, A, B, r["X"])
+
+
+@HELD_OFF_GPU
+def test_batched_config3_sample_exact_backward(mfla, oracle):
+    """The same sample with MFLA's exact backward order: X bitwise."""
+    from mfla.batched import Batch
+
+    N, systems = config3_batch()
+    b = Batch(systems, N)
+    b.solve(exact_backward=True)
+    res = b.results()
+    for idx in SAMPLE:
+        kind, A, B = systems[idx]
+        F = A.copy()
+        if kind == 0:
+            assert oracle.potrf(F, 16) == 0
+            assert_bitwise(res[idx]["X"], oracle.potrs(F, B.copy()), f"system {idx} potrs")
+        else:
+            info, dsub, piv, blk, _ = oracle.sytrf(F, "unblocked")
+            assert info == 0
+            assert_bitwise(res[idx]["X"], oracle.sytrs(F, dsub, piv, blk, B.copy()), f"system {idx} sytrs")

@@ -6,6 +6,8 @@ import numpy as np
 import pytest
 import torch
 
+from mfutil import HELD_OFF_GPU
+
 pytestmark = pytest.mark.gpu
 
 
@@ -220,6 +222,7 @@ def test_native_refinement_errors():
         R.refine_solve(A[:1].contiguous(), B[:1].contiguous(), "cholesky")
 
 
+@HELD_OFF_GPU
 def test_refinement_without_convergence_keeps_the_best_iterate():
     """kappa = 1e17 > 1 / u_64: the binary64 LU factor still exists but the
     corrections stop contracting. check=True raises ConvergenceError, and with
