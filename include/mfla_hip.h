@@ -254,6 +254,23 @@ int mfla_getrf(mfla_handle h, int limbs, mfla_matrix A, int64_t *ipiv_dev, doubl
 int mfla_sytrf(mfla_handle h, int limbs, mfla_matrix A, mfla_matrix dsub, int64_t *pivots_dev, uint8_t *blocks_dev,
                double *original_maximum, int64_t *info, const mfla_config *config, mfla_workspace ws);
 
+/* ldlt! that allocates nothing. No counterpart in MFLA: for callers that
+ * cannot allocate device memory during a call (code that sets up a HIP graph
+ * capture, tight serving loops, allocation-free harnesses). Arguments and
+ * results are those of mfla_sytrf (bit for bit) without a workspace: the
+ * blocked route's weighted panel and the original_maximum column maxima
+ * live in a static device array of MFLA_NOALLOC_SCRATCH_BYTES on each GPU
+ * (held until the process ends; its address is resolved per device when a
+ * handle is created). A call needs limbs * n * (block + 2) * 8 + 256 bytes
+ * of it (block = the ldlt_plan panel width; the unblocked route
+ * limbs * n * 8 + 256), else it returns MFLA_ERR_WORKSPACE before touching
+ * anything: e.g. Float64x4 (block 8) up to n = 26213. Every call on one GPU
+ * shares the array and leaves its contents behind: calls that overlap on
+ * one GPU (two handles, two host threads) race. */
+#define MFLA_NOALLOC_SCRATCH_BYTES (8u * 1024u * 1024u)
+int mfla_sytrf_noalloc(mfla_handle h, int limbs, mfla_matrix A, mfla_matrix dsub, int64_t *pivots_dev, uint8_t *blocks_dev,
+                       double *original_maximum, int64_t *info, const mfla_config *config);
+
 /* Householder QR. tau is a min(m, n) x 1 device matrix, perm_dev n int64
  * (one-based, A_original[:, perm] = Q R). Non-finite input returns info = -1
  * with tau = 0 and the identity permutation; A is then untouched.

@@ -54,6 +54,37 @@ struct mfla_workspace_s {
     double *weighted = nullptr;  // limbs * factor_capacity * (block_capacity + 1)
 };
 
+// Device scratch of the no-allocation entry point (mfla_sytrf_noalloc): a
+// file-scope device array, one copy per GPU, held until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): each device's address is resolved once,
+// with that device current, by hipGetSymbolAddress and kept in
+// g_scratch_addr.
+static __device__ __align__(256) double scratch[MFLA_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+double *g_scratch_addr[kMaxDevices] = {};
+hipError_t g_scratch_status[kMaxDevices] = {};
+bool g_scratch_resolved[kMaxDevices] = {};
+
+// Resolves the current device's copy of `scratch` (once per device).
+void resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> g(g_scratch_lock);
+    if (g_scratch_resolved[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_status[device] = e;
+    g_scratch_addr[device] = e == hipSuccess ? static_cast<double *>(p) : nullptr;
+    if (e != hipSuccess) (void)hipGetLastError();
+    g_scratch_resolved[device] = true;
+}
+
+}  // namespace
+
 namespace {
 
 thread_local std::string g_err;
@@ -112,7 +143,7 @@ const mfla_config &cfg_or_default(const mfla_config *c, mfla_config &tmp) {
     return tmp;
 }
 
-void *scratch(mfla_handle h, size_t bytes) {
+void *grow_scratch(mfla_handle h, size_t bytes) {
     if (h->scratch_bytes < bytes) {
         if (h->scratch) { (void)hipStreamSynchronize(h->stream); (void)hipFree(h->scratch); }
         h->scratch = nullptr;
@@ -214,7 +245,7 @@ int ozaki_gemm(mfla_handle h, int N, int64_t m, int64_t n, int64_t k, const DV &
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
     const size_t o_ea = take(sizeof(int) * m), o_eb = take(sizeof(int) * n), o_part = take(sizeof(double) * mfk::oz_chunks() * mx);
     const size_t o_da = take((size_t)S * m * ld), o_db = take((size_t)S * n * ld), o_pl = take(sizeof(int32_t) * (size_t)(U + 1) * m * n);
-    char *base = (char *)scratch(h, off);
+    char *base = (char *)grow_scratch(h, off);
     if (!base) return fail(MFLA_ERR_HIP, "scratch allocation failed (%zu bytes)", off);
     int *ea = (int *)(base + o_ea), *eb = (int *)(base + o_eb);
     double *part = (double *)(base + o_part);
@@ -303,7 +334,7 @@ int original_maximum(mfla_handle h, int N, const mfla_matrix &A, int lower_only,
     const int64_t cols = A.cols;
     for (int l = 0; l < N; ++l) out[l] = 0.0;
     if (A.rows * cols == 0) return MFLA_OK;
-    double *buf = (double *)scratch(h, sizeof(double) * N * cols);
+    double *buf = (double *)grow_scratch(h, sizeof(double) * N * cols);
     if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     HIPCHK(mfk::col_absmax_region(N, h->stream, dvn(A), A.rows, cols, lower_only, DV{buf, 1, cols, cols}));
     std::vector<double> host((size_t)N * cols);
@@ -376,6 +407,7 @@ int mfla_handle_create(mfla_handle *out) {
     if (!out) return fail(MFLA_ERR_ARGUMENT, "null handle pointer");
     auto *h = new mfla_handle_s();
     if (hipGetDevice(&h->device) != hipSuccess) { delete h; return fail(MFLA_ERR_HIP, "no HIP device"); }
+    resolve_scratch(h->device);
     if (hipMalloc(&h->status, sizeof(Status)) != hipSuccess) { delete h; return fail(MFLA_ERR_HIP, "status allocation failed"); }
     if (hipMalloc(&h->flags, 64 * sizeof(int)) != hipSuccess) { (void)hipFree(h->status); delete h; return fail(MFLA_ERR_HIP, "flag allocation failed"); }
     void *pin = nullptr;
@@ -904,10 +936,29 @@ int mfla_ldlt_plan(int N, int64_t n, const mfla_config *config, mfla_ldlt_plan_t
     return ldlt_plan_core(N, n, c, out);
 }
 
-int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *pivots, uint8_t *blocks, double *origmax, int64_t *info,
-               const mfla_config *config, mfla_workspace ws) {
-    mfla_config tmp;
-    const mfla_config &c = cfg_or_default(config, tmp);
+}  // extern "C"
+
+// Layout of the no-allocation ldlt!'s device scratch (in doubles): the
+// column maxima (limbs * n) at 0, their fold (limbs) at *fold_off, the
+// blocked route's weighted panel (limbs * n * (block + 1)) at *panel_off;
+// returns the total.
+static size_t noalloc_layout(int N, int64_t n, const mfla_ldlt_plan_t &plan, size_t *fold_off, size_t *panel_off) {
+    const size_t colmax_len = (size_t)N * (size_t)n;
+    *fold_off = (colmax_len + 31) / 32 * 32;
+    *panel_off = *fold_off + 32;
+    const size_t panel_len = plan.strategy == MFLA_LDLT_BLOCKED ? (size_t)N * (size_t)n * (size_t)(plan.block_size + 1) : 0;
+    return *panel_off + panel_len;
+}
+
+// ldlt! shared by mfla_sytrf and mfla_sytrf_noalloc. static_buf == nullptr:
+// the blocked route's weighted panel comes from the workspace (or is
+// allocated for the call) and original_maximum folds the column maxima on
+// the host. Otherwise static_buf (static_doubles long) holds the column
+// maxima (limbs * n), their fold (limbs) and the weighted panel
+// (limbs * n * (block + 1)), and nothing is allocated.
+static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matrix &dsub, int64_t *pivots, uint8_t *blocks,
+                     double *origmax, int64_t *info, const mfla_config &c, mfla_workspace ws, double *static_buf,
+                     size_t static_doubles) {
     const int64_t n = A.rows;
     if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "ldlt! requires a square matrix");
     if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
@@ -917,6 +968,11 @@ int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *p
     if (ldlt_plan_core(N, n, c, &plan) != MFLA_OK) return MFLA_ERR_ARGUMENT;
     const bool blocked = plan.strategy == MFLA_LDLT_BLOCKED;
     const int64_t b = plan.block_size;
+    size_t fold_off = 0, panel_off = 0;
+    const size_t need = noalloc_layout(N, n, plan, &fold_off, &panel_off);
+    if (static_buf && need > static_doubles)
+        return fail(MFLA_ERR_WORKSPACE, "ldlt! without allocation needs %zu bytes of device scratch (limbs * n * (block + 2) "
+                    "doubles plus 256 bytes), the library holds %zu", sizeof(double) * need, sizeof(double) * static_doubles);
 
     std::lock_guard<std::mutex> g(h->lock);
     hipStream_t s = h->stream;
@@ -928,7 +984,18 @@ int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *p
     HIPCHK(mfk::finite_check(N, s, n, n, dvn(A), 1, st));
     HIPCHK(mfk::mirror_lower(N, s, dvn(A), n, st));
     double om[4] = {0, 0, 0, 0};
-    if (int rc = original_maximum(h, N, A, 1, om)) return rc;
+    if (!static_buf) {
+        if (int rc = original_maximum(h, N, A, 1, om)) return rc;
+    } else if (n > 0) {
+        // original_maximum without allocation: the column maxima and their
+        // ordered fold on the device, N doubles read back
+        HIPCHK(mfk::col_absmax_region(N, s, dvn(A), n, n, 1, DV{static_buf, 1, n, n}));
+        HIPCHK(mfk::absmax_fold(N, s, DV{static_buf, 1, n, n}, n, static_buf + fold_off));
+        void *dst_[1] = {om};
+        const void *src_[1] = {static_buf + fold_off};
+        const size_t by_[1] = {sizeof(double) * (size_t)N};
+        HIPCHK(read_back(h, s, 1, dst_, src_, by_));
+    }
     double alpha[4];
     ldlt_alpha(N, alpha);
     const DV a = dvn(A), d = dvn(dsub);
@@ -937,10 +1004,11 @@ int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *p
     } else {
         double *W = nullptr;
         bool own = false;
-        const size_t need = sizeof(double) * (size_t)N * (size_t)n * (size_t)(b + 1);
-        if (ws && ws->limbs == N && ws->factor_capacity >= n && ws->block_capacity >= b) W = ws->weighted;
+        if (static_buf) W = static_buf + panel_off;
+        else if (ws && ws->limbs == N && ws->factor_capacity >= n && ws->block_capacity >= b) W = ws->weighted;
         else {
-            if (hipMalloc(&W, need) != hipSuccess) return fail(MFLA_ERR_HIP, "weighted panel allocation failed");
+            if (hipMalloc(&W, sizeof(double) * (size_t)N * (size_t)n * (size_t)(b + 1)) != hipSuccess)
+                return fail(MFLA_ERR_HIP, "weighted panel allocation failed");
             own = true;
         }
         int64_t pfirst = 0;
@@ -968,6 +1036,43 @@ int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *p
     *info = r;
     for (int l = 0; l < N; ++l) origmax[l] = (r == -1) ? 0.0 : om[l];
     return MFLA_OK;
+}
+
+extern "C" {
+
+int mfla_sytrf(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *pivots, uint8_t *blocks, double *origmax, int64_t *info,
+               const mfla_config *config, mfla_workspace ws) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    return sytrf_run(h, N, A, dsub, pivots, blocks, origmax, info, c, ws, nullptr, 0);
+}
+
+int mfla_sytrf_noalloc(mfla_handle h, int N, mfla_matrix A, mfla_matrix dsub, int64_t *pivots, uint8_t *blocks, double *origmax,
+                       int64_t *info, const mfla_config *config) {
+    mfla_config tmp;
+    const mfla_config &c = cfg_or_default(config, tmp);
+    // size check before the handle is used: a call that does not fit is
+    // rejected without touching the device
+    if (check_limbs(N)) return MFLA_ERR_ARGUMENT;
+    if (A.rows != A.cols) return fail(MFLA_ERR_DIMENSION, "ldlt! requires a square matrix");
+    if (A.rows < 0) return fail(MFLA_ERR_DIMENSION, "A has negative dimensions");
+    mfla_ldlt_plan_t plan;
+    if (ldlt_plan_core(N, A.rows, c, &plan) != MFLA_OK) return MFLA_ERR_ARGUMENT;
+    size_t fold_off = 0, panel_off = 0;
+    const size_t need = noalloc_layout(N, A.rows, plan, &fold_off, &panel_off);
+    if (need > MFLA_NOALLOC_SCRATCH_BYTES / sizeof(double))
+        return fail(MFLA_ERR_WORKSPACE, "ldlt! without allocation needs %zu bytes of device scratch (limbs * n * (block + 2) "
+                    "doubles plus 256 bytes), the library holds %zu", sizeof(double) * need, (size_t)MFLA_NOALLOC_SCRATCH_BYTES);
+    if (!h) return fail(MFLA_ERR_ARGUMENT, "null handle");
+    double *buf = nullptr;
+    hipError_t e = hipErrorInvalidDevice;
+    if (h->device >= 0 && h->device < kMaxDevices) {
+        std::lock_guard<std::mutex> g(g_scratch_lock);
+        buf = g_scratch_addr[h->device];
+        e = g_scratch_status[h->device];
+    }
+    if (!buf) return fail(MFLA_ERR_HIP, "device scratch unavailable on device %d (%s)", h->device, hipGetErrorString(e));
+    return sytrf_run(h, N, A, dsub, pivots, blocks, origmax, info, c, nullptr, buf, MFLA_NOALLOC_SCRATCH_BYTES / sizeof(double));
 }
 
 // ------------------------------------------------------------- solves
@@ -1143,7 +1248,7 @@ int mfla_backward_error(mfla_handle h, int N, int uplo, mfla_matrix A, mfla_matr
     hipStream_t s = h->stream;
     // scratch: row sums (m), three column-max vectors (nrhs each), flags
     const size_t dbl = (size_t)N * (size_t)(m + 3 * nrhs);
-    char *base = (char *)scratch(h, sizeof(double) * dbl + sizeof(int) * 3 * (size_t)nrhs + 64);
+    char *base = (char *)grow_scratch(h, sizeof(double) * dbl + sizeof(int) * 3 * (size_t)nrhs + 64);
     if (!base) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     double *rows = (double *)base;
     double *xm = rows + (size_t)N * m, *bm = xm + (size_t)N * nrhs, *rm = bm + (size_t)N * nrhs;
@@ -1786,7 +1891,7 @@ int mfla_logdet(mfla_handle h, int N, mfla_matrix F, int64_t info, double *out) 
     const int64_t n = F.rows;
     if (n == 0) { for (int l = 0; l < N; ++l) out[l] = 0.0; return MFLA_OK; }
     std::lock_guard<std::mutex> g(h->lock);
-    double *buf = (double *)scratch(h, sizeof(double) * (N + 1));
+    double *buf = (double *)grow_scratch(h, sizeof(double) * (N + 1));
     if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     HIPCHK(mfk::diag_logprod(N, h->stream, n, dvn(F), buf));
     double me[5];
@@ -1892,7 +1997,7 @@ int mfla_trmm(mfla_handle h, int N, int side, int uplo, int trans, int diag, mfl
     std::lock_guard<std::mutex> g(h->lock);
     hipStream_t s = h->stream;
     // the products read B's original values: work from a copy
-    double *bc = (double *)scratch(h, sizeof(double) * (size_t)N * m * n);
+    double *bc = (double *)grow_scratch(h, sizeof(double) * (size_t)N * m * n);
     if (!bc) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     const DV Bc{bc, 1, m, m * n};
     HIPCHK(mfk::copy(N, s, m, n, Bc, dvn(B)));
@@ -1910,7 +2015,7 @@ int mfla_dot(mfla_handle h, int N, mfla_matrix x, mfla_matrix y, double *out) {
     if (check_desc(x, "x") || check_desc(y, "y")) return MFLA_ERR_ARGUMENT;
     if (x.rows != y.rows || x.cols != y.cols) return fail(MFLA_ERR_DIMENSION, "dot product shapes differ");
     std::lock_guard<std::mutex> g(h->lock);
-    double *buf = (double *)scratch(h, sizeof(double) * N);
+    double *buf = (double *)grow_scratch(h, sizeof(double) * N);
     if (!buf) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     HIPCHK(mfk::dot(N, h->stream, x.rows, x.cols, dvn(x), dvn(y), buf));
     {
@@ -1937,7 +2042,7 @@ int mfla_syrk_packed(mfla_handle h, int N, mfla_matrix packed, mfla_matrix P, co
     if (n == 0) return MFLA_OK;
     std::lock_guard<std::mutex> g(h->lock);
     hipStream_t s = h->stream;
-    double *dense = (double *)scratch(h, sizeof(double) * (size_t)N * n * n);
+    double *dense = (double *)grow_scratch(h, sizeof(double) * (size_t)N * n * n);
     if (!dense) return fail(MFLA_ERR_HIP, "scratch allocation failed");
     const DV D{dense, 1, n, n * n};
     const double *be = beta ? beta : kZero;
@@ -1957,7 +2062,7 @@ namespace {
 int qr_state(mfla_handle h, int N, int64_t n, mfk::QrState &q) {
     const size_t vec = (size_t)N * n, dbl = 4 * vec + (size_t)N * mfk::QR_PANEL * n + (size_t)N * mfk::QR_PANEL + N;
     const size_t bytes = sizeof(double) * dbl + sizeof(int) * 2 * (size_t)n + 64 + 64;
-    char *base = (char *)scratch(h, bytes);
+    char *base = (char *)grow_scratch(h, bytes);
     if (!base) return fail(MFLA_ERR_HIP, "scratch allocation failed (%zu bytes)", bytes);
     double *d = (double *)base;
     q.scale = d; q.sum = d + vec; q.tscale = d + 2 * vec; q.tsum = d + 3 * vec;

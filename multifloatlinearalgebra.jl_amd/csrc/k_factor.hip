@@ -799,6 +799,30 @@ __global__ void k_col_absmax_region(DV A, int64_t m, int lower_only, DV out) {
     if (threadIdx.x == 0) stv<N>(out, c, 0, mx);
 }
 
+// max over count column maxima (limb-planar count x 1) into out[0..N): the
+// result of the sequential fold mx = vmax(mx, v_c), c ascending, from zero
+// (the host reduction of the same values). One wave: each lane folds a
+// contiguous chunk in order, then lane 0 folds the non-empty chunks' results
+// in order; vmax keeps the later of two equal values, so this is the
+// sequential fold's value, bit for bit.
+template <int N>
+__global__ __launch_bounds__(64) void k_absmax_fold(DV colmax, int64_t count, double *out) {
+    __shared__ double part[64 * N];
+    const int lane = threadIdx.x;
+    const int64_t chunk = (count + 63) / 64;
+    const int64_t lo = lane * chunk, hi = lo + chunk < count ? lo + chunk : count;
+    mf::V<N> mx = mf::zero<N>();
+    for (int64_t c = lo; c < hi; ++c) mx = mf::vmax(mx, ld<N>(colmax, c, 0));
+    lds_put<N>(&part[lane * N], mx);
+    __syncthreads();
+    if (lane == 0) {
+        mf::V<N> r = mf::zero<N>();
+        for (int l = 0; l < 64 && (int64_t)l * chunk < count; ++l) r = mf::vmax(r, lds_get<N>(&part[l * N]));
+#pragma unroll
+        for (int q = 0; q < N; ++q) out[q] = r.x[q];
+    }
+}
+
 __global__ void k_iota1(int64_t *p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = i + 1;
@@ -902,6 +926,11 @@ hipError_t ldlt_dsolve(int N, hipStream_t s, const DV &B, int64_t n, int64_t nrh
 hipError_t col_absmax_region(int N, hipStream_t s, const DV &A, int64_t m, int64_t ncols, int lower_only, const DV &out) {
     if (ncols == 0) return hipSuccess;
     MF_DISPATCH(N, hipLaunchKernelGGL(k_col_absmax_region<NN>, dim3((unsigned)ncols), dim3(256), 0, s, A, m, lower_only, out));
+    return hipGetLastError();
+}
+
+hipError_t absmax_fold(int N, hipStream_t s, const DV &colmax, int64_t count, double *out) {
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_absmax_fold<NN>, dim3(1), dim3(64), 0, s, colmax, count, out));
     return hipGetLastError();
 }
 

@@ -26,6 +26,7 @@ hipError_t ldlt_pivots(int N, hipStream_t s, const DV &B, int64_t n, int64_t nrh
 hipError_t ldlt_dsolve(int N, hipStream_t s, const DV &B, int64_t n, int64_t nrhs, const DV &F, const DV &dsub, const uint8_t *blocks,
                        Status *st);
 hipError_t col_absmax_region(int N, hipStream_t s, const DV &A, int64_t m, int64_t ncols, int lower_only, const DV &out);
+hipError_t absmax_fold(int N, hipStream_t s, const DV &colmax, int64_t count, double *out);
 hipError_t iota1(hipStream_t s, int64_t *p, int64_t n);
 hipError_t fill_u8(hipStream_t s, uint8_t *p, int64_t n, uint8_t v);
 

@@ -9,7 +9,7 @@ from .api import (  # noqa: F401
     SingularException, arith, cholesky_, ensure_workspace_capacity_, from_interleaved, gemm_, gemmt_, gemv_, host_arith, ldiv_, ldlt_,
     lu_, normwise_backward_error, refinement_correction_, residual_, residual_mixed_, solve, symv_, synchronize, syrk_, to_interleaved,
     trsm_, trsv_, workspace_capacity, trmm_, mfdot, syrk_packed_, capabilities, BoundsError, dgemm_, dpotrf_inverse_, dgetrf_,
-    MFQR, rrqr_, qr_, apply_q_, solve_r_, GemmPlan, LDLTPlan, gemm_plan, ldlt_plan,
+    MFQR, rrqr_, qr_, apply_q_, solve_r_, GemmPlan, LDLTPlan, gemm_plan, ldlt_plan, ldlt_noalloc_,
 )
 from .factor import (  # noqa: F401
     factor_blocks, factor_diagnostics, factor_inertia, factor_kind, factor_matrix, factor_permutation, factor_pivots,

@@ -112,6 +112,7 @@ PROTOTYPES = {
     "mfla_potrf": (_I, [_P, _I, _M, ctypes.POINTER(_I64), _CP]),
     "mfla_getrf": (_I, [_P, _I, _M, _P, _DP, ctypes.POINTER(_I64), _CP, _P]),
     "mfla_sytrf": (_I, [_P, _I, _M, _M, _P, _P, _DP, ctypes.POINTER(_I64), _CP, _P]),
+    "mfla_sytrf_noalloc": (_I, [_P, _I, _M, _M, _P, _P, _DP, ctypes.POINTER(_I64), _CP]),
     "mfla_potrs": (_I, [_P, _I, _M, _I64, _M, _CP]),
     "mfla_getrs": (_I, [_P, _I, _M, _P, _I64, _M, _CP]),
     "mfla_sytrs": (_I, [_P, _I, _M, _M, _P, _P, _I64, _M, _CP]),

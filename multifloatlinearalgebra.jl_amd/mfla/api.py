@@ -615,6 +615,28 @@ def ldlt_(A, *, check=True, config=None, workspace=None):
     return MFLDLT(A, dsub, piv, blocks, info, om[:N].copy())
 
 
+def ldlt_noalloc_(A, *, check=True, config=None):
+    """ldlt! through mfla_sytrf_noalloc: the same factorization (bit for
+    bit) with the library's static per-GPU device scratch instead of any
+    allocation during the call (the outputs dsub, pivots and blocks are
+    allocated here, by the caller, as for ldlt_)."""
+    N = _limbs(A)
+    n = rows_cols(A)[0]
+    dsub = torch.zeros((N, n), dtype=torch.float64, device=A.device)
+    piv = torch.empty(n, dtype=torch.int64, device=A.device)
+    blocks = torch.empty(n, dtype=torch.uint8, device=A.device)
+    om = np.zeros(4)
+    info = ctypes.c_int64(0)
+    _raise(L.load().mfla_sytrf_noalloc(_handle(), N, _mat(A), _mat(dsub), ctypes.c_void_p(piv.data_ptr()),
+                                       ctypes.c_void_p(blocks.data_ptr()), _sp(om), ctypes.byref(info), _cfg(config)))
+    info = int(info.value)
+    if check and info == -1:
+        raise DomainError("ldlt!: input matrix contains non-finite entries")
+    if check and info != 0:
+        raise SingularException(info)
+    return MFLDLT(A, dsub, piv, blocks, info, om[:N].copy())
+
+
 # ----------------------------------------------------------------- solves
 def _as_matrix(t):
     return t.unsqueeze(1) if t.dim() == 2 else t

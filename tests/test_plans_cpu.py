@@ -92,3 +92,38 @@ def test_ldlt_plan(N, block):
     assert M.ldlt_plan(N, 4096, M.KernelConfig(ldlt_block=1)).strategy == "unblocked"   # block 1: never blocked
     with pytest.raises(M.ArgumentError):
         M.ldlt_plan(N, 10, M.KernelConfig(ldlt_strategy="bogus"))
+
+
+def test_sytrf_noalloc_rejects_what_does_not_fit_before_touching_anything():
+    """mfla_sytrf_noalloc checks the static scratch size before it uses the
+    handle or any pointer: limbs * n * (block + 2) doubles + 256 bytes must fit
+    in MFLA_NOALLOC_SCRATCH_BYTES (8 MiB). Float64x4 (blocked, block 8): n =
+    26213 fits (the null handle is reported next), n = 26214 does not; the
+    unblocked route needs only the column maxima."""
+    import ctypes
+
+    from mfla import _lib
+
+    lib = _lib.load()
+    fake = 0x1000                       # never dereferenced: every call below returns before any access
+    info = ctypes.c_int64(0)
+    om = (ctypes.c_double * 4)()
+
+    def call(N, n, strategy=0):
+        cfg = _lib.MFConfig()
+        lib.mfla_config_default(ctypes.byref(cfg))
+        cfg.ldlt_strategy = strategy
+        A = _lib.MFMatrix(fake, n, n, n, n * n)
+        d = _lib.MFMatrix(fake, n, 1, n, n)
+        return lib.mfla_sytrf_noalloc(None, N, A, d, ctypes.c_void_p(fake), ctypes.c_void_p(fake), om, ctypes.byref(info),
+                                      ctypes.byref(cfg))
+
+    assert call(4, 26214) == _lib.MFLA_ERR_WORKSPACE
+    assert "device scratch" in _lib.last_error()
+    assert call(4, 26213) == _lib.MFLA_ERR_ARGUMENT and "null handle" in _lib.last_error()
+    # x1 (block 16): n = 58251 fits, 58252 does not
+    assert call(1, 58252) == _lib.MFLA_ERR_WORKSPACE
+    assert call(1, 58251) == _lib.MFLA_ERR_ARGUMENT
+    # forced unblocked: the column maxima and their fold only
+    assert call(4, 262136, strategy=1) == _lib.MFLA_ERR_ARGUMENT
+    assert call(4, 262137, strategy=1) == _lib.MFLA_ERR_WORKSPACE
