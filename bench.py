@@ -4,6 +4,8 @@ Headline (the JSON `value`): Float64x4 gemm! C = A*B at n = 4096, inputs
 resident in HBM, reported in FP64-TFLOP/s with EFT FLOPs counted:
     FLOPs = m*k*n * F_4, F_4 = 239 (x4 add network 124 adds + x4 product
     network 89 adds + 6 muls + 10 FMAs counted as 2; 229 FP64 instructions).
+The traffic figure is the rocprofv3 FETCH_SIZE + WRITE_SIZE of one dispatch of
+this kernel (copy-calibrated), measured separately and carried as a constant.
 One step = one gemm! call = one kernel launch. The roofline entry prices the
 kernel against MI355X's FP64 peak (78.6 TFLOP/s); `issue_frac` gives FP64
 instruction issue against 39.3 T lane-instructions/s.
@@ -40,7 +42,7 @@ PEAK_FP64_INSTR = 39.3e12
 # HBM bytes per launch of the Float64x4 n=4096 GEMM kernel: rocprofv3 FETCH_SIZE
 # and WRITE_SIZE of one dispatch, each scaled by the same counter's reading for
 # a plain 512 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
-MEASURED_TRAFFIC = {(4, 4096): {"bytes": 39.29e9, "fetch_kib": 18923466, "write_kib": 524288, "fetch_scale": 2.0,
+MEASURED_TRAFFIC = {(4, 4096): {"bytes": 39.24e9, "fetch_kib": 18899661, "write_kib": 524288, "fetch_scale": 2.0,
                                 "write_scale": 1.0,
                                 "source": "profiles/gemm_x4_n4096.md: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of one dispatch, "
                                           "copy-calibrated"}}
