@@ -38,7 +38,7 @@ struct mfla_handle_s {
     size_t scratch_bytes = 0;
     std::mutex lock;            // calls on one handle are serialised
     hipStream_t side = nullptr; // second stream of the look-ahead factorization
-    hipEvent_t ev_join = nullptr, ev_panel = nullptr, ev_update = nullptr;
+    hipEvent_t ev_join = nullptr, ev_panel = nullptr, ev_update = nullptr, ev_update2 = nullptr;
     // pinned host staging for the small device-to-host reads a call waits on
     // (status, per-iteration column maxima): one DMA transfer each instead of
     // a copy through the runtime's pageable staging; null = pageable path
@@ -230,6 +230,7 @@ hipError_t ensure_side(mfla_handle h) {
     if (e != hipSuccess) { h->side = nullptr; return e; }
     if ((e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&h->ev_update2, hipEventDisableTiming)) != hipSuccess) return e;
     return hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming);
 }
 
@@ -303,7 +304,7 @@ hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int di
         a.forward = !eff_lower;
     }
     if (n > 3 * 64 && (a.forward || !a.exact_order) && ensure_side(h) == hipSuccess) {
-        a.side = h->side; a.ev_main = h->ev_panel; a.ev_side = h->ev_update;
+        a.side = h->side; a.ev_main = h->ev_panel; a.ev_side = h->ev_update; a.ev_side2 = h->ev_update2;
         HIPRET_(hipEventRecord(h->ev_join, h->stream));
         HIPRET_(hipStreamWaitEvent(h->side, h->ev_join, 0));
     }
@@ -430,6 +431,7 @@ int mfla_handle_destroy(mfla_handle h) {
         (void)hipEventDestroy(h->ev_join);
         (void)hipEventDestroy(h->ev_panel);
         (void)hipEventDestroy(h->ev_update);
+        (void)hipEventDestroy(h->ev_update2);
     }
     delete h;
     return MFLA_OK;

@@ -70,8 +70,9 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
 // operations and their order are those of k_trsm_diag.
 //
 // With look-ahead these solves run beside the side stream's tile updates:
-// held to 64 VGPRs (8 waves per SIMD) they fit next to the update's waves,
-// and their raised wave priority lets the SIMD issue them first.
+// held to 64 VGPRs (8 waves per SIMD; 80 for four limbs, whose previous-block
+// terms would spill otherwise) they fit next to the update's waves, and their
+// raised wave priority lets the SIMD issue them first.
 //
 // The serial chain waits on no global load: the block's diagonal is staged
 // in LDS once (all groups of the workgroup share the block), and each step's
@@ -92,9 +93,14 @@ __device__ __forceinline__ double group_bcast(double x, int grp, int s) {
     }
 }
 
+//
+// With K > 0 the kernel first subtracts the K terms of the block solved just
+// before (unknowns q0 .. q0 + K - 1, ascending q: the order and operations of
+// the MODE_SUB tile update it replaces), so that update is not a kernel of
+// its own on the serial chain of blocks.
 template <int N, bool FWD, int G>
-__global__ __launch_bounds__(256, 8) void k_trsm_diag_wave(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap,
-                                                           const Status *st, int64_t lim) {
+__global__ __launch_bounds__(256, N == 4 ? 5 : 8) void k_trsm_diag_wave(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap,
+                                                           const Status *st, int64_t lim, int64_t q0, int K) {
     __builtin_amdgcn_s_setprio(3);
     if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
     __shared__ double dg[G * N];
@@ -107,6 +113,20 @@ __global__ __launch_bounds__(256, 8) void k_trsm_diag_wave(int64_t p0, int nbk, 
     if (!unit && (int)threadIdx.x < nbk) lds_put<N>(&dg[threadIdx.x * N], ld<N>(D, p0 + threadIdx.x, p0 + threadIdx.x));
     constexpr bool kAhead = N <= 3;
     mf::V<N> v = valid ? ld<N>(X, p, c) : mf::zero<N>();
+    if (K > 0) {
+        // terms of the previous block (every lane computes; invalid lanes'
+        // values are never stored), the next term's operands loaded ahead
+        const int64_t cc = c < nrhs ? c : nrhs - 1;
+        mf::V<N> an2 = ld<N>(CA, pc, q0), xn2 = ld<N>(X, q0, cc);
+        for (int k = 0; k < K; ++k) {
+            const mf::V<N> a = an2, xq = xn2;
+            if (k + 1 < K) {
+                an2 = ld<N>(CA, pc, q0 + k + 1);
+                xn2 = ld<N>(X, q0 + k + 1, cc);
+            }
+            v = swap ? mf::msub(v, xq, a) : mf::msub(v, a, xq);
+        }
+    }
     mf::V<N> an = kAhead ? ld<N>(CA, pc, p0 + (FWD ? 0 : nbk - 1)) : mf::zero<N>();
     __syncthreads();
     for (int s0 = 0; s0 < nbk; ++s0) {
@@ -153,13 +173,13 @@ static DV sub(const DV &v, int64_t i, int64_t j) { return DV{v.p + i * v.rs + j 
 
 template <int N>
 static hipError_t diag(hipStream_t s, bool fwd, int64_t p0, int nbk, int64_t nrhs, const DV &X, const DV &CA, const DV &D, int unit,
-                       int swap, const Status *st, int64_t lim) {
+                       int swap, const Status *st, int64_t lim, int64_t q0 = 0, int K = 0) {
     // one group of G lanes per right-hand side, 4 waves per workgroup
     auto launch = [&](auto gtag) {
         constexpr int G = decltype(gtag)::value;
         const dim3 grid((unsigned)cdiv(cdiv(nrhs, 64 / G), 4)), block(256);
-        if (fwd) hipLaunchKernelGGL((k_trsm_diag_wave<N, true, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
-        else hipLaunchKernelGGL((k_trsm_diag_wave<N, false, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim);
+        if (fwd) hipLaunchKernelGGL((k_trsm_diag_wave<N, true, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim, q0, K);
+        else hipLaunchKernelGGL((k_trsm_diag_wave<N, false, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim, q0, K);
     };
     if (nbk <= 16) launch(std::integral_constant<int, 16>());
     else if (nbk <= 32) launch(std::integral_constant<int, 32>());
@@ -188,41 +208,69 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
         u.rows = rows; u.K = K;
         return tile_launch_n(N, ss, u, sub(CA, r0, q0), sub(X, q0, 0), sub(X, r0, 0), nullptr, nullptr, st);
     };
-    // Look-ahead: block b's update of the next block's rows runs on s after
-    // the side stream's update from block b-1 reached them (every element
-    // keeps MFLA's block order); its update of the rows beyond runs on the
-    // side stream while s solves the next block.
+    // Look-ahead (blocked orders, n > 3 blocks, a side stream): the terms of
+    // block b reach the rows of block b + 1 in the next block's diagonal
+    // kernel (its prologue), and the rows beyond on the side stream while
+    // that kernel runs. Block b's kernel therefore waits for the side
+    // stream's update from block b - 2 (the last one covering its rows);
+    // every element still takes the blocks' terms in MFLA's block order, and
+    // each block's terms in ascending q.
     const bool ahead = a.side != nullptr && n > 3 * TB;
-    bool pending = false;
-    auto split_update = [&](int64_t q0, int64_t K, int64_t near0, int64_t nnear, int64_t far0, int64_t nfar) -> hipError_t {
-        hipError_t r;
-        if (!ahead) {
-            if ((r = update(s, near0, nnear, q0, K)) != hipSuccess) return r;
-            return update(s, far0, nfar, q0, K);
+    if (ahead) {
+        hipEvent_t far_ev[2] = {a.ev_side, a.ev_side2};
+        bool far_pending[2] = {false, false};
+        int64_t prev0 = 0;
+        int prevK = 0;
+        int64_t b = 0;
+        auto block_step = [&](int64_t p0, int nbk, int64_t far0, int64_t nfar) -> hipError_t {
+            hipError_t r;
+            const int slot = (int)(b & 1);
+            if (far_pending[slot]) {   // the update from block b - 2
+                if ((r = hipStreamWaitEvent(s, far_ev[slot], 0)) != hipSuccess) return r;
+                far_pending[slot] = false;
+            }
+            if ((r = diag<N>(s, a.forward != 0, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit, prev0, prevK)) !=
+                hipSuccess)
+                return r;
+            if (nfar > 0) {
+                if ((r = hipEventRecord(a.ev_main, s)) != hipSuccess) return r;
+                if ((r = hipStreamWaitEvent(a.side, a.ev_main, 0)) != hipSuccess) return r;
+                if ((r = update(a.side, far0, nfar, p0, nbk)) != hipSuccess) return r;
+                if ((r = hipEventRecord(far_ev[slot], a.side)) != hipSuccess) return r;
+                far_pending[slot] = true;
+            }
+            prev0 = p0;
+            prevK = nbk;
+            ++b;
+            return hipSuccess;
+        };
+        if (a.forward) {
+            for (int64_t p0 = 0; p0 < n; p0 += TB) {
+                const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
+                const int64_t p1 = p0 + nbk, nn = (n - p1) < TB ? (n - p1) : TB;
+                if ((e = block_step(p0, nbk, p1 + nn, n - p1 - nn)) != hipSuccess) return e;
+            }
+        } else {
+            for (int64_t p1 = n; p1 > 0; p1 -= TB) {
+                const int64_t p0 = p1 > TB ? p1 - TB : 0;
+                const int64_t nn = p0 < TB ? p0 : TB;
+                if ((e = block_step(p0, (int)(p1 - p0), 0, p0 - nn)) != hipSuccess) return e;
+            }
         }
-        if ((r = hipEventRecord(a.ev_main, s)) != hipSuccess) return r;
-        if (pending && (r = hipStreamWaitEvent(s, a.ev_side, 0)) != hipSuccess) return r;
-        pending = false;
-        if ((r = update(s, near0, nnear, q0, K)) != hipSuccess) return r;
-        if (nfar <= 0) return hipSuccess;
-        if ((r = hipStreamWaitEvent(a.side, a.ev_main, 0)) != hipSuccess) return r;
-        if ((r = update(a.side, far0, nfar, q0, K)) != hipSuccess) return r;
-        if ((r = hipEventRecord(a.ev_side, a.side)) != hipSuccess) return r;
-        pending = true;
+        for (int q = 0; q < 2; ++q)
+            if (far_pending[q] && (e = hipStreamWaitEvent(s, far_ev[q], 0)) != hipSuccess) return e;
         return hipSuccess;
-    };
-    auto finish = [&]() -> hipError_t { return pending ? hipStreamWaitEvent(s, a.ev_side, 0) : hipSuccess; };
+    }
+    // Without look-ahead: each block's update of the rows below (above) it
+    // right after its solve, on the caller's stream.
     if (a.forward) {
         for (int64_t p0 = 0; p0 < n; p0 += TB) {
             const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
             if ((e = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit)) != hipSuccess) return e;
             const int64_t p1 = p0 + nbk;
-            if (p1 < n) {
-                const int64_t nn = (n - p1) < TB ? (n - p1) : TB;
-                if ((e = split_update(p0, nbk, p1, nn, p1 + nn, n - p1 - nn)) != hipSuccess) return e;
-            }
+            if (p1 < n && (e = update(s, p1, n - p1, p0, nbk)) != hipSuccess) return e;
         }
-        return finish();
+        return hipSuccess;
     }
     if (a.exact_order) {
         hipLaunchKernelGGL(k_trsm_seq_bwd<N>, dim3((unsigned)cdiv(nrhs, 64)), dim3(64), 0, s, n, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit);
@@ -232,12 +280,9 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
         const int64_t p0 = p1 > TB ? p1 - TB : 0;
         const int nbk = (int)(p1 - p0);
         if ((e = diag<N>(s, false, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit)) != hipSuccess) return e;
-        if (p0 > 0) {
-            const int64_t nn = p0 < TB ? p0 : TB;
-            if ((e = split_update(p0, nbk, p0 - nn, nn, 0, p0 - nn)) != hipSuccess) return e;
-        }
+        if (p0 > 0 && (e = update(s, 0, p0, p0, nbk)) != hipSuccess) return e;
     }
-    return finish();
+    return hipSuccess;
 }
 
 hipError_t trsm_solve(int N, hipStream_t s, const TrsmArgs &a, const DV &X, const DV &CA, const DV &D, const Status *st) {

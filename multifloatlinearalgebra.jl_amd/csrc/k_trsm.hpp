@@ -14,11 +14,11 @@ struct TrsmArgs {
                       // trsm.jl:136-149, 159-172); 0: CA(p,q)*X(q) (left side)
     int64_t status_limit;  // 0: stop on any recorded failure; > 0: only on
                            // failures at (one-based) columns <= status_limit
-    // Look-ahead (blocked orders): with a second stream and two events, the
+    // Look-ahead (blocked orders): with a second stream and three events, the
     // update of the rows beyond the next diagonal block runs on `side` while
     // the next block is solved; null = everything on the caller's stream.
     hipStream_t side;
-    hipEvent_t ev_main, ev_side;
+    hipEvent_t ev_main, ev_side, ev_side2;
 };
 
 // Solves in place on the solution view X(p, c) with coefficient view CA(p, q)
