@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -952,6 +953,21 @@ static size_t noalloc_layout(int N, int64_t n, const mfla_ldlt_plan_t &plan, siz
     return *panel_off + panel_len;
 }
 
+// Rows below a blocked-LDLT panel per workgroup of its multi-workgroup
+// kernel. The environment variable MFLA_LDLT_PANEL_WORKGROUPS caps the
+// workgroups (0: the 512-thread single-workgroup kernel); every choice gives
+// the same bits.
+constexpr int64_t kLdltPanelRows = 192;
+constexpr int64_t kGridTimeoutInfo = -2;
+constexpr int64_t kPanelExtendedInfo = -3;
+constexpr int64_t kLdltSpecMaxDepth = 32;
+static int64_t ldlt_panel_workgroup_cap() {
+    const char *v = std::getenv("MFLA_LDLT_PANEL_WORKGROUPS");
+    if (!v || !*v) return 64;
+    const long long g = std::atoll(v);
+    return g < 0 ? 0 : (g > 64 ? 64 : g);
+}
+
 // ldlt! shared by mfla_sytrf and mfla_sytrf_noalloc. static_buf == nullptr:
 // the blocked route's weighted panel comes from the workspace (or is
 // allocated for the call) and original_maximum folds the column maxima on
@@ -1013,23 +1029,64 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
                 return fail(MFLA_ERR_HIP, "weighted panel allocation failed");
             own = true;
         }
+        // the panel kernel runs over one workgroup per kLdltPanelRows rows
+        // below the panel (at most the MFLA_LDLT_PANEL_WORKGROUPS cap) when
+        // the weighted-panel buffer holds its keep sets and partials
+        const bool grid_fits = mfk::ldlt_panel_grid_doubles(N, n) <= (size_t)N * (size_t)n * (size_t)(b + 1);
+        const int64_t gcap = ldlt_panel_workgroup_cap();
+        // The host launches the panels and their trailing updates ahead,
+        // assuming each panel ends at its requested last column, and reads the
+        // status after `depth` panels: a panel that extended (a 2x2 pivot at
+        // its last column) recorded kPanelExtendedInfo, so the kernels
+        // launched after it did nothing; the loop then resumes after it. The
+        // depth doubles after a read without extension (up to
+        // kLdltSpecMaxDepth) and drops to one after an extension.
+        auto trailing = [&](int64_t pf, int64_t pl) -> hipError_t {
+            if (pl >= n - 1) return hipSuccess;
+            const int64_t tfirst = pl + 1, tcount = n - pl - 1, width = pl - pf + 1;
+            const DV Wv{W, 1, tcount, tcount * (b + 1)};
+            hipError_t e = mfk::ldlt_weighted(N, s, a, pf, pl, tfirst, tcount, d, blocks, Wv, st);
+            if (e == hipSuccess)
+                e = lower_launch(h, N, c.gemm_tile, tcount, width, sub(a, tfirst, pf), DV{W, tcount, 1, tcount * (b + 1)},
+                                 sub(a, tfirst, tfirst), kMinusOne, kOne, st);
+            if (e == hipSuccess) e = mfk::mirror_lower(N, s, sub(a, tfirst, tfirst), tcount, st);
+            return e;
+        };
+        std::vector<std::pair<int64_t, int64_t>> pending;   // (pfirst, reqlast) since the last status read
+        int64_t depth = 4;
         int64_t pfirst = 0;
         while (pfirst < n) {
             const int64_t reqlast = std::min(pfirst + b, n) - 1;
-            HIPCHK(mfk::ldlt_panel(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, st));
-            int64_t plast = 0;
-            const int64_t r = read_info(h, &plast);
-            if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
-            if (r != 0) break;
-            if (plast < n - 1) {
-                const int64_t tfirst = plast + 1, tcount = n - plast - 1, width = plast - pfirst + 1;
-                const DV Wv{W, 1, tcount, tcount * (b + 1)};
-                HIPCHK(mfk::ldlt_weighted(N, s, a, pfirst, plast, tfirst, tcount, d, blocks, Wv, st));
-                HIPCHK(lower_launch(h, N, c.gemm_tile, tcount, width, sub(a, tfirst, pfirst), DV{W, tcount, 1, tcount * (b + 1)},
-                                    sub(a, tfirst, tfirst), kMinusOne, kOne, st));
-                HIPCHK(mfk::mirror_lower(N, s, sub(a, tfirst, tfirst), tcount, st));
+            if (grid_fits && gcap > 0) {
+                const int64_t G = std::max<int64_t>(1, std::min(gcap, mfk::cdiv(n - pfirst - 1, kLdltPanelRows)));
+                HIPCHK(mfk::ldlt_panel_grid(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, 1, st, (int)G));
+            } else {
+                HIPCHK(mfk::ldlt_panel(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, 1, st));
             }
-            pfirst = plast + 1;
+            pending.emplace_back(pfirst, reqlast);
+            HIPCHK(trailing(pfirst, reqlast));
+            pfirst = reqlast + 1;
+            if ((int64_t)pending.size() < depth && pfirst < n) continue;
+            int64_t aux = 0;
+            const int64_t r = read_info(h, &aux);
+            if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
+            if (r == kGridTimeoutInfo) return fail(MFLA_ERR_HIP, "ldlt! panel: a grid barrier timed out");
+            if (r == kPanelExtendedInfo) {
+                // aux = the extended panel's last column
+                int64_t pf = -1;
+                for (const auto &pr : pending)
+                    if (pr.second + 1 == aux) pf = pr.first;
+                if (pf < 0) return fail(MFLA_ERR_HIP, "ldlt! panel: extended panel not found");
+                HIPCHK(mfk::status_reset(s, st));
+                HIPCHK(trailing(pf, aux));
+                pfirst = aux + 1;
+                pending.clear();
+                depth = 1;
+                continue;
+            }
+            if (r != 0) break;
+            pending.clear();
+            depth = std::min(kLdltSpecMaxDepth, 2 * depth);
         }
         if (own) { (void)hipStreamSynchronize(s); (void)hipFree(W); }
     }

@@ -8,7 +8,7 @@ namespace mfk {
 
 // ----------------------------------------------------------- elementwise
 __global__ void k_status_reset(Status *st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) { st->info = 0; st->aux = 0; }
+    if (threadIdx.x == 0 && blockIdx.x == 0) { st->info = 0; st->aux = 0; st->sync = 0; st->sync_base = 0; }
 }
 
 template <int N>

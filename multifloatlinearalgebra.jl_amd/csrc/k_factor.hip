@@ -5,6 +5,7 @@
 // All kernels share the call's device Status word: they return immediately
 // once a previous kernel recorded a failure, so the matrix is left exactly as
 // MFLA leaves it when the factorization stops.
+#include <type_traits>
 #include "mf_common.hpp"
 #include "k_factor.hpp"
 #include "k_factor_dev.hpp"
@@ -335,7 +336,11 @@ __global__ __launch_bounds__(512) void k_ldlt_unblocked(DV A, int64_t n, DV dsub
 }
 
 // Lazy blocked panel (ldlt.jl:414-593). Writes info to st->info and the
-// (possibly extended) last panel column to st->aux.
+// (possibly extended) last panel column to st->aux. With `spec` (the host
+// launched the following panels assuming this one ends at reqlast), a panel
+// that extends by a 2x2 pivot records kPanelExtended instead of 0, so every
+// kernel launched after it returns at once and the host resumes after it.
+constexpr int64_t kPanelExtended = -3;
 //
 // A Schur entry (i1, i2) -> (row, col) = (max, min) is taken against the
 // pivots q = pfirst..k-1 of this panel in MFLA's order: a 1x1 pivot
@@ -599,7 +604,7 @@ struct LazyPanelEntries {
 
 template <int N>
 __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfirst, int64_t reqlast, DV dsub, int64_t *pivots,
-                                                    uint8_t *blocks, S<N> alpha_s, double *keepbuf, Status *st) {
+                                                    uint8_t *blocks, S<N> alpha_s, double *keepbuf, int spec, Status *st) {
     if (st->info != 0) return;
     constexpr int W = ldlt_pw<N>();
     __shared__ double sv_buf[512 * N];
@@ -682,7 +687,415 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
             k += 2;
         }
     }
-    if (threadIdx.x == 0) { st->info = info; st->aux = plast; }
+    if (threadIdx.x == 0) {
+        st->info = (info == 0 && spec && plast != reqlast) ? kPanelExtended : info;
+        st->aux = plast;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The lazy panel over G co-resident workgroups (one cooperative launch per
+// panel when G > 1). Every per-element operation is that of k_ldlt_panel;
+// the work is spread and the serial chain of a step kept short:
+//  * row threads (the first GP_T of a workgroup): thread g of the grid owns
+//    the rows pfirst + 1 + g + m * (G * GP_T) for the whole panel; the panel
+//    entries of its first row stay in registers between steps (reloaded after
+//    an interchange) and that row's next original entry is loaded while the
+//    grid barrier is pending;
+//  * a helper wave (the last 64 threads) forms entry(k, k) beside the column
+//    pass, and after the barrier the next pivot row's new entry;
+//  * between a workgroup's arrival at the barrier and its wait, the row
+//    threads divide their column values by entry(k, k): a step that takes a
+//    1x1 pivot without interchange (the usual case) then only stores (and the
+//    helper divides one value) after the barrier.
+// Per step: column pass and each workgroup's first strict maximum; barrier;
+// every workgroup reduces the partials in the same order and takes the same
+// Bunch-Kaufman decision (the first strict maximum is a total order on
+// (value, row), so any reduction order gives the same winner); the row-max
+// pass likewise when needed (one barrier); an interchange -- the row part,
+// the column part and the four crossing elements (composed as the row swap
+// followed by the column swap) in one pass -- and a barrier; the step's
+// columns on the owned rows. A step with an interchange ends with a barrier.
+// One without needs none: every value a workgroup reads in the next step
+// before its barrier is its own, was written before this step's barrier, or
+// is the next pivot row's new entry, which each workgroup forms and stores
+// itself (the same bits as the owner's) from the owner's column value,
+// published before this step's barrier. (Another row's newest entry is
+// written after that barrier: it is read only after the next one.)
+//
+// The barrier counts arrivals on st->sync (monotonic within a call;
+// st->sync_base holds its value when a launch starts); a wait is bounded: a
+// workgroup that waits ~1 s records kGridTimeout in st->info and leaves, and
+// the others then time out at their next barrier.
+constexpr int GP_T = 192;                 // row threads per workgroup
+constexpr int GP_THREADS = GP_T + 64;     // and the helper wave (one wave per SIMD)
+constexpr int GP_MAXG = 64;               // workgroups per launch (co-resident)
+constexpr int64_t kGridTimeout = -2;      // st->info after a barrier timed out
+
+__host__ __device__ constexpr size_t ldlt_grid_partials(int N) { return (size_t)GP_MAXG * (2 * (N + 1) + N + 1); }
+
+struct GridBarrier {
+    unsigned long long *ctr;
+    unsigned long long target, G;
+    int *ok;                              // LDS
+    // every thread: this workgroup's writes so far reach the device scope
+    __device__ void arrive() {
+        if (G > 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (G == 1) return;
+        target += G;
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // every thread: the other workgroups' writes before their arrival are seen
+    __device__ bool wait() {
+        if (G == 1) { __syncthreads(); return true; }
+        if (threadIdx.x == 0) {
+            int good = 1;
+            unsigned spins = 0;
+            while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1u << 20)) { good = 0; break; }
+            }
+            *ok = good;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return *ok != 0;
+    }
+    __device__ bool sync() { arrive(); return wait(); }
+};
+
+// workgroup max / min over the waves (butterfly in each wave, then the waves
+// in order); for finite values the result does not depend on the order
+template <int N>
+__device__ mf::V<N> waves_max(mf::V<N> v, double *sv_buf) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        mf::V<N> b;
+#pragma unroll
+        for (int l = 0; l < N; ++l) b.x[l] = __shfl_xor(v.x[l], off, 64);
+        v = mf::vmax(v, b);
+    }
+    if (lane == 0) lds_put<N>(&sv_buf[wave * N], v);
+    __syncthreads();
+    v = lds_get<N>(&sv_buf[0]);
+    for (int w = 1; w < nw; ++w) v = mf::vmax(v, lds_get<N>(&sv_buf[w * N]));
+    __syncthreads();
+    return v;
+}
+__device__ int64_t waves_min(int64_t v, int64_t *si_buf) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const int64_t b = __shfl_xor(v, off, 64);
+        v = b < v ? b : v;
+    }
+    if (lane == 0) si_buf[wave] = v;
+    __syncthreads();
+    v = si_buf[0];
+    for (int w = 1; w < nw; ++w) v = si_buf[w] < v ? si_buf[w] : v;
+    __syncthreads();
+    return v;
+}
+
+template <int N>
+__global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n, int64_t pfirst, int64_t reqlast, DV dsub, int64_t *pivots,
+                                                               uint8_t *blocks, S<N> alpha_s, double *keepbuf, double *part, int spec,
+                                                               Status *st) {
+    if (st->info != 0) return;
+    constexpr int W = ldlt_pw<N>();
+    constexpr int NW = GP_THREADS / 64;
+    __shared__ double sv_buf[NW * N];
+    __shared__ int64_t si_buf[NW];
+    __shared__ int kind[W];
+    __shared__ double dqq[W * N], dsq[W * N], coef[2 * W * N], rowv[W * N], er[W * N], ec[W * N];
+    __shared__ double s_akk[N];
+    __shared__ int bar_ok;
+    const mf::V<N> alpha = sv(alpha_s);
+    const int G = (int)gridDim.x, wg = (int)blockIdx.x;
+    const bool helper = threadIdx.x >= GP_T;
+    const int64_t GT = (int64_t)G * GP_T;
+    const int64_t gt = (int64_t)wg * GP_T + (helper ? 0 : threadIdx.x);
+    const int64_t gstart = helper ? n : gt;         // this thread's first index of a grid-strided pass
+    GridBarrier bar{&st->sync, st->sync_base, (unsigned long long)G, &bar_ok};
+    LazyPanelEntries<N> e{A, dsub, blocks, pfirst, pfirst, reqlast - pfirst + 2 <= W, kind, dqq, dsq, coef, rowv, er, ec, keepbuf, n};
+    // partials: column maxima (two sets, by step parity), row maxima, first failures
+    double *p_col = part, *p_row = part + 2 * GP_MAXG * (N + 1);
+    int64_t *p_fail = reinterpret_cast<int64_t *>(p_row + GP_MAXG * N);
+    // quotients and 2x2 solutions of the owned rows wait in sets 2 / 3 of the keep buffer
+    auto sol = [&](int64_t r, int set) {
+        mf::V<N> v;
+        for (int l = 0; l < N; ++l) v.x[l] = keepbuf[((2 + set) * N + l) * n + r];
+        return v;
+    };
+    auto put_sol = [&](int64_t r, int set, const mf::V<N> &a) {
+        for (int l = 0; l < N; ++l) keepbuf[((2 + set) * N + l) * n + r] = a.x[l];
+    };
+    // the first owned row (registers) and the further ones
+    const int64_t r0 = helper ? n : pfirst + 1 + gt;
+    auto more_rows = [&](int64_t after) {           // first owned row > after, beyond r0
+        if (helper) return n;
+        int64_t r = r0 + GT;
+        if (r <= after) r += ((after - r) / GT + 1) * GT;
+        return r;
+    };
+    auto first_row = [&](int64_t after) { return (r0 > after) ? r0 : more_rows(after); };
+    auto better = [](const mf::V<N> &b, int64_t ib, const mf::V<N> &a, int64_t ia) {
+        if (ib < 0) return false;
+        if (ia < 0) return true;
+        if (mf::gt(b, a)) return true;
+        return mf::eq(a, b) && ib < ia;
+    };
+    const bool regs = e.hoist && r0 < n;             // row r0's finalized panel entries live in xr
+    mf::V<N> xr[W];
+    bool xr_ok = false;                              // xr holds A(r0, pfirst + j), j < nt
+    mf::V<N> aorig;                                  // A(r0, k) when aorig_ok
+    bool aorig_ok = false;
+    int64_t plast = reqlast, k = pfirst, info = 0;
+    bool ok = true;
+    unsigned it = 0;
+    while (k <= plast) {
+        e.kk = k;
+        e.setup();
+        const int nt = (int)(k - pfirst);
+        // ---------------------------------------------- pivot choice
+        int bs;
+        int64_t piv = k;
+        mf::V<N> absakk;
+        mf::V<N> v0 = mf::zero<N>(), q0 = mf::zero<N>();
+        if (k == n - 1) {
+            absakk = mf::vabs(e.entry(k, k));
+            bs = mf::is_zero(absakk) ? 0 : 1;
+        } else {
+            e.prep(k);
+            mf::V<N> best = mf::zero<N>();
+            int64_t bi = -1;
+            if (!helper) {
+                if (r0 > k && r0 < n) {
+                    if (regs) {
+                        if (!xr_ok) {
+                            const int last = nt > 0 ? nt - 1 : 0;
+#pragma unroll
+                            for (int j = 0; j < W; ++j) xr[j] = ld<N>(A, r0, pfirst + (j < last ? j : last));
+                            xr_ok = true;
+                        }
+                        if (!aorig_ok) aorig = ld<N>(A, r0, k);
+                        v0 = e.reduce_coef(aorig, xr, 0);
+                    } else {
+                        v0 = e.centry(r0, k);
+                    }
+                    e.keep(r0, v0);
+                    best = mf::vabs(v0);
+                    bi = r0;
+                }
+                for (int64_t r = more_rows(k); r < n; r += GT) {
+                    const mf::V<N> v = e.centry(r, k);
+                    e.keep(r, v);
+                    const mf::V<N> c = mf::vabs(v);
+                    if (bi < 0 || mf::gt(c, best)) { best = c; bi = r; }
+                }
+            } else {
+                // entry(k, k) beside the column pass (row k's entries: its
+                // newest one is this workgroup's own store of the previous step)
+                const mf::V<N> d0 = e.diag_entry(k);
+                if (threadIdx.x == GP_T) lds_put<N>(s_akk, d0);
+            }
+            wg_argmax<N>(best, bi, sv_buf, si_buf);
+            double *pc = p_col + (it & 1) * GP_MAXG * (N + 1);
+            if (threadIdx.x == 0) {
+                for (int l = 0; l < N; ++l) pc[wg * (N + 1) + l] = best.x[l];
+                pc[wg * (N + 1) + N] = __builtin_bit_cast(double, bi);
+            }
+            bar.arrive();
+            e.akk = lds_get<N>(s_akk);
+            absakk = mf::vabs(e.akk);
+            // while the barrier is pending: the quotients of a 1x1 pivot k
+            // without interchange, and the next original entry of row r0
+            if (!helper) {
+                if (r0 > k && r0 < n) q0 = mf::div(v0, e.akk);
+                for (int64_t r = more_rows(k); r < n; r += GT) put_sol(r, 0, mf::div(e.kept(r), e.akk));
+                aorig_ok = false;
+                if (regs && r0 > k + 1) { aorig = ld<N>(A, r0, k + 1); aorig_ok = true; }
+            }
+            if (!(ok = bar.wait())) break;
+            mf::V<N> colmax = mf::zero<N>();
+            int64_t imax = -1;
+            for (int w = 0; w < G; ++w) {
+                mf::V<N> b;
+                for (int l = 0; l < N; ++l) b.x[l] = pc[w * (N + 1) + l];
+                const int64_t ib = __builtin_bit_cast(int64_t, pc[w * (N + 1) + N]);
+                if (better(b, ib, colmax, imax)) { colmax = b; imax = ib; }
+            }
+            if (mf::eq(mf::vmax(absakk, colmax), mf::zero<N>())) {
+                bs = 0;
+            } else if (mf::ge(absakk, mf::mul(alpha, colmax))) {
+                bs = 1;
+            } else {
+                mf::V<N> rm = mf::zero<N>();
+                e.prow(imax);
+                for (int64_t c = k + gstart; c < imax; c += GT) rm = mf::vmax(rm, mf::vabs(e.rentry(imax, c)));
+                e.prep(imax);
+                for (int64_t r = first_row(imax); r < n; r += GT) {
+                    const mf::V<N> v = e.centry(r, imax);
+                    e.keep_row(r, v);
+                    rm = mf::vmax(rm, mf::vabs(v));
+                }
+                rm = waves_max<N>(rm, sv_buf);
+                if (threadIdx.x == 0)
+                    for (int l = 0; l < N; ++l) p_row[wg * N + l] = rm.x[l];
+                if (!(ok = bar.sync())) break;
+                mf::V<N> rowmax = mf::zero<N>();
+                for (int w = 0; w < G; ++w) {
+                    mf::V<N> b;
+                    for (int l = 0; l < N; ++l) b.x[l] = p_row[w * N + l];
+                    rowmax = mf::vmax(rowmax, b);
+                }
+                if (mf::ge(absakk, mf::mul(mf::mul(alpha, colmax), mf::div(colmax, rowmax)))) {
+                    bs = 1;
+                } else if (mf::ge(mf::vabs(e.entry(imax, imax)), mf::mul(alpha, rowmax))) {
+                    bs = 1;
+                    piv = imax;
+                } else {
+                    bs = 2;
+                    piv = imax;
+                }
+            }
+        }
+        ++it;
+        if (bs == 0) { info = k + 1; break; }
+        if (bs == 2 && k == plast) {
+            if (!(k < n - 1)) { info = k + 1; break; }
+            plast += 1;
+        }
+        // ---------------------------------------------- interchange
+        const bool swapped = piv != (bs == 1 ? k : k + 1);
+        if (swapped) {
+            const int64_t f = bs == 1 ? k : k + 1, s = piv;
+            for (int64_t c = gstart; c < n; c += GT)
+                if (c != f && c != s) swap_el<N>(A, f, c, s, c);
+            for (int64_t r = k + gstart; r < n; r += GT)
+                if (r != f && r != s) swap_el<N>(A, r, f, r, s);
+            if (gt == 0 && !helper) {
+                const mf::V<N> ff = ld<N>(A, f, f), fs = ld<N>(A, f, s), sf = ld<N>(A, s, f), ss = ld<N>(A, s, s);
+                stv<N>(A, f, f, ss);
+                stv<N>(A, f, s, sf);
+                stv<N>(A, s, f, fs);
+                stv<N>(A, s, s, ff);
+            }
+            if (!(ok = bar.sync())) break;
+            if (bs == 1) e.rowk = false;
+        }
+        if (threadIdx.x == 0) {
+            pivots[k] = piv + 1;
+            blocks[k] = (uint8_t)bs;
+            if (bs == 2) blocks[k + 1] = 0;
+        }
+        // ---------------------------------------------- the step's columns
+        if (bs == 1 && !swapped) {
+            // the quotients formed while the barrier was pending
+            const mf::V<N> d = e.akk;
+            if (mf::is_zero(d)) { info = k + 1; break; }
+            if (!helper) {
+                if (r0 > k && r0 < n) {
+                    stv<N>(A, r0, k, q0);
+                    if (regs) {
+#pragma unroll
+                        for (int j = 0; j < W; ++j)
+                            if (j == nt) xr[j] = q0;
+                    }
+                }
+                for (int64_t r = more_rows(k); r < n; r += GT) stv<N>(A, r, k, sol(r, 0));
+            }
+            if (helper) {
+                // the next pivot row's new entry from the owner's column value
+                // (published before the barrier): the same bits as the owner's
+                const mf::V<N> q1 = k + 1 < n ? mf::div(e.kept(k + 1), d) : mf::zero<N>();
+                if (threadIdx.x == GP_T) {
+                    stv<N>(A, k, k, d);
+                    if (k + 1 < n) stv<N>(A, k + 1, k, q1);
+                }
+            }
+            __syncthreads();
+            k += 1;
+        } else if (bs == 1) {
+            const mf::V<N> d = e.entry(k, k);
+            if (mf::is_zero(d)) { info = k + 1; break; }
+            e.prep(k, 0);
+            for (int64_t r = first_row(k); r < n; r += GT)
+                stv<N>(A, r, k, mf::div(r > piv ? e.kept(r, 1) : e.centry(r, k, 0), d));
+            if (!(ok = bar.sync())) break;
+            if (threadIdx.x == 0) stv<N>(A, k, k, d);
+            xr_ok = false;
+            aorig_ok = false;
+            __syncthreads();
+            k += 1;
+        } else {
+            const int64_t p = piv;
+            const mf::V<N> d11 = e.akk, d21 = e.entry(k + 1, k), d22 = e.entry(k + 1, k + 1);
+            e.prep(k + 1, 1);
+            mf::V<N> s1, s2;
+            if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) {
+                if (!(ok = bar.sync())) break;
+                if (threadIdx.x == 0) { stv<N>(A, k, k, d11); stv<N>(A, k + 1, k + 1, d22); stv<N>(dsub, k, 0, d21); }
+                info = k + 1;
+                break;
+            }
+            // entry(r, k) of a row r >= k + 2 is the column pass's value (of
+            // row k + 1 for r = p), entry(r, k + 1) of a row below p the
+            // row-max pass's; the solutions wait in sets 2 / 3 until the first
+            // failure is known
+            int64_t myfail = INT64_MAX;
+            for (int64_t r = first_row(k + 1); r < n; r += GT) {
+                const mf::V<N> fv = e.kept(r == p ? k + 1 : r, 0);
+                const mf::V<N> sv2 = r > p ? e.kept(r, 1) : e.centry(r, k + 1, 1);
+                mf::V<N> x1, x2;
+                const bool good = solve2x2<N>(d11, d21, d22, fv, sv2, x1, x2);
+                if (!(good && mf::is_finite(x1) && mf::is_finite(x2)) && r < myfail) myfail = r;
+                put_sol(r, 0, x1);
+                put_sol(r, 1, x2);
+            }
+            // the next pivot row's solution, formed by every workgroup's helper
+            mf::V<N> n1 = mf::zero<N>(), n2 = mf::zero<N>();
+            if (helper && k + 2 < n) {
+                const int64_t r = k + 2;
+                const mf::V<N> fv = e.kept(r == p ? k + 1 : r, 0);
+                const mf::V<N> sv2 = r > p ? e.kept(r, 1) : e.centry(r, k + 1, 1);
+                (void)solve2x2<N>(d11, d21, d22, fv, sv2, n1, n2);
+            }
+            myfail = waves_min(myfail, si_buf);
+            if (threadIdx.x == 0) p_fail[wg] = myfail;
+            if (!(ok = bar.sync())) break;
+            int64_t firstfail = INT64_MAX;
+            for (int w = 0; w < G; ++w) firstfail = p_fail[w] < firstfail ? p_fail[w] : firstfail;
+            for (int64_t r = first_row(k + 1); r < n && r < firstfail; r += GT) {
+                stv<N>(A, r, k, sol(r, 0));
+                stv<N>(A, r, k + 1, sol(r, 1));
+            }
+            if (threadIdx.x == 0) { stv<N>(A, k, k, d11); stv<N>(A, k + 1, k + 1, d22); stv<N>(dsub, k, 0, d21); }
+            if (firstfail != INT64_MAX) { info = k + 1; break; }
+            if (threadIdx.x == 0) {
+                stv<N>(A, k + 1, k, mf::zero<N>());
+                stv<N>(A, k, k + 1, mf::zero<N>());
+            }
+            if (threadIdx.x == GP_T && k + 2 < n) { stv<N>(A, k + 2, k, n1); stv<N>(A, k + 2, k + 1, n2); }
+            xr_ok = false;
+            aorig_ok = false;
+            __syncthreads();
+            k += 2;
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (!ok) {
+            __hip_atomic_exchange(&st->info, kGridTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (wg == 0) {
+            int64_t expect = 0;
+            const int64_t code = (info == 0 && spec && plast != reqlast) ? kPanelExtended : info;
+            __hip_atomic_compare_exchange_strong(&st->info, &expect, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            st->aux = plast;
+            st->sync_base = bar.target;
+        }
+    }
 }
 
 // ldlt.jl:595-632: weighted panel W(r, local) = L21 * D for the trailing rows.
@@ -888,10 +1301,38 @@ hipError_t ldlt_unblocked(int N, hipStream_t s, const DV &A, int64_t n, const DV
 }
 
 hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
-                      uint8_t *blocks, const double *alpha, double *keepbuf, Status *st) {
+                      uint8_t *blocks, const double *alpha, double *keepbuf, int spec, Status *st) {
     MF_DISPATCH(N, hipLaunchKernelGGL(k_ldlt_panel<NN>, dim3(1), dim3(512), 0, s, A, n, pfirst, reqlast, dsub, pivots, blocks,
-                                      mk<NN>(alpha), keepbuf, st));
+                                      mk<NN>(alpha), keepbuf, spec, st));
     return hipGetLastError();
+}
+
+size_t ldlt_panel_grid_doubles(int N, int64_t n) { return 4 * (size_t)N * (size_t)n + ldlt_grid_partials(N); }
+
+hipError_t ldlt_panel_grid(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
+                           uint8_t *blocks, const double *alpha, double *buf, int spec, Status *st, int workgroups) {
+    if (workgroups < 1 || workgroups > GP_MAXG) return hipErrorInvalidValue;
+    double *part = buf + 4 * (size_t)N * (size_t)n;
+    auto launch = [&](auto ntag) -> hipError_t {
+        constexpr int NN = decltype(ntag)::value;
+        DV a = A, d = dsub;
+        int64_t n_ = n, pf = pfirst, rl = reqlast;
+        S<NN> al = mk<NN>(alpha);
+        int sp = spec;
+        void *args[] = {&a, &n_, &pf, &rl, &d, &pivots, &blocks, &al, &buf, &part, &sp, &st};
+        if (workgroups == 1) {
+            hipLaunchKernelGGL(k_ldlt_panel_grid<NN>, dim3(1), dim3(GP_THREADS), 0, s, a, n_, pf, rl, d, pivots, blocks, al, buf, part, sp, st);
+            return hipGetLastError();
+        }
+        return hipLaunchCooperativeKernel((const void *)k_ldlt_panel_grid<NN>, dim3((unsigned)workgroups), dim3(GP_THREADS), args, 0, s);
+    };
+    switch (N) {
+        case 1: return launch(std::integral_constant<int, 1>());
+        case 2: return launch(std::integral_constant<int, 2>());
+        case 3: return launch(std::integral_constant<int, 3>());
+        case 4: return launch(std::integral_constant<int, 4>());
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t ldlt_weighted(int N, hipStream_t s, const DV &A, int64_t pfirst, int64_t plast, int64_t tfirst, int64_t tcount,

@@ -44,6 +44,8 @@ template <int N> __device__ __forceinline__ mf::V<N> sv(const S<N> &s) {
 struct Status {
     int64_t info;   // MFLA status code (0, -1, k > 0)
     int64_t aux;    // kernel-private scratch (pivot index, flags)
+    unsigned long long sync;        // grid-barrier arrivals of the multi-workgroup kernels of a call
+    unsigned long long sync_base;   // the arrivals before the next such launch
 };
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -273,6 +273,83 @@ def test_ldlt_blocked_failure_state_bitwise(M, oracle, N, zero):
     assert_bitwise(host(Kd), ref, f"failed blocked ldlt x{N} zero column {zero}")
 
 
+def _ldlt_with_panel_workgroups(M, K, cap, **cfg):
+    """ldlt! with the panel's workgroup cap set through the environment (0:
+    the 512-thread single-workgroup panel kernel)."""
+    import os
+
+    old = os.environ.get("MFLA_LDLT_PANEL_WORKGROUPS")
+    os.environ["MFLA_LDLT_PANEL_WORKGROUPS"] = str(cap)
+    try:
+        Kd = dev(K)
+        F = M.ldlt_(Kd, check=False, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_blocked_crossover=1, **cfg))
+        torch.cuda.synchronize()
+    finally:
+        if old is None:
+            del os.environ["MFLA_LDLT_PANEL_WORKGROUPS"]
+        else:
+            os.environ["MFLA_LDLT_PANEL_WORKGROUPS"] = old
+    return F, host(Kd)
+
+
+def _assert_same_ldlt(Fa, Ka, Fb, Kb, what):
+    assert Fa.info == Fb.info, what
+    assert np.array_equal(Fa.pivots.cpu().numpy(), Fb.pivots.cpu().numpy()), f"{what}: pivots"
+    assert np.array_equal(Fa.blocks.cpu().numpy(), Fb.blocks.cpu().numpy()), f"{what}: blocks"
+    assert_bitwise(host(Fa.dsub), host(Fb.dsub), f"{what}: dsub")
+    assert_bitwise(Ka, Kb, f"{what}: matrix")
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("kind,n,block", [("indefinite", 900, 0), ("indefinite", 700, 40), ("dominant", 1000, 0),
+                                          ("indefinite", 2600, 0)])
+def test_ldlt_panel_over_workgroups_equals_one_workgroup(M, N, kind, n, block):
+    """The multi-workgroup panel kernel (rows below the panel spread over up
+    to ceil(rows / 256) workgroups, grid barriers between the phases of a
+    step; one workgroup under a cap of 1, several rows per thread under a cap
+    of 2) against the 512-thread single-workgroup kernel, bitwise:
+    interchanges and 2x2 pivots (indefinite), 1x1 steps without interchange
+    (dominant), the per-element entries of a panel wider than its LDS slots
+    (block 40), and 11 workgroups (n = 2600)."""
+    if n > 1000 and N != 2:
+        pytest.skip("the 11-workgroup case runs for two limbs")
+    rng = np.random.default_rng(n + 13 * N + block)
+    H = rand_mf(rng, N, n, n)
+    K = renorm(H + np.transpose(H, (0, 2, 1)))
+    if kind == "dominant":
+        K[0, np.arange(n), np.arange(n)] += 4.0 * n
+    F0, K0 = _ldlt_with_panel_workgroups(M, K, 0, ldlt_block=block)
+    assert F0.info == 0
+    if kind == "indefinite":
+        assert (F0.blocks.cpu().numpy() == 2).any()
+    for cap in (64, 1, 2):
+        Fg, Kg = _ldlt_with_panel_workgroups(M, K, cap, ldlt_block=block)
+        _assert_same_ldlt(Fg, Kg, F0, K0, f"ldlt x{N} n={n} {kind} b={block} cap {cap}")
+
+
+@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("zero", [150, 301])
+def test_ldlt_panel_over_workgroups_failure_state(M, oracle, N, zero):
+    """A zero row / column reached inside a panel that runs over several
+    workgroups: info, pivots, blocks, dsub and the matrix are MFLA's (the
+    oracle) and those of the single-workgroup panel."""
+    rng = np.random.default_rng(zero + 7 * N)
+    K = kkt(rng, N, 420)
+    n = K.shape[1]
+    K[:, zero, :] = 0.0
+    K[:, :, zero] = 0.0
+    ref = K.copy()
+    info_ref, dsub_ref, piv_ref, blk_ref, _ = oracle.sytrf(ref, "blocked", 0, 1, threads=8)
+    assert info_ref > 0
+    Fg, Kg = _ldlt_with_panel_workgroups(M, K, 64)
+    F1, K1 = _ldlt_with_panel_workgroups(M, K, 0)
+    assert Fg.info == info_ref
+    assert np.array_equal(Fg.pivots.cpu().numpy(), piv_ref) and np.array_equal(Fg.blocks.cpu().numpy(), blk_ref)
+    assert_bitwise(host(Fg.dsub), dsub_ref, "dsub")
+    assert_bitwise(Kg, ref, f"failed blocked ldlt x{N} n={n} zero column {zero}")
+    _assert_same_ldlt(Fg, Kg, F1, K1, f"failed ldlt x{N} zero {zero}")
+
+
 def test_status_codes_and_check(M):
     A = torch.zeros((2, 3, 3), dtype=torch.float64, device="cuda")
     A[0, 1, 2] = float("nan")                       # (row 2, col 1): lower triangle
