@@ -1034,6 +1034,10 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
         // the weighted-panel buffer holds its keep sets and partials
         const bool grid_fits = mfk::ldlt_panel_grid_doubles(N, n) <= (size_t)N * (size_t)n * (size_t)(b + 1);
         const int64_t gcap = ldlt_panel_workgroup_cap();
+        // The panels work on the lower triangle only (their interchanges move
+        // lower-triangle elements, as MFLA's full swap does on a symmetric
+        // block) and write row k's upper part when column k freezes, so the
+        // trailing block is never mirrored.
         // The host launches the panels and their trailing updates ahead,
         // assuming each panel ends at its requested last column, and reads the
         // status after `depth` panels: a panel that extended (a 2x2 pivot at
@@ -1041,15 +1045,15 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
         // launched after it did nothing; the loop then resumes after it. The
         // depth doubles after a read without extension (up to
         // kLdltSpecMaxDepth) and drops to one after an extension.
+        const bool use_grid = grid_fits && gcap > 0;     // its panels also write the weighted rows
         auto trailing = [&](int64_t pf, int64_t pl) -> hipError_t {
             if (pl >= n - 1) return hipSuccess;
             const int64_t tfirst = pl + 1, tcount = n - pl - 1, width = pl - pf + 1;
             const DV Wv{W, 1, tcount, tcount * (b + 1)};
-            hipError_t e = mfk::ldlt_weighted(N, s, a, pf, pl, tfirst, tcount, d, blocks, Wv, st);
+            hipError_t e = use_grid ? hipSuccess : mfk::ldlt_weighted(N, s, a, pf, pl, tfirst, tcount, d, blocks, Wv, st);
             if (e == hipSuccess)
                 e = lower_launch(h, N, c.gemm_tile, tcount, width, sub(a, tfirst, pf), DV{W, tcount, 1, tcount * (b + 1)},
                                  sub(a, tfirst, tfirst), kMinusOne, kOne, st);
-            if (e == hipSuccess) e = mfk::mirror_lower(N, s, sub(a, tfirst, tfirst), tcount, st);
             return e;
         };
         std::vector<std::pair<int64_t, int64_t>> pending;   // (pfirst, reqlast) since the last status read
@@ -1057,9 +1061,9 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
         int64_t pfirst = 0;
         while (pfirst < n) {
             const int64_t reqlast = std::min(pfirst + b, n) - 1;
-            if (grid_fits && gcap > 0) {
+            if (use_grid) {
                 const int64_t G = std::max<int64_t>(1, std::min(gcap, mfk::cdiv(n - pfirst - 1, kLdltPanelRows)));
-                HIPCHK(mfk::ldlt_panel_grid(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, 1, st, (int)G));
+                HIPCHK(mfk::ldlt_panel_grid(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, 1, b + 1, st, (int)G));
             } else {
                 HIPCHK(mfk::ldlt_panel(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, 1, st));
             }
@@ -1083,6 +1087,13 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
                 pending.clear();
                 depth = 1;
                 continue;
+            }
+            if (r > 0) {
+                // the panels keep the lower triangle and write each row's upper
+                // part when it freezes; the block from the first unfrozen row on
+                // is symmetric in MFLA's matrix at the failure
+                const int64_t m = std::min(aux, n);
+                HIPCHK(mfk::mirror_lower(N, s, sub(a, m, m), n - m, nullptr));
             }
             if (r != 0) break;
             pending.clear();

@@ -615,22 +615,29 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
     LazyPanelEntries<N> e{A, dsub, blocks, pfirst, pfirst, reqlast - pfirst + 2 <= W, kind, dqq, dsq, coef, rowv, er, ec, keepbuf, n};
     int64_t plast = reqlast;
     int64_t k = pfirst;
-    int64_t info = 0;
+    int64_t info = 0, frozen = 0;
+    // the upper triangle of row k: the symmetric block's row as it stands
+    // after the step's interchange (MFLA's upper row k is final from then on)
+    auto freeze_row = [&](int64_t row, int64_t from) {
+        for (int64_t r = from + threadIdx.x; r < n; r += blockDim.x) stv<N>(A, row, r, ld<N>(A, r, row));
+    };
     while (k <= plast) {
         e.kk = k;
         e.setup();
         BKChoice ch = wg_select_bk<N>(n, k, alpha, e, sv_buf, si_buf);
-        if (ch.bs == 0) { info = k + 1; break; }
+        if (ch.bs == 0) { freeze_row(k, k + 1); info = k + 1; frozen = k + 1; break; }
         if (ch.bs == 2 && k == plast) {
-            if (!(k < n - 1)) { info = k + 1; break; }
+            if (!(k < n - 1)) { freeze_row(k, k + 1); info = k + 1; frozen = k + 1; break; }
             plast += 1;
         }
         if (ch.bs == 1) {
             if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 1; }
             __syncthreads();
-            wg_sym_swap<N>(A, n, k, ch.pivot, k);
+            wg_sym_swap_lower<N>(A, n, k, ch.pivot);
             if (ch.pivot != k) e.rowk = false;       // row k now holds the pivot row
+            freeze_row(k, k + 1);                    // each thread's rows (it stores them below)
             const mf::V<N> d = ch.pivot == k ? e.akk : e.entry(k, k);
+            frozen = k + 1;
             if (mf::is_zero(d)) { info = k + 1; break; }
             if (ch.pivot == k) {
                 // no interchange: column k is what the column pass saw (each
@@ -651,7 +658,11 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
         } else {
             if (threadIdx.x == 0) { pivots[k] = ch.pivot + 1; blocks[k] = 2; blocks[k + 1] = 0; }
             __syncthreads();
-            wg_sym_swap<N>(A, n, k + 1, ch.pivot, k);
+            wg_sym_swap_lower<N>(A, n, k + 1, ch.pivot);
+            freeze_row(k, k + 2);
+            freeze_row(k + 1, k + 2);
+            if (threadIdx.x == 0) stv<N>(A, k, k + 1, ld<N>(A, k + 1, k));
+            frozen = k + 2;
             // the interchange of k + 1 and the pivot leaves row k, so d11 is
             // the entry the pivot choice formed
             const mf::V<N> d11 = e.akk, d21 = e.entry(k + 1, k), d22 = e.entry(k + 1, k + 1);
@@ -689,7 +700,7 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
     }
     if (threadIdx.x == 0) {
         st->info = (info == 0 && spec && plast != reqlast) ? kPanelExtended : info;
-        st->aux = plast;
+        st->aux = info > 0 ? frozen : plast;
     }
 }
 
@@ -802,7 +813,7 @@ __device__ int64_t waves_min(int64_t v, int64_t *si_buf) {
 template <int N>
 __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n, int64_t pfirst, int64_t reqlast, DV dsub, int64_t *pivots,
                                                                uint8_t *blocks, S<N> alpha_s, double *keepbuf, double *part, int spec,
-                                                               Status *st) {
+                                                               int64_t wld, Status *st) {
     if (st->info != 0) return;
     constexpr int W = ldlt_pw<N>();
     constexpr int NW = GP_THREADS / 64;
@@ -853,6 +864,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
     mf::V<N> aorig;                                  // A(r0, k) when aorig_ok
     bool aorig_ok = false;
     int64_t plast = reqlast, k = pfirst, info = 0;
+    int64_t frozen = 0;                              // after a failure: rows before it have their upper row written
     bool ok = true;
     unsigned it = 0;
     while (k <= plast) {
@@ -881,8 +893,10 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                             xr_ok = true;
                         }
                         if (!aorig_ok) aorig = ld<N>(A, r0, k);
+                        stv<N>(A, k, r0, aorig);
                         v0 = e.reduce_coef(aorig, xr, 0);
                     } else {
+                        stv<N>(A, k, r0, ld<N>(A, r0, k));
                         v0 = e.centry(r0, k);
                     }
                     e.keep(r0, v0);
@@ -890,6 +904,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                     bi = r0;
                 }
                 for (int64_t r = more_rows(k); r < n; r += GT) {
+                    stv<N>(A, k, r, ld<N>(A, r, k));
                     const mf::V<N> v = e.centry(r, k);
                     e.keep(r, v);
                     const mf::V<N> c = mf::vabs(v);
@@ -963,26 +978,22 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             }
         }
         ++it;
-        if (bs == 0) { info = k + 1; break; }
+        if (bs == 0) { info = k + 1; frozen = k + (bs == 2 ? 2 : 1); break; }
         if (bs == 2 && k == plast) {
-            if (!(k < n - 1)) { info = k + 1; break; }
+            if (!(k < n - 1)) { info = k + 1; frozen = k + (bs == 2 ? 2 : 1); break; }
             plast += 1;
         }
         // ---------------------------------------------- interchange
         const bool swapped = piv != (bs == 1 ? k : k + 1);
         if (swapped) {
+            // the lower triangle's part of the symmetric interchange (rows f, s
+            // left of f; (r, f) <-> (s, r) for f < r < s; columns f, s below s;
+            // the two diagonals): disjoint pairs, one pass
             const int64_t f = bs == 1 ? k : k + 1, s = piv;
-            for (int64_t c = gstart; c < n; c += GT)
-                if (c != f && c != s) swap_el<N>(A, f, c, s, c);
-            for (int64_t r = k + gstart; r < n; r += GT)
-                if (r != f && r != s) swap_el<N>(A, r, f, r, s);
-            if (gt == 0 && !helper) {
-                const mf::V<N> ff = ld<N>(A, f, f), fs = ld<N>(A, f, s), sf = ld<N>(A, s, f), ss = ld<N>(A, s, s);
-                stv<N>(A, f, f, ss);
-                stv<N>(A, f, s, sf);
-                stv<N>(A, s, f, fs);
-                stv<N>(A, s, s, ff);
-            }
+            for (int64_t c = gstart; c < f; c += GT) swap_el<N>(A, f, c, s, c);
+            for (int64_t r = f + 1 + gstart; r < s; r += GT) swap_el<N>(A, r, f, s, r);
+            for (int64_t r = s + 1 + gstart; r < n; r += GT) swap_el<N>(A, r, f, r, s);
+            if (gt == 0 && !helper) swap_el<N>(A, f, f, s, s);
             if (!(ok = bar.sync())) break;
             if (bs == 1) e.rowk = false;
         }
@@ -995,7 +1006,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
         if (bs == 1 && !swapped) {
             // the quotients formed while the barrier was pending
             const mf::V<N> d = e.akk;
-            if (mf::is_zero(d)) { info = k + 1; break; }
+            if (mf::is_zero(d)) { info = k + 1; frozen = k + (bs == 2 ? 2 : 1); break; }
             if (!helper) {
                 if (r0 > k && r0 < n) {
                     stv<N>(A, r0, k, q0);
@@ -1019,8 +1030,9 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             __syncthreads();
             k += 1;
         } else if (bs == 1) {
+            for (int64_t r = first_row(k); r < n; r += GT) stv<N>(A, k, r, ld<N>(A, r, k));
             const mf::V<N> d = e.entry(k, k);
-            if (mf::is_zero(d)) { info = k + 1; break; }
+            if (mf::is_zero(d)) { info = k + 1; frozen = k + (bs == 2 ? 2 : 1); break; }
             e.prep(k, 0);
             for (int64_t r = first_row(k); r < n; r += GT)
                 stv<N>(A, r, k, mf::div(r > piv ? e.kept(r, 1) : e.centry(r, k, 0), d));
@@ -1032,6 +1044,11 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             k += 1;
         } else {
             const int64_t p = piv;
+            for (int64_t r = first_row(k + 1); r < n; r += GT) {
+                stv<N>(A, k, r, ld<N>(A, r, k));
+                stv<N>(A, k + 1, r, ld<N>(A, r, k + 1));
+            }
+            if (threadIdx.x == 0) stv<N>(A, k, k + 1, ld<N>(A, k + 1, k));
             const mf::V<N> d11 = e.akk, d21 = e.entry(k + 1, k), d22 = e.entry(k + 1, k + 1);
             e.prep(k + 1, 1);
             mf::V<N> s1, s2;
@@ -1039,6 +1056,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 if (!(ok = bar.sync())) break;
                 if (threadIdx.x == 0) { stv<N>(A, k, k, d11); stv<N>(A, k + 1, k + 1, d22); stv<N>(dsub, k, 0, d21); }
                 info = k + 1;
+                frozen = k + 2;
                 break;
             }
             // entry(r, k) of a row r >= k + 2 is the column pass's value (of
@@ -1073,7 +1091,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 stv<N>(A, r, k + 1, sol(r, 1));
             }
             if (threadIdx.x == 0) { stv<N>(A, k, k, d11); stv<N>(A, k + 1, k + 1, d22); stv<N>(dsub, k, 0, d21); }
-            if (firstfail != INT64_MAX) { info = k + 1; break; }
+            if (firstfail != INT64_MAX) { info = k + 1; frozen = k + (bs == 2 ? 2 : 1); break; }
             if (threadIdx.x == 0) {
                 stv<N>(A, k + 1, k, mf::zero<N>());
                 stv<N>(A, k, k + 1, mf::zero<N>());
@@ -1085,6 +1103,31 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             k += 2;
         }
     }
+    // wld > 0: the weighted panel W(r - plast - 1, q - pfirst) = L21 D of the
+    // rows below the panel (ldlt.jl:595-632, the operations of
+    // k_ldlt_weighted), limb stride (n - plast - 1) * wld, written over the
+    // keep buffer once every workgroup is past its last read of it
+    if (ok && info == 0 && wld > 0 && plast < n - 1 && (ok = bar.sync())) {
+        const int64_t tcount = n - plast - 1;
+        const DV Wv{keepbuf, 1, tcount, tcount * wld};
+        for (int64_t r = first_row(plast); r < n; r += GT) {
+            const int64_t lr = r - plast - 1;
+            int64_t q = pfirst;
+            while (q <= plast) {
+                const int64_t lc = q - pfirst;
+                if (blocks[q] == 1) {
+                    stv<N>(Wv, lr, lc, mf::mul(ld<N>(A, r, q), ld<N>(A, q, q)));
+                    q += 1;
+                } else {
+                    const mf::V<N> d11 = ld<N>(A, q, q), d21 = ld<N>(dsub, q, 0), d22 = ld<N>(A, q + 1, q + 1);
+                    const mf::V<N> f = ld<N>(A, r, q), s = ld<N>(A, r, q + 1);
+                    stv<N>(Wv, lr, lc, mf::add(mf::mul(f, d11), mf::mul(s, d21)));
+                    stv<N>(Wv, lr, lc + 1, mf::add(mf::mul(f, d21), mf::mul(s, d22)));
+                    q += 2;
+                }
+            }
+        }
+    }
     if (threadIdx.x == 0) {
         if (!ok) {
             __hip_atomic_exchange(&st->info, kGridTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1092,7 +1135,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             int64_t expect = 0;
             const int64_t code = (info == 0 && spec && plast != reqlast) ? kPanelExtended : info;
             __hip_atomic_compare_exchange_strong(&st->info, &expect, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            st->aux = plast;
+            st->aux = info > 0 ? frozen : plast;
             st->sync_base = bar.target;
         }
     }
@@ -1310,7 +1353,8 @@ hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfir
 size_t ldlt_panel_grid_doubles(int N, int64_t n) { return 4 * (size_t)N * (size_t)n + ldlt_grid_partials(N); }
 
 hipError_t ldlt_panel_grid(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
-                           uint8_t *blocks, const double *alpha, double *buf, int spec, Status *st, int workgroups) {
+                           uint8_t *blocks, const double *alpha, double *buf, int spec, int64_t weighted_ld, Status *st,
+                           int workgroups) {
     if (workgroups < 1 || workgroups > GP_MAXG) return hipErrorInvalidValue;
     double *part = buf + 4 * (size_t)N * (size_t)n;
     auto launch = [&](auto ntag) -> hipError_t {
@@ -1319,9 +1363,11 @@ hipError_t ldlt_panel_grid(int N, hipStream_t s, const DV &A, int64_t n, int64_t
         int64_t n_ = n, pf = pfirst, rl = reqlast;
         S<NN> al = mk<NN>(alpha);
         int sp = spec;
-        void *args[] = {&a, &n_, &pf, &rl, &d, &pivots, &blocks, &al, &buf, &part, &sp, &st};
+        int64_t wl = weighted_ld;
+        void *args[] = {&a, &n_, &pf, &rl, &d, &pivots, &blocks, &al, &buf, &part, &sp, &wl, &st};
         if (workgroups == 1) {
-            hipLaunchKernelGGL(k_ldlt_panel_grid<NN>, dim3(1), dim3(GP_THREADS), 0, s, a, n_, pf, rl, d, pivots, blocks, al, buf, part, sp, st);
+            hipLaunchKernelGGL(k_ldlt_panel_grid<NN>, dim3(1), dim3(GP_THREADS), 0, s, a, n_, pf, rl, d, pivots, blocks, al, buf, part, sp, wl,
+                               st);
             return hipGetLastError();
         }
         return hipLaunchCooperativeKernel((const void *)k_ldlt_panel_grid<NN>, dim3((unsigned)workgroups), dim3(GP_THREADS), args, 0, s);

@@ -21,13 +21,16 @@ hipError_t ldlt_unblocked(int N, hipStream_t s, const DV &A, int64_t n, const DV
 // and its last column in st->aux.
 hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
                       uint8_t *blocks, const double *alpha, double *keepbuf, int spec, Status *st);
-// The lazy panel over `workgroups` (2..64) co-resident workgroups: the same
+// The lazy panel over `workgroups` (1..64) co-resident workgroups: the same
 // bits as ldlt_panel. buf holds ldlt_panel_grid_doubles(N, n) doubles (the
-// column / row-max values of four sets and the per-workgroup partials).
-// st->info = -2 if a grid barrier timed out.
+// column / row-max values of four sets and the per-workgroup partials);
+// weighted_ld > 0: at the end the panel's weighted rows W = L21 D overwrite
+// buf as ldlt_weighted would write them with W's limb stride
+// (n - plast - 1) * weighted_ld. st->info = -2 if a grid barrier timed out.
 size_t ldlt_panel_grid_doubles(int N, int64_t n);
 hipError_t ldlt_panel_grid(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
-                           uint8_t *blocks, const double *alpha, double *buf, int spec, Status *st, int workgroups);
+                           uint8_t *blocks, const double *alpha, double *buf, int spec, int64_t weighted_ld, Status *st,
+                           int workgroups);
 hipError_t ldlt_weighted(int N, hipStream_t s, const DV &A, int64_t pfirst, int64_t plast, int64_t tfirst, int64_t tcount,
                          const DV &dsub, const uint8_t *blocks, const DV &W, const Status *st);
 hipError_t mirror_lower(int N, hipStream_t s, const DV &A, int64_t n, const Status *st);
