@@ -705,8 +705,7 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
 }
 
 // ---------------------------------------------------------------------------
-// The lazy panel over G co-resident workgroups (one cooperative launch per
-// panel when G > 1). Every per-element operation is that of k_ldlt_panel;
+// The lazy panel over G co-resident workgroups (one launch per panel). Every per-element operation is that of k_ldlt_panel;
 // the work is spread and the serial chain of a step kept short:
 //  * row threads (the first GP_T of a workgroup): thread g of the grid owns
 //    the rows pfirst + 1 + g + m * (G * GP_T) for the whole panel; the panel
@@ -821,7 +820,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
     __shared__ int64_t si_buf[NW];
     __shared__ int kind[W];
     __shared__ double dqq[W * N], dsq[W * N], coef[2 * W * N], rowv[W * N], er[W * N], ec[W * N];
-    __shared__ double s_akk[N];
+    __shared__ double s_akk[N], s_dpp[N];
     __shared__ int bar_ok;
     const mf::V<N> alpha = sv(alpha_s);
     const int G = (int)gridDim.x, wg = (int)blockIdx.x;
@@ -876,6 +875,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
         int64_t piv = k;
         mf::V<N> absakk;
         mf::V<N> v0 = mf::zero<N>(), q0 = mf::zero<N>();
+        mf::V<N> dpp = mf::zero<N>();                // entry(imax, imax) when the row-max pass ran
         if (k == n - 1) {
             absakk = mf::vabs(e.entry(k, k));
             bs = mf::is_zero(absakk) ? 0 : 1;
@@ -949,6 +949,14 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             } else {
                 mf::V<N> rm = mf::zero<N>();
                 e.prow(imax);
+                // the helper forms entry(imax, imax) from the staged row (the
+                // operations of entry()) while the row threads run the pass
+                if (helper) {
+                    const mf::V<N> v = e.hoist ? e.reduce(ld<N>(A, imax, imax), [&](int j) { return lds_get<N>(&rowv[j * N]); },
+                                                          [&](int j) { return lds_get<N>(&rowv[j * N]); })
+                                               : e.slow(imax, imax);
+                    if (threadIdx.x == GP_T) lds_put<N>(s_dpp, v);
+                }
                 for (int64_t c = k + gstart; c < imax; c += GT) rm = mf::vmax(rm, mf::vabs(e.rentry(imax, c)));
                 e.prep(imax);
                 for (int64_t r = first_row(imax); r < n; r += GT) {
@@ -966,9 +974,10 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                     for (int l = 0; l < N; ++l) b.x[l] = p_row[w * N + l];
                     rowmax = mf::vmax(rowmax, b);
                 }
+                dpp = lds_get<N>(s_dpp);
                 if (mf::ge(absakk, mf::mul(mf::mul(alpha, colmax), mf::div(colmax, rowmax)))) {
                     bs = 1;
-                } else if (mf::ge(mf::vabs(e.entry(imax, imax)), mf::mul(alpha, rowmax))) {
+                } else if (mf::ge(mf::vabs(dpp), mf::mul(alpha, rowmax))) {
                     bs = 1;
                     piv = imax;
                 } else {
@@ -985,6 +994,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
         }
         // ---------------------------------------------- interchange
         const bool swapped = piv != (bs == 1 ? k : k + 1);
+        Prepared2x2<N> pr2;
         if (swapped) {
             // the lower triangle's part of the symmetric interchange (rows f, s
             // left of f; (r, f) <-> (s, r) for f < r < s; columns f, s below s;
@@ -994,8 +1004,15 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             for (int64_t r = f + 1 + gstart; r < s; r += GT) swap_el<N>(A, r, f, s, r);
             for (int64_t r = s + 1 + gstart; r < n; r += GT) swap_el<N>(A, r, f, r, s);
             if (gt == 0 && !helper) swap_el<N>(A, f, f, s, s);
-            if (!(ok = bar.sync())) break;
+            bar.arrive();
+            // the 2x2 pivot block's own part of the solves while the barrier
+            // is pending: after the interchange entry(k + 1, k) is the column
+            // pass's value of row piv and entry(k + 1, k + 1) is entry(piv, piv)
+            if (bs == 2) pr2 = prepare2x2<N>(e.akk, e.kept(piv, 0), dpp);
+            if (!(ok = bar.wait())) break;
             if (bs == 1) e.rowk = false;
+        } else if (bs == 2) {
+            pr2 = prepare2x2<N>(e.akk, e.kept(piv, 0), dpp);
         }
         if (threadIdx.x == 0) {
             pivots[k] = piv + 1;
@@ -1031,7 +1048,9 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             k += 1;
         } else if (bs == 1) {
             for (int64_t r = first_row(k); r < n; r += GT) stv<N>(A, k, r, ld<N>(A, r, k));
-            const mf::V<N> d = e.entry(k, k);
+            // entry(k, k) after the interchange is entry(piv, piv) before it
+            // (row k now holds row piv's entries and diagonal)
+            const mf::V<N> d = dpp;
             if (mf::is_zero(d)) { info = k + 1; frozen = k + (bs == 2 ? 2 : 1); break; }
             e.prep(k, 0);
             for (int64_t r = first_row(k); r < n; r += GT)
@@ -1049,10 +1068,12 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 stv<N>(A, k + 1, r, ld<N>(A, r, k + 1));
             }
             if (threadIdx.x == 0) stv<N>(A, k, k + 1, ld<N>(A, k + 1, k));
-            const mf::V<N> d11 = e.akk, d21 = e.entry(k + 1, k), d22 = e.entry(k + 1, k + 1);
+            // after the interchange of k + 1 and p, entry(k + 1, k) is the
+            // column pass's value of row p and entry(k + 1, k + 1) is
+            // entry(p, p) (row k + 1 holds row p's entries and diagonal)
+            const mf::V<N> d11 = e.akk, d21 = e.kept(p, 0), d22 = dpp;
             e.prep(k + 1, 1);
-            mf::V<N> s1, s2;
-            if (!solve2x2<N>(d11, d21, d22, mf::zero<N>(), mf::zero<N>(), s1, s2)) {
+            if (!pr2.ok) {
                 if (!(ok = bar.sync())) break;
                 if (threadIdx.x == 0) { stv<N>(A, k, k, d11); stv<N>(A, k + 1, k + 1, d22); stv<N>(dsub, k, 0, d21); }
                 info = k + 1;
@@ -1068,8 +1089,8 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 const mf::V<N> fv = e.kept(r == p ? k + 1 : r, 0);
                 const mf::V<N> sv2 = r > p ? e.kept(r, 1) : e.centry(r, k + 1, 1);
                 mf::V<N> x1, x2;
-                const bool good = solve2x2<N>(d11, d21, d22, fv, sv2, x1, x2);
-                if (!(good && mf::is_finite(x1) && mf::is_finite(x2)) && r < myfail) myfail = r;
+                apply2x2<N>(pr2, fv, sv2, x1, x2);
+                if (!(mf::is_finite(x1) && mf::is_finite(x2)) && r < myfail) myfail = r;
                 put_sol(r, 0, x1);
                 put_sol(r, 1, x2);
             }
@@ -1079,7 +1100,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 const int64_t r = k + 2;
                 const mf::V<N> fv = e.kept(r == p ? k + 1 : r, 0);
                 const mf::V<N> sv2 = r > p ? e.kept(r, 1) : e.centry(r, k + 1, 1);
-                (void)solve2x2<N>(d11, d21, d22, fv, sv2, n1, n2);
+                apply2x2<N>(pr2, fv, sv2, n1, n2);
             }
             myfail = waves_min(myfail, si_buf);
             if (threadIdx.x == 0) p_fail[wg] = myfail;
@@ -1364,13 +1385,13 @@ hipError_t ldlt_panel_grid(int N, hipStream_t s, const DV &A, int64_t n, int64_t
         S<NN> al = mk<NN>(alpha);
         int sp = spec;
         int64_t wl = weighted_ld;
-        void *args[] = {&a, &n_, &pf, &rl, &d, &pivots, &blocks, &al, &buf, &part, &sp, &wl, &st};
-        if (workgroups == 1) {
-            hipLaunchKernelGGL(k_ldlt_panel_grid<NN>, dim3(1), dim3(GP_THREADS), 0, s, a, n_, pf, rl, d, pivots, blocks, al, buf, part, sp, wl,
-                               st);
-            return hipGetLastError();
-        }
-        return hipLaunchCooperativeKernel((const void *)k_ldlt_panel_grid<NN>, dim3((unsigned)workgroups), dim3(GP_THREADS), args, 0, s);
+        // an ordinary launch: the panel's workgroups (at most 64, one per CU)
+        // are resident together on a device that runs nothing else meanwhile
+        // (a cooperative launch costs ~11 us before and after the kernel);
+        // should they not be, the bounded barrier ends the call with an error
+        hipLaunchKernelGGL(k_ldlt_panel_grid<NN>, dim3((unsigned)workgroups), dim3(GP_THREADS), 0, s, a, n_, pf, rl, d, pivots, blocks, al, buf,
+                           part, sp, wl, st);
+        return hipGetLastError();
     };
     switch (N) {
         case 1: return launch(std::integral_constant<int, 1>());

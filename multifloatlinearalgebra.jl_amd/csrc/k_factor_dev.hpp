@@ -117,6 +117,49 @@ __device__ bool solve2x2(const mf::V<N> &d11, const mf::V<N> &d21, const mf::V<N
     return true;
 }
 
+// solve2x2 split into the part that depends on the pivot block only
+// (prepared once per step) and the part of one right-hand side; together
+// they perform solve2x2's operations in its order, so the bits are equal.
+template <int N>
+struct Prepared2x2 {
+    bool ok, first;                     // first: |a| >= |b|
+    mf::V<N> scale, a, b, c, t, u;
+};
+template <int N>
+__device__ Prepared2x2<N> prepare2x2(const mf::V<N> &d11, const mf::V<N> &d21, const mf::V<N> &d22) {
+    Prepared2x2<N> p;
+    p.scale = mf::vmax(mf::vmax(mf::vabs(d11), mf::vabs(d21)), mf::vabs(d22));
+    p.ok = false;
+    p.first = true;
+    if (mf::is_zero(p.scale) || !mf::is_finite(p.scale)) return p;
+    p.a = mf::div(d11, p.scale);
+    p.b = mf::div(d21, p.scale);
+    p.c = mf::div(d22, p.scale);
+    p.first = mf::ge(mf::vabs(p.a), mf::vabs(p.b));
+    if (p.first) {
+        if (mf::is_zero(p.a)) return p;
+        p.t = mf::div(p.b, p.a);
+        p.u = mf::msub(p.c, p.t, p.b);
+    } else {
+        if (mf::is_zero(p.b)) return p;
+        p.t = mf::div(p.a, p.b);
+        p.u = mf::msub(p.b, p.t, p.c);
+    }
+    p.ok = !(mf::is_zero(p.u) || !mf::is_finite(p.u));
+    return p;
+}
+template <int N>
+__device__ void apply2x2(const Prepared2x2<N> &p, const mf::V<N> &f, const mf::V<N> &s, mf::V<N> &x1, mf::V<N> &x2) {
+    const mf::V<N> r1 = mf::div(f, p.scale), r2 = mf::div(s, p.scale);
+    if (p.first) {
+        x2 = mf::div(mf::msub(r2, p.t, r1), p.u);
+        x1 = mf::div(mf::msub(r1, p.b, x2), p.a);
+    } else {
+        x2 = mf::div(mf::msub(r1, p.t, r2), p.u);
+        x1 = mf::div(mf::msub(r2, p.c, x2), p.b);
+    }
+}
+
 // ldlt.jl:155-181, executed cooperatively by one workgroup.
 template <int N>
 __device__ void wg_sym_swap(const DV &A, int64_t n, int64_t f, int64_t s, int64_t active) {

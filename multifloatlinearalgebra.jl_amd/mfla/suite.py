@@ -125,18 +125,45 @@ def case_trsm(N, n, nrhs, order="blocked", reps=1):
 
 
 def case_factor(kind, N, n):
-    A0 = spd_dev(N, n, 6) if kind != "lu" else rand_dev(N, n, n, 7)
+    """One factorization of a seeded matrix: SPD R R^T + n I (cholesky,
+    ldlt), random (lu), or the symmetric indefinite R + R^T (ldlt_indefinite:
+    interchanges and 2x2 pivots in most panels)."""
+    if kind == "lu":
+        A0 = rand_dev(N, n, n, 7)
+    elif kind == "ldlt_indefinite":
+        R = rand_dev(N, n, n, 9)
+        A0 = (R + R.transpose(1, 2)).contiguous()
+    else:
+        A0 = spd_dev(N, n, 6)
     A = A0.clone()
-    small = A0[:, :64, :64].contiguous()
-    fn = {"cholesky": M.cholesky_, "lu": M.lu_, "ldlt": M.ldlt_}[kind]
+    # kernel loading on a small matrix of the same routine (for LDLT an SPD
+    # one: the unblocked route on a slice of the indefinite matrix is held
+    # off the GPU)
+    small = (A0 if kind != "ldlt_indefinite" else spd_dev(N, 64, 6))[:, :64, :64].contiguous()
+    fn = {"cholesky": M.cholesky_, "lu": M.lu_, "ldlt": M.ldlt_, "ldlt_indefinite": M.ldlt_}[kind]
     fn(small.clone(), check=False)           # kernel loading
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fn(A, check=False)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    macs = {"cholesky": n**3 // 6, "lu": n**3 // 3, "ldlt": n**3 // 6}[kind]
+    macs = {"cholesky": n**3 // 6, "lu": n**3 // 3, "ldlt": n**3 // 6, "ldlt_indefinite": n**3 // 6}[kind]
     return _compute(f"{kind} n={n}", N, macs, t)
+
+
+def case_potrs(N, n, nrhs, reps=2):
+    """ldiv!(cholesky(S), B): the forward (bitwise MFLA order) and the blocked
+    backward triangular solves; work = 2 nrhs n(n-1)/2 MACs."""
+    F = M.cholesky_(spd_dev(N, n, 6))
+    B0 = rand_dev(N, n, nrhs, 12)
+    B = B0.clone()
+    M.ldiv_(B, F)
+
+    def run():
+        B.copy_(B0)
+        M.ldiv_(B, F)
+    t = _time(run, reps)
+    return _compute(f"potrs n={n} nrhs={nrhs}", N, nrhs * n * (n - 1), t)
 
 
 def case_qr(N, m, n, kind="rrqr"):
@@ -258,6 +285,11 @@ def case_autodiff(batch=32, n=1024, nrhs=32, N=4):
 def run(which=("blas3", "factor", "blas2", "refine", "batched", "autodiff"), out=None):
     rows = []
 
+    def go(fn, *args, **kw):
+        # the case is named before it runs (a failure then names it)
+        print(f"# starting {fn.__name__}{args}{kw if kw else ''}", flush=True)
+        emit(fn(*args, **kw))
+
     def emit(r):
         rows.append(r)
         line = json.dumps(r)
@@ -268,19 +300,20 @@ def run(which=("blas3", "factor", "blas2", "refine", "batched", "autodiff"), out
 
     if "blas3" in which:
         for N in (2, 3, 4):
-            emit(case_gemm(N, 2048, 2048, 2048))
-            emit(case_gemm(N, 4096, 4096, 4096, reps=1))
-            emit(case_gemm(N, 8192, 512, 8192, reps=1))
-            emit(case_syrk(N, 4096, 1024, reps=1))
-            emit(case_trsm(N, 4096, 256))
+            go(case_gemm, N, 2048, 2048, 2048)
+            go(case_gemm, N, 4096, 4096, 4096, reps=1)
+            go(case_gemm, N, 8192, 512, 8192, reps=1)
+            go(case_syrk, N, 4096, 1024, reps=1)
+            go(case_trsm, N, 4096, 256)
     if "factor" in which:
-        for kind in ("cholesky", "lu", "ldlt"):
-            emit(case_factor(kind, 4, 4096 if kind != "ldlt" else 2048))
+        for kind in ("cholesky", "lu", "ldlt", "ldlt_indefinite"):
+            go(case_factor, kind, 4, 4096 if kind in ("cholesky", "lu") else 2048)
+        go(case_potrs, 4, 4096, 64)
     if "blas2" in which:
         for N in (2, 3, 4):
-            emit(case_gemv(N, 4096))
-            emit(case_trsv(N, 4096))
-            emit(case_residual(N, 4096, 64))
+            go(case_gemv, N, 4096)
+            go(case_trsv, N, 4096)
+            go(case_residual, N, 4096, 64)
     if "refine" in which:
         from .refine import bench_refine
         for kind in ("cholesky", "lu"):
@@ -289,7 +322,7 @@ def run(which=("blas3", "factor", "blas2", "refine", "batched", "autodiff"), out
             emit(r)
     if "ozaki" in which:
         for N in (2, 3, 4):
-            emit(case_gemm(N, 4096, 4096, 4096, reps=2, strategy="ozaki"))
+            go(case_gemm, N, 4096, 4096, 4096, reps=2, strategy="ozaki")
         from .refine import bench_refine
         for kind in ("cholesky", "lu"):
             r = bench_refine(kind=kind, residual="ozaki")
@@ -297,12 +330,12 @@ def run(which=("blas3", "factor", "blas2", "refine", "batched", "autodiff"), out
             emit(r)
     if "qr" in which:
         for N in (2, 4):
-            emit(case_qr(N, 500, 40))
-            emit(case_qr(N, 1024, 256))
-        emit(case_qr(4, 1024, 1024))
-        emit(case_qr(4, 1024, 1024, kind="qr"))
+            go(case_qr, N, 500, 40)
+            go(case_qr, N, 1024, 256)
+        go(case_qr, 4, 1024, 1024)
+        go(case_qr, 4, 1024, 1024, kind="qr")
     if "batched" in which:
-        emit(case_batched())
+        go(case_batched)
     if "autodiff" in which:
-        emit(case_autodiff())
+        go(case_autodiff)
     return rows
