@@ -912,11 +912,22 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 }
             } else {
                 // entry(k, k) beside the column pass (row k's entries: its
-                // newest one is this workgroup's own store of the previous step)
-                const mf::V<N> d0 = e.diag_entry(k);
+                // newest one is this workgroup's own store of the previous
+                // step). With row k staged, coef(j) = dqq(j) * A(k, q) is the
+                // inner product of entry()'s terms, so the chain takes row k
+                // from registers and the prepared factors like the row threads'
+                mf::V<N> d0;
+                if (e.hoist) {
+                    mf::V<N> xk[W];
+#pragma unroll
+                    for (int j = 0; j < W; ++j) xk[j] = lds_get<N>(&er[j * N]);
+                    d0 = e.reduce_coef(ld<N>(A, k, k), xk, 0);
+                } else {
+                    d0 = e.diag_entry(k);
+                }
                 if (threadIdx.x == GP_T) lds_put<N>(s_akk, d0);
             }
-            wg_argmax<N>(best, bi, sv_buf, si_buf);
+            wg_argmax_lead<N>(best, bi, sv_buf, si_buf);
             double *pc = p_col + (it & 1) * GP_MAXG * (N + 1);
             if (threadIdx.x == 0) {
                 for (int l = 0; l < N; ++l) pc[wg * (N + 1) + l] = best.x[l];

@@ -89,6 +89,60 @@ __device__ int64_t wg_min_i64(int64_t v, int64_t *si_buf) {
     return r;
 }
 
+// wg_argmax with a quicker wave step for the usual case: when a wave's
+// candidates have finite leading limbs and exactly one of them holds the
+// largest, that lane is the wave's winner under the same rule (every other
+// candidate is smaller in the leading limb); otherwise the full butterfly
+// runs. A wave without candidates keeps idx < 0. Same result as wg_argmax.
+template <int N>
+__device__ void wg_argmax_lead(mf::V<N> &val, int64_t &idx, double *sv_buf, int64_t *si_buf) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = (int)(blockDim.x >> 6);
+    auto better = [](const mf::V<N> &b, int64_t ib, const mf::V<N> &a, int64_t ia) {
+        if (ib < 0) return false;
+        if (ia < 0) return true;
+        if (mf::gt(b, a)) return true;
+        return mf::eq(a, b) && ib < ia;
+    };
+    const bool cand = idx >= 0;
+    const double h = val.x[0];
+    const unsigned long long cands = __ballot(cand);
+    if (cands != 0) {
+        double m = cand ? h : -__builtin_inf();
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double o = __shfl_xor(m, off, 64);
+            m = o > m ? o : m;
+        }
+        const unsigned long long top = __ballot(cand && h == m);
+        const bool nan = __ballot(cand && !(h == h)) != 0;
+        if (!nan && __popcll(top) == 1) {
+            const int w = __ffsll((long long)top) - 1;
+#pragma unroll
+            for (int l = 0; l < N; ++l) val.x[l] = __shfl(val.x[l], w, 64);
+            idx = __shfl(idx, w, 64);
+        } else {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                mf::V<N> b;
+#pragma unroll
+                for (int l = 0; l < N; ++l) b.x[l] = __shfl_xor(val.x[l], off, 64);
+                const int64_t ib = __shfl_xor(idx, off, 64);
+                if (better(b, ib, val, idx)) { val = b; idx = ib; }
+            }
+        }
+    }
+    if (lane == 0) { lds_put<N>(&sv_buf[wave * N], val); si_buf[wave] = idx; }
+    __syncthreads();
+    val = lds_get<N>(&sv_buf[0]);
+    idx = si_buf[0];
+    for (int w = 1; w < nw; ++w) {
+        const mf::V<N> b = lds_get<N>(&sv_buf[w * N]);
+        const int64_t ib = si_buf[w];
+        if (better(b, ib, val, idx)) { val = b; idx = ib; }
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------------------------------ LDLT
 // ldlt.jl:65-101 (scaled 2x2 elimination)
 template <int N>
