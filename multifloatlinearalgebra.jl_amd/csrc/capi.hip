@@ -40,6 +40,10 @@ struct mfla_handle_s {
     std::mutex lock;            // calls on one handle are serialised
     hipStream_t side = nullptr; // second stream of the look-ahead factorization
     hipEvent_t ev_join = nullptr, ev_panel = nullptr, ev_update = nullptr, ev_update2 = nullptr;
+    // CU-partitioned streams of ldlt!'s look-ahead (created on first use):
+    // panels on la_cus CUs, trailing updates on the others
+    hipStream_t la_panel = nullptr, la_update = nullptr;
+    int la_cus = 0;
     // pinned host staging for the small device-to-host reads a call waits on
     // (status, per-iteration column maxima): one DMA transfer each instead of
     // a copy through the runtime's pageable staging; null = pageable path
@@ -52,7 +56,7 @@ struct mfla_workspace_s {
     int limbs = 0;
     int64_t factor_capacity = 0;
     int64_t block_capacity = 0;
-    double *weighted = nullptr;  // limbs * factor_capacity * (block_capacity + 1)
+    double *weighted = nullptr;  // three parts of limbs * factor_capacity * (block_capacity + 1)
 };
 
 // Device scratch of the no-allocation entry point (mfla_sytrf_noalloc): a
@@ -155,11 +159,12 @@ void *grow_scratch(mfla_handle h, size_t bytes) {
     return h->scratch;
 }
 
-int64_t read_info(mfla_handle h, int64_t *aux = nullptr) {
+int64_t read_info(mfla_handle h, int64_t *aux = nullptr, hipStream_t on = nullptr) {
     Status local{};
     Status *st = h->pinned ? reinterpret_cast<Status *>(h->pinned) : &local;
-    if (hipMemcpyAsync(st, h->status, sizeof(Status), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return INT64_MIN;
-    if (hipStreamSynchronize(h->stream) != hipSuccess) return INT64_MIN;
+    hipStream_t q = on ? on : h->stream;
+    if (hipMemcpyAsync(st, h->status, sizeof(Status), hipMemcpyDeviceToHost, q) != hipSuccess) return INT64_MIN;
+    if (hipStreamSynchronize(q) != hipSuccess) return INT64_MIN;
     if (aux) *aux = st->aux;
     return st->info;
 }
@@ -233,6 +238,48 @@ hipError_t ensure_side(mfla_handle h) {
     if ((e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&h->ev_update2, hipEventDisableTiming)) != hipSuccess) return e;
     return hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming);
+}
+
+// The CU-partitioned streams of ldlt!'s look-ahead (default priority):
+// panel_cus CUs for the panels, the device's other CUs for the trailing
+// updates, so a panel's latency-bound waves never share a SIMD with the
+// update's. Mask bit C x + 8 y + x (XCD x < 8, y < panel_cus / 8, C = CUs /
+// 8) gives each of the eight XCDs panel_cus / 8 of them whether the driver
+// maps mask bits to XCDs in blocks of C or in turn, so every XCD runs work
+// of both streams (a kernel's workgroups are dealt to the XCDs in turn);
+// panel_cus <= C keeps every bit inside its XCD's block. Fails (and the
+// caller runs without look-ahead) on a device without eight XCDs of at
+// least 16 CUs.
+hipError_t ensure_la_streams(mfla_handle h, int panel_cus) {
+    if (h->la_panel && h->la_cus == panel_cus) return hipSuccess;
+    int cus = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e != hipSuccess) return e;
+    const int C = cus / 8;
+    if (cus % 8 != 0 || C < 16 || panel_cus % 8 != 0 || panel_cus < 8 || panel_cus > C || cus > 1024) return hipErrorNotSupported;
+    for (hipStream_t *q : {&h->la_panel, &h->la_update})
+        if (*q) { (void)hipStreamSynchronize(*q); (void)hipStreamDestroy(*q); *q = nullptr; }
+    h->la_cus = 0;
+    uint32_t pm[32] = {}, um[32] = {};
+    for (int x = 0; x < 8; ++x)
+        for (int y = 0; y < panel_cus / 8; ++y) {
+            const int bit = C * x + 8 * y + x;
+            pm[bit / 32] |= 1u << (bit % 32);
+        }
+    const uint32_t words = (uint32_t)((cus + 31) / 32);
+    for (uint32_t w = 0; w < words; ++w) {
+        const int valid = std::min(32, cus - (int)w * 32);
+        const uint32_t all = valid == 32 ? 0xffffffffu : ((1u << valid) - 1);
+        um[w] = all & ~pm[w];
+    }
+    if ((e = hipExtStreamCreateWithCUMask(&h->la_panel, words, pm)) != hipSuccess) { h->la_panel = nullptr; return e; }
+    if ((e = hipExtStreamCreateWithCUMask(&h->la_update, words, um)) != hipSuccess) {
+        (void)hipStreamDestroy(h->la_panel);
+        h->la_panel = h->la_update = nullptr;
+        return e;
+    }
+    h->la_cus = panel_cus;
+    return hipSuccess;
 }
 
 // Ozaki-split gemm: C = alpha*A*B + beta*C with A (m x k) and B (k x n)
@@ -434,6 +481,8 @@ int mfla_handle_destroy(mfla_handle h) {
         (void)hipEventDestroy(h->ev_update);
         (void)hipEventDestroy(h->ev_update2);
     }
+    for (hipStream_t q : {h->la_panel, h->la_update})
+        if (q) { (void)hipStreamSynchronize(q); (void)hipStreamDestroy(q); }
     delete h;
     return MFLA_OK;
 }
@@ -482,7 +531,9 @@ int mfla_workspace_ensure(mfla_workspace w, int64_t factor_capacity, int64_t ldl
     const int64_t f = std::max(w->factor_capacity, factor_capacity);
     const int64_t b = std::max(w->block_capacity, ldlt_block_capacity);
     if (f == w->factor_capacity && b == w->block_capacity) return MFLA_OK;
-    const size_t bytes = sizeof(double) * (size_t)w->limbs * (size_t)f * (size_t)(b > 0 ? b + 1 : 0);
+    // three parts: ldlt!'s look-ahead keeps the two previous panels'
+    // weighted rows while the next panel runs
+    const size_t bytes = 3 * sizeof(double) * (size_t)w->limbs * (size_t)f * (size_t)(b > 0 ? b + 1 : 0);
     double *p = nullptr;
     if (bytes > 0 && hipMalloc(&p, bytes) != hipSuccess) return fail(MFLA_ERR_HIP, "workspace allocation failed");
     if (w->weighted) { (void)hipStreamSynchronize(w->h->stream); (void)hipFree(w->weighted); }
@@ -959,13 +1010,24 @@ static size_t noalloc_layout(int N, int64_t n, const mfla_ldlt_plan_t &plan, siz
 // the same bits.
 constexpr int64_t kLdltPanelRows = 192;
 constexpr int64_t kGridTimeoutInfo = -2;
-constexpr int64_t kPanelExtendedInfo = -3;
 constexpr int64_t kLdltSpecMaxDepth = 32;
 static int64_t ldlt_panel_workgroup_cap() {
     const char *v = std::getenv("MFLA_LDLT_PANEL_WORKGROUPS");
     if (!v || !*v) return 64;
     const long long g = std::atoll(v);
     return g < 0 ? 0 : (g > 64 ? 64 : g);
+}
+// The environment variable MFLA_LDLT_LOOKAHEAD: 0 never, 1 whenever the
+// call can, otherwise from kLdltLookaheadMinN on (the bits never change).
+constexpr int64_t kLdltLookaheadMinN = 1536;
+// CUs of the look-ahead's panel stream (four per XCD; the panels use at most
+// 32 workgroups there, and the first part of each update runs beside them)
+constexpr int kLdltPanelCUs = 32;
+static int ldlt_lookahead_mode() {
+    const char *v = std::getenv("MFLA_LDLT_LOOKAHEAD");
+    if (v && v[0] == '0' && !v[1]) return 0;
+    if (v && v[0] == '1' && !v[1]) return 1;
+    return 2;
 }
 
 // ldlt! shared by mfla_sytrf and mfla_sytrf_noalloc. static_buf == nullptr:
@@ -1020,70 +1082,184 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
     if (!blocked) {
         HIPCHK(mfk::ldlt_unblocked(N, s, a, n, d, pivots, blocks, alpha, st));
     } else {
+        // one part: the weighted panel (limbs * n * (block + 1) doubles),
+        // which the multi-workgroup panels also use for their keep sets and
+        // partials; the look-ahead rotates three parts
+        const size_t half = (size_t)N * (size_t)n * (size_t)(b + 1);
         double *W = nullptr;
+        size_t avail = 0;
         bool own = false;
-        if (static_buf) W = static_buf + panel_off;
-        else if (ws && ws->limbs == N && ws->factor_capacity >= n && ws->block_capacity >= b) W = ws->weighted;
-        else {
-            if (hipMalloc(&W, sizeof(double) * (size_t)N * (size_t)n * (size_t)(b + 1)) != hipSuccess)
-                return fail(MFLA_ERR_HIP, "weighted panel allocation failed");
-            own = true;
+        if (static_buf) {
+            W = static_buf + panel_off;
+            avail = static_doubles - panel_off;
+        } else if (ws && ws->limbs == N && ws->factor_capacity >= n && ws->block_capacity >= b) {
+            W = ws->weighted;
+            avail = 3 * (size_t)N * (size_t)ws->factor_capacity * (size_t)(ws->block_capacity + 1);
         }
         // the panel kernel runs over one workgroup per kLdltPanelRows rows
         // below the panel (at most the MFLA_LDLT_PANEL_WORKGROUPS cap) when
         // the weighted-panel buffer holds its keep sets and partials
-        const bool grid_fits = mfk::ldlt_panel_grid_doubles(N, n) <= (size_t)N * (size_t)n * (size_t)(b + 1);
+        const bool grid_fits = mfk::ldlt_panel_grid_doubles(N, n) <= half;
         const int64_t gcap = ldlt_panel_workgroup_cap();
+        const bool use_grid = grid_fits && gcap > 0;     // its panels also write the weighted rows
+        auto workgroups = [&](int64_t pf) { return std::max<int64_t>(1, std::min(gcap, mfk::cdiv(n - pf - 1, kLdltPanelRows))); };
+        // Look-ahead: while it splits, the panels run on a stream confined to
+        // a few CUs and each panel's trailing update is cut in three: the
+        // next panel's columns (the panel stream, before the next panel), the
+        // columns of the panel after it and the rest (a second stream over
+        // the other CUs, beside the next panel). The next panel's columns then
+        // wait only for the second part of the previous update, done long
+        // before. Every element still receives the panels' updates in
+        // ascending order with the same arithmetic, so the bits do not change.
+        // The next panel must not read past its columns meanwhile: it runs in
+        // truncation mode and stops at its first column that needs the
+        // row-max pass (a truncation stop); the host then restores the panel's
+        // columns and runs it again after the updates, and splits no update
+        // for the next la_hold panels, which run with their updates on the
+        // call's stream as without look-ahead. The weighted panels rotate over
+        // three buffers (panel j + 3 reuses panel j's after panel j's last
+        // part). Not in the no-allocation call (creating the streams allocates).
+        const int lam = ldlt_lookahead_mode();
+        const int64_t g0 = use_grid ? workgroups(0) : 0;
+        bool la = use_grid && !static_buf && lam != 0 && (lam == 1 || n >= kLdltLookaheadMinN) && g0 <= 32 && n > 2 * b;
+        if (la) {
+            if (ensure_side(h) != hipSuccess || ensure_la_streams(h, kLdltPanelCUs) != hipSuccess) { (void)hipGetLastError(); la = false; }
+        }
+        if (!W) {
+            const size_t len = (la ? 3 : 1) * half;
+            if (hipMalloc(&W, sizeof(double) * len) != hipSuccess) return fail(MFLA_ERR_HIP, "weighted panel allocation failed");
+            own = true;
+            avail = len;
+        }
+        if (avail < 3 * half) la = false;
+        hipStream_t ps = s, us = s;                      // while splitting: panels, trailing updates
+        // `to` waits for the work launched on `from` so far
+        auto order = [&](hipStream_t from, hipStream_t to, hipEvent_t ev) -> hipError_t {
+            if (from == to) return hipSuccess;
+            const hipError_t e = hipEventRecord(ev, from);
+            return e == hipSuccess ? hipStreamWaitEvent(to, ev, 0) : e;
+        };
+        // work launched on ps / us since the call's stream last waited for it
+        bool ps_open = false, us_open = false;
+        // the call's stream after every kernel launched so far
+        auto gather = [&]() -> hipError_t {
+            hipError_t e = hipSuccess;
+            if (ps_open && (e = order(ps, s, h->ev_join)) != hipSuccess) return e;
+            if (us_open && (e = order(us, s, h->ev_update2)) != hipSuccess) return e;
+            ps_open = us_open = false;
+            return e;
+        };
+        if (la) {
+            ps = h->la_panel;
+            us = h->la_update;
+            HIPCHK(order(s, ps, h->ev_join));
+            HIPCHK(order(s, us, h->ev_update2));
+        }
         // The panels work on the lower triangle only (their interchanges move
         // lower-triangle elements, as MFLA's full swap does on a symmetric
         // block) and write row k's upper part when column k freezes, so the
-        // trailing block is never mirrored.
+        // trailing block is never mirrored. update: panel [pf, pl]'s trailing
+        // update of columns [c0, c1), rows from c0 down, on stream q; limit:
+        // the status limit (nonzero for the parts that may run beside later
+        // panels: skipped only when this panel or an earlier one stopped).
+        auto update = [&](hipStream_t q, int64_t pf, int64_t pl, const double *Wh, int64_t c0, int64_t c1, int64_t limit) -> hipError_t {
+            if (c0 >= c1) return hipSuccess;
+            const int64_t tfirst = pl + 1, tcount = n - tfirst;
+            mfk::TileArgs t{};
+            t.mode = mfk::MODE_AB; t.net = mf::NET_STD; t.lower = 1; t.beta_zero = 0; t.tile = c.gemm_tile;
+            t.row_lo = 0; t.rows = n - c0; t.n = c1 - c0; t.K = pl - pf + 1; t.status_limit = limit;
+            const DV Wt{const_cast<double *>(Wh) + (c0 - tfirst), tcount, 1, tcount * (b + 1)};
+            return mfk::tile_launch_n(N, q, t, sub(a, c0, pf), Wt, sub(a, c0, c0), kMinusOne, kOne, st);
+        };
+        // the whole update (after the weighted panel when the panels do not
+        // write it)
+        auto trailing = [&](hipStream_t q, int64_t pf, int64_t pl, double *Wh) -> hipError_t {
+            if (pl >= n - 1) return hipSuccess;
+            const int64_t tfirst = pl + 1, tcount = n - tfirst;
+            hipError_t e = hipSuccess;
+            if (!use_grid && (e = mfk::ldlt_weighted(N, q, a, pf, pl, tfirst, tcount, d, blocks, DV{Wh, 1, tcount, tcount * (b + 1)}, st)) !=
+                                 hipSuccess)
+                return e;
+            return update(q, pf, pl, Wh, tfirst, n, 0);
+        };
         // The host launches the panels and their trailing updates ahead,
         // assuming each panel ends at its requested last column, and reads the
         // status after `depth` panels: a panel that extended (a 2x2 pivot at
-        // its last column) recorded kPanelExtendedInfo, so the kernels
-        // launched after it did nothing; the loop then resumes after it. The
-        // depth doubles after a read without extension (up to
-        // kLdltSpecMaxDepth) and drops to one after an extension.
-        const bool use_grid = grid_fits && gcap > 0;     // its panels also write the weighted rows
-        auto trailing = [&](int64_t pf, int64_t pl) -> hipError_t {
-            if (pl >= n - 1) return hipSuccess;
-            const int64_t tfirst = pl + 1, tcount = n - pl - 1, width = pl - pf + 1;
-            const DV Wv{W, 1, tcount, tcount * (b + 1)};
-            hipError_t e = use_grid ? hipSuccess : mfk::ldlt_weighted(N, s, a, pf, pl, tfirst, tcount, d, blocks, Wv, st);
-            if (e == hipSuccess)
-                e = lower_launch(h, N, c.gemm_tile, tcount, width, sub(a, tfirst, pf), DV{W, tcount, 1, tcount * (b + 1)},
-                                 sub(a, tfirst, tfirst), kMinusOne, kOne, st);
-            return e;
-        };
-        std::vector<std::pair<int64_t, int64_t>> pending;   // (pfirst, reqlast) since the last status read
-        int64_t depth = 4;
-        int64_t pfirst = 0;
+        // its last column) or stopped recorded a stop code (is_panel_stop),
+        // so the kernels launched after it did nothing; the loop then resumes
+        // after it (or at it). The depth doubles after a read without a stop
+        // (up to kLdltSpecMaxDepth) and drops to one after a stop.
+        struct Launched { int64_t pfirst, reqlast; double *Wh; };
+        std::vector<Launched> pending;
+        int64_t depth = 4, pfirst = 0, hidx = 0;
+        bool prev_split = false;                         // the previous update's later parts may still run
+        int64_t la_hold = 0, la_backoff = 8, la_stops = 0;
+        int rc = MFLA_OK;
         while (pfirst < n) {
             const int64_t reqlast = std::min(pfirst + b, n) - 1;
+            double *Wh = la ? W + (size_t)(hidx % 3) * half : W;
+            // beside the previous update's later parts: on the panel stream,
+            // in truncation mode
+            const hipStream_t pq = prev_split ? ps : s;
             if (use_grid) {
-                const int64_t G = std::max<int64_t>(1, std::min(gcap, mfk::cdiv(n - pfirst - 1, kLdltPanelRows)));
-                HIPCHK(mfk::ldlt_panel_grid(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, 1, b + 1, st, (int)G));
+                HIPCHK(mfk::ldlt_panel_grid(N, pq, a, n, pfirst, reqlast, d, pivots, blocks, alpha, Wh, 1, prev_split ? 1 : 0, b + 1, st,
+                                            (int)workgroups(pfirst)));
             } else {
-                HIPCHK(mfk::ldlt_panel(N, s, a, n, pfirst, reqlast, d, pivots, blocks, alpha, W, 1, st));
+                HIPCHK(mfk::ldlt_panel(N, pq, a, n, pfirst, reqlast, d, pivots, blocks, alpha, Wh, 1, st));
             }
-            pending.emplace_back(pfirst, reqlast);
-            HIPCHK(trailing(pfirst, reqlast));
+            pending.push_back({pfirst, reqlast, Wh});
+            const bool split = la && la_hold == 0 && reqlast + 1 + b < n;
+            if (la_hold > 0) --la_hold;
+            const int64_t t0c = reqlast + 1, t1c = std::min(t0c + b, n), t2c = std::min(t0c + 2 * b, n);
+            if (!split) {
+                // after the previous update's later parts; the next panel on
+                // the call's stream
+                const hipStream_t tq = prev_split ? us : s;
+                HIPCHK(order(pq, tq, h->ev_panel));
+                HIPCHK(trailing(tq, pfirst, reqlast, Wh));
+                HIPCHK(order(tq, s, h->ev_update));
+                us_open = us_open || tq == us;
+            } else {
+                // first part on the panel stream (after the previous update's
+                // second part), the others on the update stream
+                HIPCHK(order(pq, ps, h->ev_join));
+                HIPCHK(order(pq, us, h->ev_panel));
+                if (prev_split) HIPCHK(hipStreamWaitEvent(ps, h->ev_update, 0));
+                HIPCHK(update(ps, pfirst, reqlast, Wh, t0c, t1c, 0));
+                HIPCHK(update(us, pfirst, reqlast, Wh, t1c, t2c, reqlast + 1));
+                HIPCHK(hipEventRecord(h->ev_update, us));
+                HIPCHK(update(us, pfirst, reqlast, Wh, t2c, n, reqlast + 1));
+                ps_open = us_open = true;
+            }
+            prev_split = split;
+            ++hidx;
             pfirst = reqlast + 1;
             if ((int64_t)pending.size() < depth && pfirst < n) continue;
             int64_t aux = 0;
-            const int64_t r = read_info(h, &aux);
-            if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");
-            if (r == kGridTimeoutInfo) return fail(MFLA_ERR_HIP, "ldlt! panel: a grid barrier timed out");
-            if (r == kPanelExtendedInfo) {
-                // aux = the extended panel's last column
-                int64_t pf = -1;
+            const int64_t r = read_info(h, &aux, pq);      // after the last panel
+            if (r == INT64_MIN) { rc = fail(MFLA_ERR_HIP, "status read failed"); break; }
+            if (r == kGridTimeoutInfo) { rc = fail(MFLA_ERR_HIP, "ldlt! panel: a grid barrier timed out"); break; }
+            if (mfk::is_panel_stop(r)) {
+                const bool extended = !mfk::panel_stop_truncated(r);
+                // aux: the extended panel's last column / the column where the
+                // panel stopped
+                const Launched *p = nullptr;
                 for (const auto &pr : pending)
-                    if (pr.second + 1 == aux) pf = pr.first;
-                if (pf < 0) return fail(MFLA_ERR_HIP, "ldlt! panel: extended panel not found");
+                    if (pr.pfirst == mfk::panel_stop_first(r)) p = &pr;
+                if (!p) { rc = fail(MFLA_ERR_HIP, "ldlt! panel: the stopped panel was not launched"); break; }
+                HIPCHK(gather());
                 HIPCHK(mfk::status_reset(s, st));
-                HIPCHK(trailing(pf, aux));
-                pfirst = aux + 1;
+                if (extended) {
+                    HIPCHK(trailing(s, p->pfirst, aux, p->Wh));
+                    pfirst = aux + 1;
+                } else {
+                    HIPCHK(mfk::ldlt_restore(N, s, a, n, p->pfirst, aux, p->Wh));
+                    pfirst = p->pfirst;
+                    // off for the rest of the call after a second stop
+                    la_hold = ++la_stops >= 2 ? n : la_backoff;
+                    la_backoff = std::min<int64_t>(8 * la_backoff, n);
+                }
+                prev_split = false;
                 pending.clear();
                 depth = 1;
                 continue;
@@ -1093,13 +1269,16 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
                 // part when it freezes; the block from the first unfrozen row on
                 // is symmetric in MFLA's matrix at the failure
                 const int64_t m = std::min(aux, n);
+                HIPCHK(gather());
                 HIPCHK(mfk::mirror_lower(N, s, sub(a, m, m), n - m, nullptr));
             }
             if (r != 0) break;
             pending.clear();
             depth = std::min(kLdltSpecMaxDepth, 2 * depth);
         }
+        HIPCHK(gather());
         if (own) { (void)hipStreamSynchronize(s); (void)hipFree(W); }
+        if (rc != MFLA_OK) return rc;
     }
     const int64_t r = read_info(h);
     if (r == INT64_MIN) return fail(MFLA_ERR_HIP, "status read failed");

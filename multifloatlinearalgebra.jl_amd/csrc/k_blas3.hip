@@ -30,10 +30,7 @@ template <int N, int NET, int TM, int TN, int BK, int MODE, bool LOWER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_tile(int64_t n, int64_t K, DV A, DV B, DV C, S<N> alpha, S<N> beta,
                                               int beta_zero, int64_t row_lo, int64_t row_hi, const Status *status,
                                               int64_t status_limit, int swap) {
-    if (status) {
-        const int64_t info = status->info;
-        if (info != 0 && (status_limit == 0 || info < 0 || info <= status_limit)) return;
-    }
+    if (status && status_skips(status, status_limit)) return;
     constexpr int BM = 16 * TM, BN = 16 * TN;
     __shared__ double As[BK * BM * N];
     // B's LDS slot of limb l of B(kk, jj). With two or more limbs: column-

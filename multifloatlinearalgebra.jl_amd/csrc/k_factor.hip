@@ -340,7 +340,6 @@ __global__ __launch_bounds__(512) void k_ldlt_unblocked(DV A, int64_t n, DV dsub
 // launched the following panels assuming this one ends at reqlast), a panel
 // that extends by a 2x2 pivot records kPanelExtended instead of 0, so every
 // kernel launched after it returns at once and the host resumes after it.
-constexpr int64_t kPanelExtended = -3;
 //
 // A Schur entry (i1, i2) -> (row, col) = (max, min) is taken against the
 // pivots q = pfirst..k-1 of this panel in MFLA's order: a 1x1 pivot
@@ -699,7 +698,7 @@ __global__ __launch_bounds__(512) void k_ldlt_panel(DV A, int64_t n, int64_t pfi
         }
     }
     if (threadIdx.x == 0) {
-        st->info = (info == 0 && spec && plast != reqlast) ? kPanelExtended : info;
+        st->info = (info == 0 && spec && plast != reqlast) ? panel_stop_code(pfirst, false) : info;
         st->aux = info > 0 ? frozen : plast;
     }
 }
@@ -812,7 +811,7 @@ __device__ int64_t waves_min(int64_t v, int64_t *si_buf) {
 template <int N>
 __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n, int64_t pfirst, int64_t reqlast, DV dsub, int64_t *pivots,
                                                                uint8_t *blocks, S<N> alpha_s, double *keepbuf, double *part, int spec,
-                                                               int64_t wld, Status *st) {
+                                                               int trunc, int64_t wld, Status *st) {
     if (st->info != 0) return;
     constexpr int W = ldlt_pw<N>();
     constexpr int NW = GP_THREADS / 64;
@@ -864,6 +863,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
     bool aorig_ok = false;
     int64_t plast = reqlast, k = pfirst, info = 0;
     int64_t frozen = 0;                              // after a failure: rows before it have their upper row written
+    int64_t tstop = -1;                              // trunc: the column that needed the row-max pass
     bool ok = true;
     unsigned it = 0;
     while (k <= plast) {
@@ -876,6 +876,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
         mf::V<N> absakk;
         mf::V<N> v0 = mf::zero<N>(), q0 = mf::zero<N>();
         mf::V<N> dpp = mf::zero<N>();                // entry(imax, imax) when the row-max pass ran
+        mf::V<N> a_kk = mf::zero<N>();               // helper, trunc: A(k, k) before this step
         if (k == n - 1) {
             absakk = mf::vabs(e.entry(k, k));
             bs = mf::is_zero(absakk) ? 0 : 1;
@@ -917,6 +918,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 // inner product of entry()'s terms, so the chain takes row k
                 // from registers and the prepared factors like the row threads'
                 mf::V<N> d0;
+                if (trunc) a_kk = ld<N>(A, k, k);
                 if (e.hoist) {
                     mf::V<N> xk[W];
 #pragma unroll
@@ -957,6 +959,12 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 bs = 0;
             } else if (mf::ge(absakk, mf::mul(alpha, colmax))) {
                 bs = 1;
+            } else if (trunc) {
+                // the row-max pass reads past the panel's columns, where the
+                // previous panel's update may still be running: stop here
+                // (every workgroup takes this branch at the same k)
+                tstop = k;
+                break;
             } else {
                 mf::V<N> rm = mf::zero<N>();
                 e.prow(imax);
@@ -1051,6 +1059,10 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
                 // (published before the barrier): the same bits as the owner's
                 const mf::V<N> q1 = k + 1 < n ? mf::div(e.kept(k + 1), d) : mf::zero<N>();
                 if (threadIdx.x == GP_T) {
+                    // trunc: the diagonal's value before the panel, kept for
+                    // the restore should a later column stop the panel (set 3
+                    // is the 2x2 pivots' and unused before a row-max pass)
+                    if (trunc && wg == 0) put_sol(k, 1, a_kk);
                     stv<N>(A, k, k, d);
                     if (k + 1 < n) stv<N>(A, k + 1, k, q1);
                 }
@@ -1139,7 +1151,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
     // rows below the panel (ldlt.jl:595-632, the operations of
     // k_ldlt_weighted), limb stride (n - plast - 1) * wld, written over the
     // keep buffer once every workgroup is past its last read of it
-    if (ok && info == 0 && wld > 0 && plast < n - 1 && (ok = bar.sync())) {
+    if (ok && info == 0 && tstop < 0 && wld > 0 && plast < n - 1 && (ok = bar.sync())) {
         const int64_t tcount = n - plast - 1;
         const DV Wv{keepbuf, 1, tcount, tcount * wld};
         for (int64_t r = first_row(plast); r < n; r += GT) {
@@ -1165,11 +1177,31 @@ __global__ __launch_bounds__(GP_THREADS) void k_ldlt_panel_grid(DV A, int64_t n,
             __hip_atomic_exchange(&st->info, kGridTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else if (wg == 0) {
             int64_t expect = 0;
-            const int64_t code = (info == 0 && spec && plast != reqlast) ? kPanelExtended : info;
+            const int64_t code = tstop >= 0                                  ? panel_stop_code(pfirst, true)
+                                 : (info == 0 && spec && plast != reqlast) ? panel_stop_code(pfirst, false)
+                                                                           : info;
             __hip_atomic_compare_exchange_strong(&st->info, &expect, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            st->aux = info > 0 ? frozen : plast;
+            st->aux = info > 0 ? frozen : (tstop >= 0 ? tstop : plast);
             st->sync_base = bar.target;
         }
+    }
+}
+
+// After a panel in truncation mode stopped at column kstop: its columns
+// [pfirst, kstop) back to the values it started from. Those columns took 1x1
+// pivots without interchange, and each column pass left its column's entries
+// below the diagonal in row q's upper part; the diagonals wait in set 3 of
+// the keep buffer (limb l of A(q, q) at (3 N + l) n + q).
+template <int N>
+__global__ __launch_bounds__(256) void k_ldlt_restore(DV A, int64_t n, int64_t pfirst, const double *keepbuf) {
+    const int64_t q = pfirst + blockIdx.y;
+    for (int64_t r = q + 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        stv<N>(A, r, q, ld<N>(A, q, r));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        mf::V<N> d;
+#pragma unroll
+        for (int l = 0; l < N; ++l) d.x[l] = keepbuf[(3 * N + l) * n + q];
+        stv<N>(A, q, q, d);
     }
 }
 
@@ -1385,7 +1417,7 @@ hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfir
 size_t ldlt_panel_grid_doubles(int N, int64_t n) { return 4 * (size_t)N * (size_t)n + ldlt_grid_partials(N); }
 
 hipError_t ldlt_panel_grid(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
-                           uint8_t *blocks, const double *alpha, double *buf, int spec, int64_t weighted_ld, Status *st,
+                           uint8_t *blocks, const double *alpha, double *buf, int spec, int trunc, int64_t weighted_ld, Status *st,
                            int workgroups) {
     if (workgroups < 1 || workgroups > GP_MAXG) return hipErrorInvalidValue;
     double *part = buf + 4 * (size_t)N * (size_t)n;
@@ -1394,14 +1426,14 @@ hipError_t ldlt_panel_grid(int N, hipStream_t s, const DV &A, int64_t n, int64_t
         DV a = A, d = dsub;
         int64_t n_ = n, pf = pfirst, rl = reqlast;
         S<NN> al = mk<NN>(alpha);
-        int sp = spec;
+        int sp = spec, tr = trunc;
         int64_t wl = weighted_ld;
         // an ordinary launch: the panel's workgroups (at most 64, one per CU)
         // are resident together on a device that runs nothing else meanwhile
         // (a cooperative launch costs ~11 us before and after the kernel);
         // should they not be, the bounded barrier ends the call with an error
         hipLaunchKernelGGL(k_ldlt_panel_grid<NN>, dim3((unsigned)workgroups), dim3(GP_THREADS), 0, s, a, n_, pf, rl, d, pivots, blocks, al, buf,
-                           part, sp, wl, st);
+                           part, sp, tr, wl, st);
         return hipGetLastError();
     };
     switch (N) {
@@ -1425,6 +1457,13 @@ hipError_t mirror_lower(int N, hipStream_t s, const DV &A, int64_t n, const Stat
     if (n <= 1) return hipSuccess;
     const unsigned t = (unsigned)((n + MIR_T - 1) / MIR_T);
     MF_DISPATCH(N, hipLaunchKernelGGL(k_mirror_lower<NN>, dim3(t, t), dim3(256), 0, s, A, n, st));
+    return hipGetLastError();
+}
+
+hipError_t ldlt_restore(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t kstop, const double *keepbuf) {
+    if (kstop <= pfirst) return hipSuccess;
+    const unsigned bx = (unsigned)std::min<int64_t>(nb(n, 256), 64);
+    MF_DISPATCH(N, hipLaunchKernelGGL(k_ldlt_restore<NN>, dim3(bx, (unsigned)(kstop - pfirst)), dim3(256), 0, s, A, n, pfirst, keepbuf));
     return hipGetLastError();
 }
 

@@ -27,10 +27,15 @@ hipError_t ldlt_panel(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfir
 // weighted_ld > 0: at the end the panel's weighted rows W = L21 D overwrite
 // buf as ldlt_weighted would write them with W's limb stride
 // (n - plast - 1) * weighted_ld. st->info = -2 if a grid barrier timed out.
+// trunc != 0: the panel stops at its first column that needs the row-max
+// pass (kPanelTruncated, that column in st->aux) without touching that
+// column's entries; ldlt_restore then returns its columns before it to the
+// values they had at the launch.
 size_t ldlt_panel_grid_doubles(int N, int64_t n);
 hipError_t ldlt_panel_grid(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t reqlast, const DV &dsub, int64_t *pivots,
-                           uint8_t *blocks, const double *alpha, double *buf, int spec, int64_t weighted_ld, Status *st,
+                           uint8_t *blocks, const double *alpha, double *buf, int spec, int trunc, int64_t weighted_ld, Status *st,
                            int workgroups);
+hipError_t ldlt_restore(int N, hipStream_t s, const DV &A, int64_t n, int64_t pfirst, int64_t kstop, const double *buf);
 hipError_t ldlt_weighted(int N, hipStream_t s, const DV &A, int64_t pfirst, int64_t plast, int64_t tfirst, int64_t tcount,
                          const DV &dsub, const uint8_t *blocks, const DV &W, const Status *st);
 hipError_t mirror_lower(int N, hipStream_t s, const DV &A, int64_t n, const Status *st);

@@ -48,6 +48,34 @@ struct Status {
     unsigned long long sync_base;   // the arrivals before the next such launch
 };
 
+// A blocked-LDLT panel that stops by design (the host resumes the
+// factorization after reading the code) records
+// info = kPanelStopBase - 2 * pfirst - truncated: truncated = 0 when it
+// extended past its last requested column by a 2x2 pivot, 1 when it reached
+// a column that needs the row-max pass while the previous panel's trailing
+// update was still running. The code names the panel's first column, so a
+// kernel that reads the word alone knows which panel stopped.
+constexpr int64_t kPanelStopBase = -16;
+__host__ __device__ inline int64_t panel_stop_code(int64_t pfirst, bool truncated) { return kPanelStopBase - 2 * pfirst - (truncated ? 1 : 0); }
+__host__ __device__ inline bool is_panel_stop(int64_t info) { return info <= kPanelStopBase; }
+__host__ __device__ inline int64_t panel_stop_first(int64_t info) { return (kPanelStopBase - info) >> 1; }
+__host__ __device__ inline bool panel_stop_truncated(int64_t info) { return ((kPanelStopBase - info) & 1) != 0; }
+
+// Whether a kernel gated by the status returns at once. limit == 0: on any
+// recorded status. limit > 0 (an update of the panel ending at one-based
+// column `limit`, which may run beside later panels): only on failures at
+// columns <= limit or a stop of a panel that starts at or before it. A code
+// recorded by a later panel while the update runs leaves it running
+// whichever value the load returns.
+__device__ inline bool status_skips(const Status *st, int64_t limit) {
+    const int64_t info = st->info;
+    if (info == 0) return false;
+    if (limit == 0) return true;
+    if (info > 0) return info <= limit;
+    if (is_panel_stop(info)) return panel_stop_first(info) + 1 <= limit;
+    return true;
+}
+
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace mfk

@@ -292,6 +292,25 @@ def _ldlt_with_panel_workgroups(M, K, cap, **cfg):
     return F, host(Kd)
 
 
+def _ldlt_with_env(M, K, env, **cfg):
+    """ldlt! (blocked route) with environment variables set for the call."""
+    import os
+
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        Kd = dev(K)
+        F = M.ldlt_(Kd, check=False, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_blocked_crossover=1, **cfg))
+        torch.cuda.synchronize()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    return F, host(Kd)
+
+
 def _assert_same_ldlt(Fa, Ka, Fb, Kb, what):
     assert Fa.info == Fb.info, what
     assert np.array_equal(Fa.pivots.cpu().numpy(), Fb.pivots.cpu().numpy()), f"{what}: pivots"
@@ -348,6 +367,62 @@ def test_ldlt_panel_over_workgroups_failure_state(M, oracle, N, zero):
     assert_bitwise(host(Fg.dsub), dsub_ref, "dsub")
     assert_bitwise(Kg, ref, f"failed blocked ldlt x{N} n={n} zero column {zero}")
     _assert_same_ldlt(Fg, Kg, F1, K1, f"failed ldlt x{N} zero {zero}")
+
+
+def _lookahead_case(rng, N, kind, n):
+    H = rand_mf(rng, N, n, n)
+    K = renorm(H + np.transpose(H, (0, 2, 1)))
+    idx = np.arange(n)
+    if kind == "dominant":
+        K[0, idx, idx] += 4.0 * n
+    elif kind == "dominant_then_indefinite":
+        # 1x1 steps without interchange up to column 403 (inside a panel),
+        # then steps that need the row-max pass
+        K[0, idx[:403], idx[:403]] += 4.0 * n
+    elif kind == "dominant_zero":
+        K[0, idx, idx] += 4.0 * n
+        K[:, 301, :] = 0.0
+        K[:, :, 301] = 0.0
+    return K
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("kind,n", [("dominant", 1000), ("indefinite", 900), ("dominant_then_indefinite", 800),
+                                    ("dominant_zero", 700)])
+def test_ldlt_lookahead_equals_serial(M, N, kind, n):
+    """Look-ahead (the panels on a few CUs, each trailing update split into
+    the next panel's columns and the rest, which runs beside the next panel)
+    against the serial order, bitwise: throughout (dominant), stopped panels
+    restored and run again (a panel that reaches a column needing the row-max
+    pass while the previous update runs: at a panel's first column
+    (indefinite) or inside it after 1x1 steps (column 403)), and a zero
+    column met beside a running update (info 302)."""
+    rng = np.random.default_rng(n + 31 * N)
+    K = _lookahead_case(rng, N, kind, n)
+    F0, K0 = _ldlt_with_env(M, K, {"MFLA_LDLT_LOOKAHEAD": 0})
+    if kind == "dominant_zero":
+        assert F0.info == 302
+    else:
+        assert F0.info == 0
+    if kind != "dominant":
+        assert (F0.blocks.cpu().numpy() == 2).any() or kind == "dominant_zero"
+    for env in ({"MFLA_LDLT_LOOKAHEAD": 1}, {"MFLA_LDLT_LOOKAHEAD": 1, "MFLA_LDLT_PANEL_WORKGROUPS": 2}):
+        F1, K1 = _ldlt_with_env(M, K, env)
+        _assert_same_ldlt(F1, K1, F0, K0, f"ldlt x{N} n={n} {kind} {env}")
+
+
+def test_ldlt_lookahead_against_oracle(M, oracle):
+    """The look-ahead route against the oracle (MFLA's blocked ldlt!) on a
+    matrix whose panels stop inside a panel and at panel starts."""
+    rng = np.random.default_rng(77)
+    K = _lookahead_case(rng, 2, "dominant_then_indefinite", 600)
+    ref = K.copy()
+    info_ref, dsub_ref, piv_ref, blk_ref, _ = oracle.sytrf(ref, "blocked", 0, 1, threads=8)
+    F, Kg = _ldlt_with_env(M, K, {"MFLA_LDLT_LOOKAHEAD": 1})
+    assert F.info == info_ref == 0
+    assert np.array_equal(F.pivots.cpu().numpy(), piv_ref) and np.array_equal(F.blocks.cpu().numpy(), blk_ref)
+    assert_bitwise(host(F.dsub), dsub_ref, "dsub")
+    assert_bitwise(Kg, ref, "look-ahead ldlt x2 n=600")
 
 
 def test_status_codes_and_check(M):
