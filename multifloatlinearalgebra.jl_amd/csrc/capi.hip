@@ -1165,6 +1165,10 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
         auto update = [&](hipStream_t q, int64_t pf, int64_t pl, const double *Wh, int64_t c0, int64_t c1, int64_t limit) -> hipError_t {
             if (c0 >= c1) return hipSuccess;
             const int64_t tfirst = pl + 1, tcount = n - tfirst;
+            const DV Wn{const_cast<double *>(Wh) + (c0 - tfirst), tcount, 1, tcount * (b + 1)};
+            if (c1 - c0 <= 8 && c.gemm_tile < 0)
+                return mfk::cols_launch_n(N, q, n - c0, c1 - c0, pl - pf + 1, true, sub(a, c0, pf), Wn, sub(a, c0, c0), kMinusOne, kOne, st,
+                                          limit);
             mfk::TileArgs t{};
             t.mode = mfk::MODE_AB; t.net = mf::NET_STD; t.lower = 1; t.beta_zero = 0; t.tile = c.gemm_tile;
             t.row_lo = 0; t.rows = n - c0; t.n = c1 - c0; t.K = pl - pf + 1; t.status_limit = limit;

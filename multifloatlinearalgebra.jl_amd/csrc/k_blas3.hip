@@ -23,6 +23,7 @@
 // copies.
 #include "mf_common.hpp"
 #include "k_blas3.hpp"
+#include <type_traits>
 
 namespace mfk {
 
@@ -273,6 +274,52 @@ hipError_t tile_launch(hipStream_t s, const TileArgs &t, const DV &A, const DV &
     }
     if (t.lower) return pick<N, mf::NET_STD, MODE_AB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);
     return pick<N, mf::NET_STD, MODE_AB, false>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);
+}
+
+// Narrow outputs (at most 8 columns): a workgroup of 256 threads takes 32
+// rows x 8 columns, one output per thread, its operands read straight from
+// memory (a(i, k) shared by the row's 8 threads, b(k, j) by the column's 32).
+// Per element the operations and their order are k_tile's MODE_AB: the MAC
+// chain over k ascending from zero, then alpha * acc + beta * C.
+template <int N, bool LOWER>
+__global__ __launch_bounds__(256) void k_cols(int64_t rows, int64_t ncols, int64_t K, DV A, DV B, DV C, S<N> alpha, S<N> beta,
+                                              const Status *status, int64_t status_limit) {
+    if (status && status_skips(status, status_limit)) return;
+    const int64_t i = (int64_t)blockIdx.x * 32 + (threadIdx.x & 31), j = threadIdx.x >> 5;
+    if (i >= rows || j >= ncols || (LOWER && i < j)) return;
+    mf::V<N> acc = mf::zero<N>();
+    mf::V<N> a = K > 0 ? ld<N>(A, i, 0) : acc, b = K > 0 ? ld<N>(B, 0, j) : acc;
+    for (int64_t k = 0; k < K; ++k) {
+        const mf::V<N> ak = a, bk = b;
+        if (k + 1 < K) {
+            a = ld<N>(A, i, k + 1);
+            b = ld<N>(B, k + 1, j);
+        }
+        acc = mf::Mac<N, mf::NET_STD>::apply(acc, ak, bk);
+    }
+    stv<N>(C, i, j, mf::add(mf::mul(sv(alpha), acc), mf::mul(sv(beta), ld<N>(C, i, j))));
+}
+
+hipError_t cols_launch_n(int N, hipStream_t s, int64_t rows, int64_t ncols, int64_t K, bool lower, const DV &A, const DV &B, const DV &C,
+                         const double *alpha, const double *beta, const Status *st, int64_t status_limit) {
+    if (rows <= 0 || ncols <= 0) return hipSuccess;
+    if (ncols > 8) return hipErrorInvalidValue;
+    auto launch = [&](auto ntag) -> hipError_t {
+        constexpr int NN = decltype(ntag)::value;
+        S<NN> al, be;
+        for (int l = 0; l < NN; ++l) { al.x[l] = alpha[l]; be.x[l] = beta[l]; }
+        const dim3 grid((unsigned)cdiv(rows, 32));
+        if (lower) hipLaunchKernelGGL((k_cols<NN, true>), grid, dim3(256), 0, s, rows, ncols, K, A, B, C, al, be, st, status_limit);
+        else hipLaunchKernelGGL((k_cols<NN, false>), grid, dim3(256), 0, s, rows, ncols, K, A, B, C, al, be, st, status_limit);
+        return hipGetLastError();
+    };
+    switch (N) {
+        case 1: return launch(std::integral_constant<int, 1>());
+        case 2: return launch(std::integral_constant<int, 2>());
+        case 3: return launch(std::integral_constant<int, 3>());
+        case 4: return launch(std::integral_constant<int, 4>());
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t tile_launch_n(int N, hipStream_t s, const TileArgs &t, const DV &A, const DV &B, const DV &C, const double *alpha,
