@@ -250,6 +250,9 @@ hipError_t ensure_side(mfla_handle h) {
 // panel_cus <= C keeps every bit inside its XCD's block. Fails (and the
 // caller runs without look-ahead) on a device without eight XCDs of at
 // least 16 CUs.
+// CUs of ldlt! look-ahead's panel stream (four per XCD; the panels use at
+// most 32 workgroups there, and the first part of each update runs beside them)
+constexpr int kLdltPanelCUs = 32;
 hipError_t ensure_la_streams(mfla_handle h, int panel_cus) {
     if (h->la_panel && h->la_cus == panel_cus) return hipSuccess;
     int cus = 0;
@@ -462,6 +465,9 @@ int mfla_handle_create(mfla_handle *out) {
     void *pin = nullptr;
     if (hipHostMalloc(&pin, mfla_handle_s::kPinnedBytes, hipHostMallocDefault) == hipSuccess) h->pinned = static_cast<unsigned char *>(pin);
     else (void)hipGetLastError();                  // pageable reads instead
+    // the look-ahead streams now, so that no call pays for their queues
+    // (a device where they cannot be made runs without look-ahead)
+    if (ensure_side(h) != hipSuccess || ensure_la_streams(h, kLdltPanelCUs) != hipSuccess) (void)hipGetLastError();
     *out = h;
     return MFLA_OK;
 }
@@ -1020,9 +1026,6 @@ static int64_t ldlt_panel_workgroup_cap() {
 // The environment variable MFLA_LDLT_LOOKAHEAD: 0 never, 1 whenever the
 // call can, otherwise from kLdltLookaheadMinN on (the bits never change).
 constexpr int64_t kLdltLookaheadMinN = 1536;
-// CUs of the look-ahead's panel stream (four per XCD; the panels use at most
-// 32 workgroups there, and the first part of each update runs beside them)
-constexpr int kLdltPanelCUs = 32;
 static int ldlt_lookahead_mode() {
     const char *v = std::getenv("MFLA_LDLT_LOOKAHEAD");
     if (v && v[0] == '0' && !v[1]) return 0;
