@@ -292,7 +292,7 @@ def _ldlt_with_panel_workgroups(M, K, cap, **cfg):
     return F, host(Kd)
 
 
-def _ldlt_with_env(M, K, env, **cfg):
+def _ldlt_with_env(M, K, env, workspace=None, **cfg):
     """ldlt! (blocked route) with environment variables set for the call."""
     import os
 
@@ -300,7 +300,8 @@ def _ldlt_with_env(M, K, env, **cfg):
     os.environ.update({k: str(v) for k, v in env.items()})
     try:
         Kd = dev(K)
-        F = M.ldlt_(Kd, check=False, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_blocked_crossover=1, **cfg))
+        F = M.ldlt_(Kd, check=False, config=M.KernelConfig(ldlt_strategy="blocked", ldlt_blocked_crossover=1, **cfg),
+                    workspace=workspace)
         torch.cuda.synchronize()
     finally:
         for k, v in old.items():
@@ -409,6 +410,11 @@ def test_ldlt_lookahead_equals_serial(M, N, kind, n):
     for env in ({"MFLA_LDLT_LOOKAHEAD": 1}, {"MFLA_LDLT_LOOKAHEAD": 1, "MFLA_LDLT_PANEL_WORKGROUPS": 2}):
         F1, K1 = _ldlt_with_env(M, K, env)
         _assert_same_ldlt(F1, K1, F0, K0, f"ldlt x{N} n={n} {kind} {env}")
+    # the weighted panels' three rotating parts taken from a workspace, twice
+    ws = M.MFWorkspace(N, factor_capacity=n, ldlt_block_capacity=M.ldlt_plan(N, n).block_size)
+    for rep in range(2):
+        F2, K2 = _ldlt_with_env(M, K, {"MFLA_LDLT_LOOKAHEAD": 1}, workspace=ws)
+        _assert_same_ldlt(F2, K2, F0, K0, f"ldlt x{N} n={n} {kind} workspace call {rep}")
 
 
 def test_ldlt_lookahead_against_oracle(M, oracle):
