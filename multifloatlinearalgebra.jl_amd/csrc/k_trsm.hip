@@ -197,8 +197,14 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
     const int64_t n = a.n, nrhs = a.nrhs;
     if (n == 0 || nrhs == 0) return hipSuccess;
     hipError_t e = hipSuccess;
+    // The updates take 16 x 16 tiles: with K = 64 a 32 x 32 tile's grid
+    // (e.g. 992 workgroups for 3968 rows x 256 right-hand sides, at three
+    // per CU) needs a second, mostly idle round, while four times as many
+    // 16 x 16 workgroups fill ~4 rounds (x4 4096 x 256: 22.3 -> 20.0 ms; x2 /
+    // x3 and 1024 right-hand sides faster too). The tile does not change
+    // the bits.
     TileArgs t{};
-    t.mode = MODE_SUB; t.net = mf::NET_STD; t.lower = 0; t.beta_zero = 0; t.tile = -1; t.row_lo = 0; t.n = nrhs;
+    t.mode = MODE_SUB; t.net = mf::NET_STD; t.lower = 0; t.beta_zero = 0; t.tile = 0; t.row_lo = 0; t.n = nrhs;
     t.swap_product = a.swap_product;
     t.status_limit = a.status_limit;
     // X(r0 : r0 + rows) -= CA(r0 : , q0 : q0 + K) X(q0 : q0 + K) on stream ss
