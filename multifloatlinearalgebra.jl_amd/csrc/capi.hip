@@ -330,8 +330,10 @@ bool trsm_exact(const mfla_config &c, int64_t n) {
     } while (0)
 
 // Right-hand sides up to which a Float64x4 trsm_core look-ahead runs on the
-// CU-partitioned streams.
+// CU-partitioned streams, and up to which a forward one streams the next
+// block's rows.
 constexpr int64_t kTrsmSplitMaxRhs = 64;
+constexpr int64_t kTrsmStreamMaxRhs = 64;
 
 // trsm! core (trsm.jl:247-301) after argument checks; alpha scaling included.
 hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int diag, const mfla_matrix &B, const mfla_matrix &A,
@@ -367,6 +369,15 @@ hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int di
     if (n > 3 * 64 && (a.forward || !a.exact_order) && ensure_side(h) == hipSuccess) {
         const bool split = h->la_panel && h->la_update && N == 4 && a.nrhs <= kTrsmSplitMaxRhs;
         a.side = split ? h->la_update : h->side; a.ev_main = h->ev_panel; a.ev_side = h->ev_update; a.ev_side2 = h->ev_update2;
+        // Forward, few right-hand sides: the next block's rows streamed inside
+        // each diagonal kernel (device words 1 and 2 of the handle's flag
+        // array). Each kernel then ends no earlier than the side stream's
+        // update from the block before, one block of slack instead of two:
+        // a gain where the diagonal chain is the bound (4096 x 64: x4 13.4 ->
+        // 12.6 ms, x2 4.3 -> 3.7 ms), a loss where the updates are (256
+        // right-hand sides and more; x3, whose updates cost more beside its
+        // chain: 7.5 -> 7.8 ms).
+        if (a.forward && a.nrhs <= kTrsmStreamMaxRhs && N != 3) a.far_flag = reinterpret_cast<unsigned *>(h->flags) + 1;
         HIPRET_(hipEventRecord(h->ev_join, h->stream));
         HIPRET_(hipStreamWaitEvent(a.side, h->ev_join, 0));
         if (split) {

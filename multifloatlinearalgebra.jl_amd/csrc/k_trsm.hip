@@ -154,6 +154,104 @@ __global__ __launch_bounds__(256, N == 4 ? 5 : 8) void k_trsm_diag_wave(int64_t 
     }
 }
 
+// Forward look-ahead with the next block's rows streamed. A workgroup of 8
+// waves takes 4 right-hand sides: waves 0-3 solve the block (the operations
+// of k_trsm_diag_wave with one wave per right-hand side and no previous-block
+// terms); waves 4-7 take the next block's rows p1 + lane and, once the side
+// stream's update from the block before has reached them (*far_flag >=
+// far_wait, written after that update), subtract this block's terms in
+// ascending q as the solver waves publish them in LDS -- the operands,
+// operations and order of the next kernel's previous-block terms
+// (k_trsm_diag_wave with K > 0), which then has none. That update was
+// enqueued before this kernel and waits only for the kernel before it, so
+// the wait ends; it is bounded all the same, and past the bound the rows
+// are written as NaN (and *err = 1) instead of values without the update.
+constexpr int kStreamRhs = 4;
+template <int N>
+__global__ __launch_bounds__(512, N == 4 ? 3 : 4) void k_trsm_diag_stream(int64_t p0, int nbk, int64_t p1, int nn, int64_t nrhs, DV X,
+                                                                        DV CA, DV D, int unit, int swap, const Status *st, int64_t lim,
+                                                                        const unsigned *far_flag, unsigned far_wait, unsigned *err) {
+    if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
+    __shared__ double dg[64 * N];
+    __shared__ double xs[kStreamRhs * 64 * N];
+    __shared__ int cnt[kStreamRhs];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool streamer = wave >= kStreamRhs;
+    const int r = streamer ? wave - kStreamRhs : wave;
+    const int64_t c = (int64_t)blockIdx.x * kStreamRhs + r;
+    const int64_t cc = c < nrhs ? c : nrhs - 1;   // in-bounds column for the unconditional loads
+    if (!unit && (int)threadIdx.x < nbk) lds_put<N>(&dg[threadIdx.x * N], ld<N>(D, p0 + threadIdx.x, p0 + threadIdx.x));
+    if ((int)threadIdx.x < kStreamRhs) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    if (!streamer) {
+        __builtin_amdgcn_s_setprio(3);
+        const bool valid = lane < nbk && c < nrhs;
+        const int64_t p = p0 + lane;
+        const int64_t pc = p0 + (lane < nbk ? lane : nbk - 1);
+        constexpr bool kAhead = N <= 3;
+        mf::V<N> v = valid ? ld<N>(X, p, c) : mf::zero<N>();
+        mf::V<N> an = kAhead ? ld<N>(CA, pc, p0) : mf::zero<N>();
+        for (int s = 0; s < nbk; ++s) {
+            const int64_t q = p0 + s;
+            mf::V<N> a;
+            if constexpr (kAhead) {
+                a = an;
+                if (s + 1 < nbk) an = ld<N>(CA, pc, q + 1);
+            } else {
+                a = ld<N>(CA, pc, q);
+            }
+            mf::V<N> xq;
+#pragma unroll
+            for (int l = 0; l < N; ++l) xq.x[l] = group_bcast<64>(v.x[l], 0, s);
+            if (!unit) xq = mf::div(xq, lds_get<N>(&dg[s * N]));
+            if (lane == 0) {
+                lds_put<N>(&xs[(r * 64 + s) * N], xq);
+                __hip_atomic_store(&cnt[r], s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            if (valid) {
+                if (lane == s) {
+                    v = xq;
+                    stv<N>(X, q, c, xq);
+                } else if (lane > s) {
+                    v = swap ? mf::msub(v, xq, a) : mf::msub(v, a, xq);
+                }
+            }
+        }
+    } else {
+        const bool valid = lane < nn && c < nrhs;
+        const int64_t p = p1 + lane, pc = p1 + (lane < nn ? lane : nn - 1);
+        bool ok = true;
+        unsigned spins = 0;
+        while (__hip_atomic_load(far_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < far_wait) {
+            __builtin_amdgcn_s_sleep(4);
+            if (++spins > (1u << 24)) { ok = false; break; }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        mf::V<N> v = ld<N>(X, pc, cc);
+        mf::V<N> an = ld<N>(CA, pc, p0);
+        for (int s = 0; s < nbk; ++s) {
+            const mf::V<N> a = an;
+            if (s + 1 < nbk) an = ld<N>(CA, pc, p0 + s + 1);
+            while (__hip_atomic_load(&cnt[r], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) <= s) __builtin_amdgcn_s_sleep(1);
+            const mf::V<N> xq = lds_get<N>(&xs[(r * 64 + s) * N]);
+            v = swap ? mf::msub(v, xq, a) : mf::msub(v, a, xq);
+        }
+        if (!ok) {
+#pragma unroll
+            for (int l = 0; l < N; ++l) v.x[l] = __builtin_nan("");
+            if (lane == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (valid) stv<N>(X, p, c, v);
+    }
+}
+
+// *flag = value after the work before it on the stream
+__global__ void k_trsm_flag(unsigned *flag, unsigned value) {
+    if (threadIdx.x != 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Exact MFLA order for backward substitution: one thread per right-hand side.
 template <int N>
 __global__ void k_trsm_seq_bwd(int64_t n, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap, const Status *st, int64_t lim) {
@@ -222,6 +320,45 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
     // every element still takes the blocks' terms in MFLA's block order, and
     // each block's terms in ascending q.
     const bool ahead = a.side != nullptr && n > 3 * TB;
+    if (ahead && a.forward && a.far_flag) {
+        // Forward look-ahead with the next block's rows streamed
+        // (k_trsm_diag_stream): block b's kernel applies its terms to block
+        // b + 1's rows once the side stream's update from block b - 1 has
+        // reached them; the side stream then updates the rows beyond block
+        // b + 1 and writes *far_flag = b + 1. No diagonal kernel waits for
+        // the side stream, and none has previous-block terms of its own.
+        hipError_t r;
+        hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, s, a.far_flag, 0u);
+        if ((r = hipGetLastError()) != hipSuccess) return r;
+        bool pending = false;
+        unsigned b = 0;
+        for (int64_t p0 = 0; p0 < n; p0 += TB, ++b) {
+            const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
+            const int64_t p1 = p0 + nbk;
+            const int nn = (int)((n - p1) < TB ? (n - p1) : TB);
+            if (nn > 0) {
+                hipLaunchKernelGGL(k_trsm_diag_stream<N>, dim3((unsigned)cdiv(nrhs, kStreamRhs)), dim3(512), 0, s, p0, nbk, p1, nn, nrhs, X, CA, D,
+                                   a.unit, a.swap_product, st, a.status_limit, a.far_flag, b, a.far_flag + 1);
+                if ((r = hipGetLastError()) != hipSuccess) return r;
+            } else if ((r = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit)) != hipSuccess) {
+                return r;
+            }
+            const int64_t far0 = p1 + nn;
+            if (far0 < n) {
+                if ((r = hipEventRecord(a.ev_main, s)) != hipSuccess) return r;
+                if ((r = hipStreamWaitEvent(a.side, a.ev_main, 0)) != hipSuccess) return r;
+                if ((r = update(a.side, far0, n - far0, p0, nbk)) != hipSuccess) return r;
+                hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, a.side, a.far_flag, b + 1);
+                if ((r = hipGetLastError()) != hipSuccess) return r;
+                pending = true;
+            }
+        }
+        if (pending) {
+            if ((e = hipEventRecord(a.ev_side, a.side)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(s, a.ev_side, 0)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     if (ahead) {
         hipEvent_t far_ev[2] = {a.ev_side, a.ev_side2};
         bool far_pending[2] = {false, false};

@@ -19,6 +19,11 @@ struct TrsmArgs {
     // the next block is solved; null = everything on the caller's stream.
     hipStream_t side;
     hipEvent_t ev_main, ev_side, ev_side2;
+    // Forward look-ahead only: two device words (a counter the side stream
+    // advances after each update, an error word); non-null: each diagonal
+    // kernel also applies its block's terms to the next block's rows as they
+    // are solved (the same operations and order as applying them after).
+    unsigned *far_flag;
 };
 
 // Solves in place on the solution view X(p, c) with coefficient view CA(p, q)
