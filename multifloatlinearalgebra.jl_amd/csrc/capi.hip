@@ -330,10 +330,11 @@ bool trsm_exact(const mfla_config &c, int64_t n) {
     } while (0)
 
 // Right-hand sides up to which a Float64x4 trsm_core look-ahead runs on the
-// CU-partitioned streams, and up to which a forward one streams the next
-// block's rows.
+// CU-partitioned streams, up to which a forward one streams the next block's
+// rows, and up to which the diagonal kernels wait on a device counter.
 constexpr int64_t kTrsmSplitMaxRhs = 64;
 constexpr int64_t kTrsmStreamMaxRhs = 64;
+constexpr int64_t kTrsmWaitMaxRhs = 256;
 
 // trsm! core (trsm.jl:247-301) after argument checks; alpha scaling included.
 hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int diag, const mfla_matrix &B, const mfla_matrix &A,
@@ -377,7 +378,13 @@ hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int di
         // 12.6 ms, x2 4.3 -> 3.7 ms), a loss where the updates are (256
         // right-hand sides and more; x3, whose updates cost more beside its
         // chain: 7.5 -> 7.8 ms).
-        if (a.forward && a.nrhs <= kTrsmStreamMaxRhs && N != 3) a.far_flag = reinterpret_cast<unsigned *>(h->flags) + 1;
+        a.stream_next = a.forward && a.nrhs <= kTrsmStreamMaxRhs && N != 3;
+        // Up to 256 right-hand sides each diagonal kernel waits for the side
+        // stream's updates on the same device words instead of the stream
+        // waiting on events (4096 x 256: x4 forward 19.9 -> 19.6 ms, backward
+        // 20.3 -> 19.7 ms; 64: x4 backward 14.0 -> 13.5 ms). With 1024 the
+        // waiting workgroups hold slots the updates need (x4 61.5 -> 64.7 ms).
+        if (a.stream_next || a.nrhs <= kTrsmWaitMaxRhs) a.far_flag = reinterpret_cast<unsigned *>(h->flags) + 1;
         HIPRET_(hipEventRecord(h->ev_join, h->stream));
         HIPRET_(hipStreamWaitEvent(a.side, h->ev_join, 0));
         if (split) {

@@ -81,6 +81,18 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
 // second coefficient in registers would spill under the 64-VGPR bound. The
 // loads use a row index clamped into the block (lanes that do not use the
 // value read an in-bounds entry they ignore).
+// Waits until *flag >= value (written by k_trsm_flag on another stream);
+// false after about 2^24 polls.
+__device__ inline bool wait_flag(const unsigned *flag, unsigned value) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < value) {
+        __builtin_amdgcn_s_sleep(4);
+        if (++spins > (1u << 24)) return false;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
 template <int G>
 __device__ __forceinline__ double group_bcast(double x, int grp, int s) {
     if constexpr (G == 64) {
@@ -98,11 +110,18 @@ __device__ __forceinline__ double group_bcast(double x, int grp, int s) {
 // before (unknowns q0 .. q0 + K - 1, ascending q: the order and operations of
 // the MODE_SUB tile update it replaces), so that update is not a kernel of
 // its own on the serial chain of blocks.
+//
+// With far_flag the kernel first waits (bounded) for *far_flag >= far_wait:
+// the side stream's update that last reaches this block's rows was enqueued
+// before this kernel and waits only for kernels before it, so the wait ends;
+// past the bound the block's values are written as NaN and *err = 1.
 template <int N, bool FWD, int G>
 __global__ __launch_bounds__(256, N == 4 ? 5 : 8) void k_trsm_diag_wave(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap,
-                                                           const Status *st, int64_t lim, int64_t q0, int K) {
+                                                           const Status *st, int64_t lim, int64_t q0, int K, const unsigned *far_flag,
+                                                           unsigned far_wait, unsigned *err) {
     __builtin_amdgcn_s_setprio(3);
     if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
+    const bool ok = far_flag == nullptr || wait_flag(far_flag, far_wait);
     __shared__ double dg[G * N];
     constexpr int RPW = 64 / G;
     const int lane = threadIdx.x & 63, grp = lane / G, pl = lane % G;
@@ -113,6 +132,11 @@ __global__ __launch_bounds__(256, N == 4 ? 5 : 8) void k_trsm_diag_wave(int64_t 
     if (!unit && (int)threadIdx.x < nbk) lds_put<N>(&dg[threadIdx.x * N], ld<N>(D, p0 + threadIdx.x, p0 + threadIdx.x));
     constexpr bool kAhead = N <= 3;
     mf::V<N> v = valid ? ld<N>(X, p, c) : mf::zero<N>();
+    if (!ok) {
+#pragma unroll
+        for (int l = 0; l < N; ++l) v.x[l] = __builtin_nan("");
+        if (threadIdx.x == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (K > 0) {
         // terms of the previous block (every lane computes; invalid lanes'
         // values are never stored), the next term's operands loaded ahead
@@ -220,13 +244,7 @@ __global__ __launch_bounds__(512, N == 4 ? 3 : 4) void k_trsm_diag_stream(int64_
     } else {
         const bool valid = lane < nn && c < nrhs;
         const int64_t p = p1 + lane, pc = p1 + (lane < nn ? lane : nn - 1);
-        bool ok = true;
-        unsigned spins = 0;
-        while (__hip_atomic_load(far_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < far_wait) {
-            __builtin_amdgcn_s_sleep(4);
-            if (++spins > (1u << 24)) { ok = false; break; }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const bool ok = wait_flag(far_flag, far_wait);
         mf::V<N> v = ld<N>(X, pc, cc);
         mf::V<N> an = ld<N>(CA, pc, p0);
         for (int s = 0; s < nbk; ++s) {
@@ -271,14 +289,20 @@ static DV sub(const DV &v, int64_t i, int64_t j) { return DV{v.p + i * v.rs + j 
 
 template <int N>
 static hipError_t diag(hipStream_t s, bool fwd, int64_t p0, int nbk, int64_t nrhs, const DV &X, const DV &CA, const DV &D, int unit,
-                       int swap, const Status *st, int64_t lim, int64_t q0 = 0, int K = 0) {
+                       int swap, const Status *st, int64_t lim, int64_t q0 = 0, int K = 0, const unsigned *far_flag = nullptr,
+                       unsigned far_wait = 0, unsigned *err = nullptr) {
     // one group of G lanes per right-hand side, 4 waves per workgroup
     auto launch = [&](auto gtag) {
         constexpr int G = decltype(gtag)::value;
         const dim3 grid((unsigned)cdiv(cdiv(nrhs, 64 / G), 4)), block(256);
-        if (fwd) hipLaunchKernelGGL((k_trsm_diag_wave<N, true, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim, q0, K);
-        else hipLaunchKernelGGL((k_trsm_diag_wave<N, false, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim, q0, K);
+        if (fwd)
+            hipLaunchKernelGGL((k_trsm_diag_wave<N, true, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim, q0, K, far_flag,
+                               far_wait, err);
+        else
+            hipLaunchKernelGGL((k_trsm_diag_wave<N, false, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim, q0, K, far_flag,
+                               far_wait, err);
     };
+    if (far_flag && nbk > TB) return hipErrorInvalidValue;
     if (nbk <= 16) launch(std::integral_constant<int, 16>());
     else if (nbk <= 32) launch(std::integral_constant<int, 32>());
     else if (nbk <= TB) launch(std::integral_constant<int, 64>());
@@ -320,7 +344,7 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
     // every element still takes the blocks' terms in MFLA's block order, and
     // each block's terms in ascending q.
     const bool ahead = a.side != nullptr && n > 3 * TB;
-    if (ahead && a.forward && a.far_flag) {
+    if (ahead && a.forward && a.far_flag && a.stream_next) {
         // Forward look-ahead with the next block's rows streamed
         // (k_trsm_diag_stream): block b's kernel applies its terms to block
         // b + 1's rows once the side stream's update from block b - 1 has
@@ -360,6 +384,15 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
         return hipSuccess;
     }
     if (ahead) {
+        // With a device counter (far_flag): the side stream writes
+        // *far_flag = b + 1 after block b's update, and block b's kernel waits
+        // for block b - 2's in its prologue instead of an event wait on the
+        // stream (each of which costs the stream ~11 us even when satisfied).
+        const bool dev_wait = a.far_flag != nullptr;
+        if (dev_wait) {
+            hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, s, a.far_flag, 0u);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
         hipEvent_t far_ev[2] = {a.ev_side, a.ev_side2};
         bool far_pending[2] = {false, false};
         int64_t prev0 = 0;
@@ -368,17 +401,23 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
         auto block_step = [&](int64_t p0, int nbk, int64_t far0, int64_t nfar) -> hipError_t {
             hipError_t r;
             const int slot = (int)(b & 1);
+            unsigned wait = 0;
             if (far_pending[slot]) {   // the update from block b - 2
-                if ((r = hipStreamWaitEvent(s, far_ev[slot], 0)) != hipSuccess) return r;
+                if (dev_wait) wait = (unsigned)(b - 1);
+                else if ((r = hipStreamWaitEvent(s, far_ev[slot], 0)) != hipSuccess) return r;
                 far_pending[slot] = false;
             }
-            if ((r = diag<N>(s, a.forward != 0, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit, prev0, prevK)) !=
-                hipSuccess)
+            if ((r = diag<N>(s, a.forward != 0, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit, prev0, prevK,
+                             dev_wait ? a.far_flag : nullptr, wait, a.far_flag ? a.far_flag + 1 : nullptr)) != hipSuccess)
                 return r;
             if (nfar > 0) {
                 if ((r = hipEventRecord(a.ev_main, s)) != hipSuccess) return r;
                 if ((r = hipStreamWaitEvent(a.side, a.ev_main, 0)) != hipSuccess) return r;
                 if ((r = update(a.side, far0, nfar, p0, nbk)) != hipSuccess) return r;
+                if (dev_wait) {
+                    hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, a.side, a.far_flag, (unsigned)(b + 1));
+                    if ((r = hipGetLastError()) != hipSuccess) return r;
+                }
                 if ((r = hipEventRecord(far_ev[slot], a.side)) != hipSuccess) return r;
                 far_pending[slot] = true;
             }

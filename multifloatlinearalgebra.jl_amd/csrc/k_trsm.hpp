@@ -19,11 +19,15 @@ struct TrsmArgs {
     // the next block is solved; null = everything on the caller's stream.
     hipStream_t side;
     hipEvent_t ev_main, ev_side, ev_side2;
-    // Forward look-ahead only: two device words (a counter the side stream
-    // advances after each update, an error word); non-null: each diagonal
-    // kernel also applies its block's terms to the next block's rows as they
-    // are solved (the same operations and order as applying them after).
+    // Look-ahead only: two device words (a counter the side stream advances
+    // after each update, an error word); non-null: each diagonal kernel
+    // waits on the counter for the update that last reaches its rows instead
+    // of the stream waiting on an event.
     unsigned *far_flag;
+    // With far_flag, forward: each diagonal kernel also applies its block's
+    // terms to the next block's rows as they are solved (the same operations
+    // and order as applying them after).
+    int stream_next;
 };
 
 // Solves in place on the solution view X(p, c) with coefficient view CA(p, q)

@@ -3,6 +3,8 @@
 Matrices are limb-planar numpy arrays of shape (N, cols, rows) (the layout of
 the oracle and of the library); vectors are (N, n).
 """
+import zlib
+
 import numpy as np
 import pytest
 
@@ -25,6 +27,12 @@ def renorm(x):
             x[l] = s
             x[l + 1] = e
     return x
+
+
+def case_seed(*parts):
+    """A seed fixed by the test case's parameters (the same in every process,
+    unlike hash() of strings)."""
+    return zlib.crc32(repr(parts).encode())
 
 
 def rand_mf(rng, N, rows, cols=None, spread=0, scale=1.0):

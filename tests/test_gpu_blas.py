@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import torch
 
-from mfutil import HELD_OFF_GPU, assert_bitwise, eps, rand_mf
+from mfutil import HELD_OFF_GPU, assert_bitwise, eps, rand_mf, case_seed
 
 pytestmark = pytest.mark.gpu
 
@@ -147,7 +147,7 @@ def _triangular(rng, N, n, uplo, unit):
 @pytest.mark.parametrize("trans", ["N", "T"])
 @pytest.mark.parametrize("diag", ["nonunit", "unit"])
 def test_trsm_all_variants_exact_order(mfla, oracle, N, side, uplo, trans, diag):
-    rng = np.random.default_rng(hash((N, side, uplo, trans, diag)) % 2**32)
+    rng = np.random.default_rng(case_seed(N, side, uplo, trans, diag))
     n, other = 70, 9
     A = _triangular(rng, N, n, uplo, diag == "unit")
     B = rand_mf(rng, N, n, other) if side == "left" else rand_mf(rng, N, other, n)
@@ -158,15 +158,18 @@ def test_trsm_all_variants_exact_order(mfla, oracle, N, side, uplo, trans, diag)
     assert_bitwise(host(Bd), ref, f"trsm x{N} {side} {uplo} {trans} {diag}")
 
 
-@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("N,other", [(2, 7), (4, 7), (3, 7), (4, 70), (2, 70)])
 @pytest.mark.parametrize("side,uplo,trans", [("left", "lower", "N"), ("left", "upper", "T"), ("right", "upper", "N"),
                                              ("right", "lower", "T")])
-def test_trsm_forward_lookahead_bitwise(mfla, oracle, N, side, uplo, trans):
+def test_trsm_forward_lookahead_bitwise(mfla, oracle, N, other, side, uplo, trans):
     """Forward orders at n = 330 (six 64-row blocks): the diagonal solves
-    overlap the previous block's trailing update on a second stream; every
-    element keeps MFLA's ascending order, so the result is bitwise."""
-    rng = np.random.default_rng(hash((N, side, uplo, trans)) % 2**32)
-    n, other = 330, 7
+    overlap the previous block's trailing update on a second stream (with 7
+    right-hand sides and two or four limbs the next block's rows are updated
+    inside each diagonal kernel; otherwise each diagonal kernel waits on a
+    device counter for the update that last reaches its rows); every element
+    keeps MFLA's ascending order, so the result is bitwise."""
+    rng = np.random.default_rng(case_seed(N, other, side, uplo, trans))
+    n = 330
     A = _triangular(rng, N, n, uplo, False)
     B = rand_mf(rng, N, n, other) if side == "left" else rand_mf(rng, N, other, n)
     Bd = dev(B)
@@ -175,11 +178,14 @@ def test_trsm_forward_lookahead_bitwise(mfla, oracle, N, side, uplo, trans):
     assert_bitwise(host(Bd), ref, f"trsm look-ahead x{N} {side} {uplo} {trans}")
 
 
-@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("N,nrhs", [(2, 5), (4, 5), (3, 70)])
 @pytest.mark.parametrize("uplo,trans", [("upper", "N"), ("lower", "T")])
-def test_trsm_blocked_backward_within_tolerance(mfla, oracle, N, uplo, trans):
-    rng = np.random.default_rng(31 + N)
-    n, nrhs = 200, 5
+def test_trsm_blocked_backward_within_tolerance(mfla, oracle, N, nrhs, uplo, trans):
+    """Blocked backward order at n = 200 (four blocks, look-ahead: each
+    diagonal kernel waits on a device counter for the side stream's update
+    of its rows)."""
+    rng = np.random.default_rng(31 + N + nrhs)
+    n = 200
     A = _triangular(rng, N, n, uplo, False)
     B = rand_mf(rng, N, n, nrhs)
     Bd = dev(B)
