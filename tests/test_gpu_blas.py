@@ -178,6 +178,52 @@ def test_trsm_forward_lookahead_bitwise(mfla, oracle, N, other, side, uplo, tran
     assert_bitwise(host(Bd), ref, f"trsm look-ahead x{N} {side} {uplo} {trans}")
 
 
+def test_trsm_lookahead_on_a_handle_made_after_many_streams(mfla, oracle):
+    """Solves whose diagonal kernels wait on a device counter advanced on
+    another stream, on a handle made once the process holds more streams than
+    the runtime's hardware queues and issued on those streams (so the two
+    streams may share a queue with each other or the caller's): each counter
+    is advanced by work enqueued before the kernel that waits for it, so a
+    shared queue runs that work first, and the results are those of the
+    process's first handle (forward: bitwise MFLA's)."""
+    import ctypes
+
+    from mfla import _lib, api
+
+    devno = torch.cuda.current_device()
+    streams = [torch.cuda.Stream() for _ in range(6)]
+    cases = [(2, 7, "N"), (3, 70, "N"), (4, 7, "T"), (2, 70, "T"), (4, 70, "N")]
+    rng = np.random.default_rng(case_seed("many streams"))
+    inputs, first = [], []
+    for N, other, trans in cases:
+        A = _triangular(rng, N, 330, "lower", False)
+        B = rand_mf(rng, N, 330, other)
+        Bd = dev(B)
+        mfla.trsm_(Bd, dev(A), side="left", uplo="lower", trans=trans)
+        inputs.append((A, B))
+        first.append(host(Bd))
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.mfla_handle_create(ctypes.byref(h)) == 0
+    old = api._handles.get(devno)
+    try:
+        api._handles[devno] = h
+        for i, ((N, other, trans), (A, B)) in enumerate(zip(cases, inputs)):
+            Ad, Bd = dev(A), dev(B)
+            q = streams[i % len(streams)]
+            q.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(q):
+                mfla.trsm_(Bd, Ad, side="left", uplo="lower", trans=trans)
+            torch.cuda.synchronize()
+            assert_bitwise(host(Bd), first[i], f"trsm x{N} {other} RHS {trans} on a new handle")
+            if trans == "N":
+                ref = oracle.trsm(B.copy(), A, 1.0, "left", "lower", "N", "nonunit")
+                assert_bitwise(host(Bd), ref, f"trsm x{N} {other} RHS forward")
+    finally:
+        api._handles[devno] = old
+        assert lib.mfla_handle_destroy(h) == 0
+
+
 @pytest.mark.parametrize("N,nrhs", [(2, 5), (4, 5), (3, 70)])
 @pytest.mark.parametrize("uplo,trans", [("upper", "N"), ("lower", "T")])
 def test_trsm_blocked_backward_within_tolerance(mfla, oracle, N, nrhs, uplo, trans):
