@@ -1190,9 +1190,18 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
             ps_open = us_open = false;
             return e;
         };
+        // Device words 3 and 4 of the handle's flag array: a counter the
+        // update stream advances after each split update's second part, and
+        // an error word. A narrow first part (k_cols) waits on the counter
+        // for the previous panel's second part instead of its stream waiting
+        // on an event (~11 us each even when satisfied); that part was
+        // enqueued before it, on the other stream, so the wait ends.
+        unsigned *la_ctr = reinterpret_cast<unsigned *>(h->flags) + 3;
+        unsigned la_seq = 0;
         if (la) {
             ps = h->la_panel;
             us = h->la_update;
+            HIPCHK(mfk::flag_launch(s, la_ctr, 0));
             HIPCHK(order(s, ps, h->ev_join));
             HIPCHK(order(s, us, h->ev_update2));
         }
@@ -1203,13 +1212,15 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
         // update of columns [c0, c1), rows from c0 down, on stream q; limit:
         // the status limit (nonzero for the parts that may run beside later
         // panels: skipped only when this panel or an earlier one stopped).
-        auto update = [&](hipStream_t q, int64_t pf, int64_t pl, const double *Wh, int64_t c0, int64_t c1, int64_t limit) -> hipError_t {
+        auto narrow = [&](int64_t c0, int64_t c1) { return c1 - c0 <= 8 && c.gemm_tile < 0; };
+        auto update = [&](hipStream_t q, int64_t pf, int64_t pl, const double *Wh, int64_t c0, int64_t c1, int64_t limit,
+                          unsigned wait_value = 0) -> hipError_t {
             if (c0 >= c1) return hipSuccess;
             const int64_t tfirst = pl + 1, tcount = n - tfirst;
             const DV Wn{const_cast<double *>(Wh) + (c0 - tfirst), tcount, 1, tcount * (b + 1)};
-            if (c1 - c0 <= 8 && c.gemm_tile < 0)
+            if (narrow(c0, c1))
                 return mfk::cols_launch_n(N, q, n - c0, c1 - c0, pl - pf + 1, true, sub(a, c0, pf), Wn, sub(a, c0, c0), kMinusOne, kOne, st,
-                                          limit);
+                                          limit, wait_value ? la_ctr : nullptr, wait_value, la_ctr + 1);
             mfk::TileArgs t{};
             t.mode = mfk::MODE_AB; t.net = mf::NET_STD; t.lower = 1; t.beta_zero = 0; t.tile = c.gemm_tile;
             t.row_lo = 0; t.rows = n - c0; t.n = c1 - c0; t.K = pl - pf + 1; t.status_limit = limit;
@@ -1269,10 +1280,12 @@ static int sytrf_run(mfla_handle h, int N, const mfla_matrix &A, const mfla_matr
                 // second part), the others on the update stream
                 HIPCHK(order(pq, ps, h->ev_join));
                 HIPCHK(order(pq, us, h->ev_panel));
-                if (prev_split) HIPCHK(hipStreamWaitEvent(ps, h->ev_update, 0));
-                HIPCHK(update(ps, pfirst, reqlast, Wh, t0c, t1c, 0));
+                const bool dev_wait = prev_split && narrow(t0c, t1c);
+                if (prev_split && !dev_wait) HIPCHK(hipStreamWaitEvent(ps, h->ev_update, 0));
+                HIPCHK(update(ps, pfirst, reqlast, Wh, t0c, t1c, 0, dev_wait ? la_seq : 0));
                 HIPCHK(update(us, pfirst, reqlast, Wh, t1c, t2c, reqlast + 1));
                 HIPCHK(hipEventRecord(h->ev_update, us));
+                HIPCHK(mfk::flag_launch(us, la_ctr, ++la_seq));
                 HIPCHK(update(us, pfirst, reqlast, Wh, t2c, n, reqlast + 1));
                 ps_open = us_open = true;
             }

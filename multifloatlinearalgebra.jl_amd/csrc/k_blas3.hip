@@ -313,10 +313,20 @@ hipError_t tile_launch(hipStream_t s, const TileArgs &t, const DV &A, const DV &
 // chain over k ascending from zero, then alpha * acc + beta * C.
 template <int N, bool LOWER>
 __global__ __launch_bounds__(256) void k_cols(int64_t rows, int64_t ncols, int64_t K, DV A, DV B, DV C, S<N> alpha, S<N> beta,
-                                              const Status *status, int64_t status_limit) {
+                                              const Status *status, int64_t status_limit, const unsigned *wait_on, unsigned wait_value,
+                                              unsigned *err) {
+    const bool ok = wait_on == nullptr || wait_flag(wait_on, wait_value);
     if (status && status_skips(status, status_limit)) return;
     const int64_t i = (int64_t)blockIdx.x * 32 + (threadIdx.x & 31), j = threadIdx.x >> 5;
     if (i >= rows || j >= ncols || (LOWER && i < j)) return;
+    if (!ok) {
+        mf::V<N> bad;
+#pragma unroll
+        for (int l = 0; l < N; ++l) bad.x[l] = __builtin_nan("");
+        stv<N>(C, i, j, bad);
+        __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
     mf::V<N> acc = mf::zero<N>();
     mf::V<N> a = K > 0 ? ld<N>(A, i, 0) : acc, b = K > 0 ? ld<N>(B, 0, j) : acc;
     for (int64_t k = 0; k < K; ++k) {
@@ -331,7 +341,9 @@ __global__ __launch_bounds__(256) void k_cols(int64_t rows, int64_t ncols, int64
 }
 
 hipError_t cols_launch_n(int N, hipStream_t s, int64_t rows, int64_t ncols, int64_t K, bool lower, const DV &A, const DV &B, const DV &C,
-                         const double *alpha, const double *beta, const Status *st, int64_t status_limit) {
+                         const double *alpha, const double *beta, const Status *st, int64_t status_limit, const unsigned *wait_on,
+                         unsigned wait_value, unsigned *err) {
+    if (wait_on && !err) return hipErrorInvalidValue;
     if (rows <= 0 || ncols <= 0) return hipSuccess;
     if (ncols > 8) return hipErrorInvalidValue;
     auto launch = [&](auto ntag) -> hipError_t {
@@ -339,8 +351,12 @@ hipError_t cols_launch_n(int N, hipStream_t s, int64_t rows, int64_t ncols, int6
         S<NN> al, be;
         for (int l = 0; l < NN; ++l) { al.x[l] = alpha[l]; be.x[l] = beta[l]; }
         const dim3 grid((unsigned)cdiv(rows, 32));
-        if (lower) hipLaunchKernelGGL((k_cols<NN, true>), grid, dim3(256), 0, s, rows, ncols, K, A, B, C, al, be, st, status_limit);
-        else hipLaunchKernelGGL((k_cols<NN, false>), grid, dim3(256), 0, s, rows, ncols, K, A, B, C, al, be, st, status_limit);
+        if (lower)
+            hipLaunchKernelGGL((k_cols<NN, true>), grid, dim3(256), 0, s, rows, ncols, K, A, B, C, al, be, st, status_limit, wait_on, wait_value,
+                               err);
+        else
+            hipLaunchKernelGGL((k_cols<NN, false>), grid, dim3(256), 0, s, rows, ncols, K, A, B, C, al, be, st, status_limit, wait_on,
+                               wait_value, err);
         return hipGetLastError();
     };
     switch (N) {

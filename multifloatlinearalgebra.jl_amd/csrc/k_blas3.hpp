@@ -29,9 +29,13 @@ hipError_t tile_launch_n(int N, hipStream_t s, const TileArgs &t, const DV &A, c
 // C(i, j) = alpha * (sum over k of A(i, k) B(k, j), MAC chain from zero) +
 // beta * C(i, j) for i < rows, j < ncols <= 8 (lower: i >= j only): the
 // per-element operations of a MODE_AB NET_STD tile launch with beta read, in
-// a layout for narrow outputs (32 x 8 per workgroup).
+// a layout for narrow outputs (32 x 8 per workgroup). With wait_on the
+// kernel first waits (bounded) for *wait_on >= wait_value, a counter advanced
+// by work enqueued before it on another stream; past the bound it writes NaN
+// and *err = 1.
 hipError_t cols_launch_n(int N, hipStream_t s, int64_t rows, int64_t ncols, int64_t K, bool lower, const DV &A, const DV &B, const DV &C,
-                         const double *alpha, const double *beta, const Status *st, int64_t status_limit);
+                         const double *alpha, const double *beta, const Status *st, int64_t status_limit,
+                         const unsigned *wait_on = nullptr, unsigned wait_value = 0, unsigned *err = nullptr);
 
 // The output tile a launch over rows x n outputs takes (requested < 0: the
 // automatic choice): its index 0..3 as the launcher resolves it, and its

@@ -81,18 +81,6 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
 // second coefficient in registers would spill under the 64-VGPR bound. The
 // loads use a row index clamped into the block (lanes that do not use the
 // value read an in-bounds entry they ignore).
-// Waits until *flag >= value (written by k_trsm_flag on another stream);
-// false after about 2^24 polls.
-__device__ inline bool wait_flag(const unsigned *flag, unsigned value) {
-    unsigned spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < value) {
-        __builtin_amdgcn_s_sleep(4);
-        if (++spins > (1u << 24)) return false;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return true;
-}
-
 template <int G>
 __device__ __forceinline__ double group_bcast(double x, int grp, int s) {
     if constexpr (G == 64) {
@@ -268,6 +256,11 @@ __global__ void k_trsm_flag(unsigned *flag, unsigned value) {
     if (threadIdx.x != 0) return;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+hipError_t flag_launch(hipStream_t s, unsigned *flag, unsigned value) {
+    hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, s, flag, value);
+    return hipGetLastError();
 }
 
 // Exact MFLA order for backward substitution: one thread per right-hand side.

@@ -30,6 +30,10 @@ struct TrsmArgs {
     int stream_next;
 };
 
+// *flag = value once the work enqueued before it on stream s is done (a
+// device-scope release; kernels on other streams wait with wait_flag).
+hipError_t flag_launch(hipStream_t s, unsigned *flag, unsigned value);
+
 // Solves in place on the solution view X(p, c) with coefficient view CA(p, q)
 // and diagonal view D(p, p).
 hipError_t trsm_solve(int N, hipStream_t s, const TrsmArgs &a, const DV &X, const DV &CA, const DV &D, const Status *st);

@@ -76,6 +76,18 @@ __device__ inline bool status_skips(const Status *st, int64_t limit) {
     return true;
 }
 
+// Waits until *flag >= value (a device counter another stream's kernel
+// advances, flag_launch); false after about 2^24 polls.
+__device__ inline bool wait_flag(const unsigned *flag, unsigned value) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < value) {
+        __builtin_amdgcn_s_sleep(4);
+        if (++spins > (1u << 24)) return false;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace mfk
