@@ -708,14 +708,14 @@ int mfla_gemm_plan(int N, int64_t m, int64_t k, int64_t n, const mfla_config *co
     if (p.strategy == MFLA_GEMM_OZAKI) {
         p.tile = -1; p.launches = 0; p.workgroups = 0;
     } else {
-        p.tile = mfk::tile_geometry(N, main_rows, nn, false, c.gemm_tile, &tr, &tc);
+        p.tile = mfk::tile_geometry(N, main_rows, nn, k, false, c.gemm_tile, &tr, &tc);
         p.tile_rows = tr; p.tile_cols = tc;
         p.launches = (main_rows > 0 && nn > 0) ? 1 : 0;
         p.workgroups = mfk::cdiv(main_rows, tr) * mfk::cdiv(nn, tc);
         p.remainder_rows = mm - main_rows;
         if (p.remainder_rows > 0 && nn > 0) {
             int rr = 0, rc = 0;
-            mfk::tile_geometry(N, p.remainder_rows, nn, false, c.gemm_tile, &rr, &rc);
+            mfk::tile_geometry(N, p.remainder_rows, nn, k, false, c.gemm_tile, &rr, &rc);
             p.launches += 1;
             p.workgroups += mfk::cdiv(p.remainder_rows, rr) * mfk::cdiv(nn, rc);
         }

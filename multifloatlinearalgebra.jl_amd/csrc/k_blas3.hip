@@ -262,20 +262,24 @@ static hipError_t pick(hipStream_t s, int tile, int64_t rows, int64_t n, int64_t
     return go<N, NET, 1, 1, 16, MODE, LOWER>(s, rows, n, K, A, B, C, al, be, bz, row_lo, st, lim, swap);
 }
 
-static int auto_tile(int N, int64_t rows, int64_t n, bool lower) {
+static int auto_tile(int N, int64_t rows, int64_t n, int64_t K, bool lower) {
     // Lower-triangle outputs (syrk!, gemmt!, the Cholesky / LDLT trailing
     // updates): 16x16 tiles. The triangle's tile count is rarely close to a
     // whole number of resident-workgroup rounds, and the small tiles shrink
-    // the idle last round (x4 syrk n = 4096, k = 1024: 67 -> 60 ms).
+    // the idle last round (x4 syrk n = 4096, k = 1024: 67 -> 60 ms; x4
+    // Cholesky n = 4096, K = 16 updates: 115.6 ms with 32x32 tiles, 104.7 ms).
     if (lower) return 0;
+    // Short reductions (the blocked factorizations' K = 16 trailing
+    // updates): 16x16 tiles as well (x4 lu! n = 4096: 196.4 -> 193.9 ms).
+    if (K <= 16) return 0;
     // 64x64 tiles for up to two limbs, 32x32 tiles otherwise, once they give
     // at least ~2 workgroups per CU (256 CUs).
     if (N <= 2 && cdiv(rows, 64) * cdiv(n, 64) >= 512) return 3;
     return (cdiv(rows, 32) * cdiv(n, 32) >= 512) ? 1 : 0;
 }
 
-int tile_geometry(int N, int64_t rows, int64_t n, bool lower, int requested, int *tile_rows, int *tile_cols) {
-    int t = requested >= 0 ? requested : auto_tile(N, rows, n, lower);
+int tile_geometry(int N, int64_t rows, int64_t n, int64_t K, bool lower, int requested, int *tile_rows, int *tile_cols) {
+    int t = requested >= 0 ? requested : auto_tile(N, rows, n, K, lower);
     if (t < 0 || t > 3) t = 0;                 // pick(): values outside 0..3 run the 16x16 tile
     if (t == 3 && N > 2) t = 2;                // 64x64 exists for 1-2 limbs only
     static const int tm[4] = {1, 2, 2, 4}, tn[4] = {1, 2, 4, 4};
@@ -290,7 +294,7 @@ hipError_t tile_launch(hipStream_t s, const TileArgs &t, const DV &A, const DV &
     if (t.rows <= 0 || t.n <= 0) return hipSuccess;
     S<N> al, be;
     for (int l = 0; l < N; ++l) { al.x[l] = alpha ? alpha[l] : 0.0; be.x[l] = beta ? beta[l] : 0.0; }
-    const int tile = t.tile >= 0 ? t.tile : auto_tile(N, t.rows, t.n, t.lower != 0);
+    const int tile = t.tile >= 0 ? t.tile : auto_tile(N, t.rows, t.n, t.K, t.lower != 0);
     const int bz = t.beta_zero;
     if (t.mode == MODE_SUB) {
         if (t.lower) return pick<N, mf::NET_STD, MODE_SUB, true>(s, tile, t.rows, t.n, t.K, A, B, C, al, be, bz, t.row_lo, st, t.status_limit, t.swap_product);

@@ -37,9 +37,9 @@ hipError_t cols_launch_n(int N, hipStream_t s, int64_t rows, int64_t ncols, int6
                          const double *alpha, const double *beta, const Status *st, int64_t status_limit,
                          const unsigned *wait_on = nullptr, unsigned wait_value = 0, unsigned *err = nullptr);
 
-// The output tile a launch over rows x n outputs takes (requested < 0: the
-// automatic choice): its index 0..3 as the launcher resolves it, and its
-// rows and columns per 256-thread workgroup.
-int tile_geometry(int N, int64_t rows, int64_t n, bool lower, int requested, int *tile_rows, int *tile_cols);
+// The output tile a launch over rows x n outputs with reduction length K
+// takes (requested < 0: the automatic choice): its index 0..3 as the
+// launcher resolves it, and its rows and columns per 256-thread workgroup.
+int tile_geometry(int N, int64_t rows, int64_t n, int64_t K, bool lower, int requested, int *tile_rows, int *tile_cols);
 
 }  // namespace mfk
