@@ -384,7 +384,11 @@ hipError_t trsm_core(mfla_handle h, int N, int side, int uplo, int trans, int di
         // waiting on events (4096 x 256: x4 forward 19.9 -> 19.6 ms, backward
         // 20.3 -> 19.7 ms; 64: x4 backward 14.0 -> 13.5 ms). With 1024 the
         // waiting workgroups hold slots the updates need (x4 61.5 -> 64.7 ms).
-        if (a.stream_next || a.nrhs <= kTrsmWaitMaxRhs) a.far_flag = reinterpret_cast<unsigned *>(h->flags) + 1;
+        if (a.stream_next || a.nrhs <= kTrsmWaitMaxRhs) {
+            a.far_flag = reinterpret_cast<unsigned *>(h->flags) + 1;   // words 1-2
+            a.start_sig = reinterpret_cast<unsigned *>(h->flags) + 5;  // word 5
+            a.dedicated = split ? 1 : 0;                                // CU-masked streams: queues of their own
+        }
         HIPRET_(hipEventRecord(h->ev_join, h->stream));
         HIPRET_(hipStreamWaitEvent(a.side, h->ev_join, 0));
         if (split) {

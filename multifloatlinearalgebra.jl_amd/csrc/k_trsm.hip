@@ -81,6 +81,15 @@ __global__ __launch_bounds__(TB *TC) void k_trsm_diag(int64_t p0, int nbk, int64
 // second coefficient in registers would spill under the 64-VGPR bound. The
 // loads use a row index clamped into the block (lanes that do not use the
 // value read an in-bounds entry they ignore).
+// Start mark for the side stream: *start_sig = start_value from thread 0 of
+// every workgroup (the same value; any one suffices). A kernel starts only
+// after the kernel before it on its stream has ended and released its
+// writes, so the mark says those writes are done; the side stream's wait
+// kernel acquires before the update that reads them.
+__device__ inline void mark_start(unsigned *start_sig, unsigned start_value) {
+    if (start_sig && threadIdx.x == 0) __hip_atomic_store(start_sig, start_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int G>
 __device__ __forceinline__ double group_bcast(double x, int grp, int s) {
     if constexpr (G == 64) {
@@ -106,8 +115,9 @@ __device__ __forceinline__ double group_bcast(double x, int grp, int s) {
 template <int N, bool FWD, int G>
 __global__ __launch_bounds__(256, N == 4 ? 5 : 8) void k_trsm_diag_wave(int64_t p0, int nbk, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap,
                                                            const Status *st, int64_t lim, int64_t q0, int K, const unsigned *far_flag,
-                                                           unsigned far_wait, unsigned *err) {
+                                                           unsigned far_wait, unsigned *err, unsigned *start_sig, unsigned start_value) {
     __builtin_amdgcn_s_setprio(3);
+    mark_start(start_sig, start_value);
     if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
     const bool ok = far_flag == nullptr || wait_flag(far_flag, far_wait);
     __shared__ double dg[G * N];
@@ -182,7 +192,9 @@ constexpr int kStreamRhs = 4;
 template <int N>
 __global__ __launch_bounds__(512, N == 4 ? 3 : 4) void k_trsm_diag_stream(int64_t p0, int nbk, int64_t p1, int nn, int64_t nrhs, DV X,
                                                                         DV CA, DV D, int unit, int swap, const Status *st, int64_t lim,
-                                                                        const unsigned *far_flag, unsigned far_wait, unsigned *err) {
+                                                                        const unsigned *far_flag, unsigned far_wait, unsigned *err,
+                                                                        unsigned *start_sig, unsigned start_value) {
+    mark_start(start_sig, start_value);
     if (st && st->info != 0 && (lim == 0 || st->info < 0 || st->info <= lim)) return;
     __shared__ double dg[64 * N];
     __shared__ double xs[kStreamRhs * 64 * N];
@@ -263,6 +275,24 @@ hipError_t flag_launch(hipStream_t s, unsigned *flag, unsigned value) {
     return hipGetLastError();
 }
 
+// On the side stream before a block's far update: waits (bounded) for the
+// start mark of the diagonal kernel after that block's (*start_sig >=
+// value). Past the bound it sets *err = 1 and writes NaN over the rows the
+// update writes, so the failure shows in the solution.
+template <int N>
+__global__ __launch_bounds__(256) void k_trsm_side_wait(const unsigned *start_sig, unsigned value, unsigned *err, DV X, int64_t r0,
+                                                        int64_t rows, int64_t nrhs) {
+    __shared__ int ok;
+    if (threadIdx.x == 0) ok = wait_flag(start_sig, value) ? 1 : 0;
+    __syncthreads();
+    if (ok) return;
+    if (threadIdx.x == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    mf::V<N> bad;
+#pragma unroll
+    for (int l = 0; l < N; ++l) bad.x[l] = __builtin_nan("");
+    for (int64_t e = threadIdx.x; e < rows * nrhs; e += blockDim.x) stv<N>(X, r0 + e % rows, e / rows, bad);
+}
+
 // Exact MFLA order for backward substitution: one thread per right-hand side.
 template <int N>
 __global__ void k_trsm_seq_bwd(int64_t n, int64_t nrhs, DV X, DV CA, DV D, int unit, int swap, const Status *st, int64_t lim) {
@@ -283,19 +313,19 @@ static DV sub(const DV &v, int64_t i, int64_t j) { return DV{v.p + i * v.rs + j 
 template <int N>
 static hipError_t diag(hipStream_t s, bool fwd, int64_t p0, int nbk, int64_t nrhs, const DV &X, const DV &CA, const DV &D, int unit,
                        int swap, const Status *st, int64_t lim, int64_t q0 = 0, int K = 0, const unsigned *far_flag = nullptr,
-                       unsigned far_wait = 0, unsigned *err = nullptr) {
+                       unsigned far_wait = 0, unsigned *err = nullptr, unsigned *start_sig = nullptr, unsigned start_value = 0) {
     // one group of G lanes per right-hand side, 4 waves per workgroup
     auto launch = [&](auto gtag) {
         constexpr int G = decltype(gtag)::value;
         const dim3 grid((unsigned)cdiv(cdiv(nrhs, 64 / G), 4)), block(256);
         if (fwd)
             hipLaunchKernelGGL((k_trsm_diag_wave<N, true, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim, q0, K, far_flag,
-                               far_wait, err);
+                               far_wait, err, start_sig, start_value);
         else
             hipLaunchKernelGGL((k_trsm_diag_wave<N, false, G>), grid, block, 0, s, p0, nbk, nrhs, X, CA, D, unit, swap, st, lim, q0, K, far_flag,
-                               far_wait, err);
+                               far_wait, err, start_sig, start_value);
     };
-    if (far_flag && nbk > TB) return hipErrorInvalidValue;
+    if ((far_flag || start_sig) && nbk > TB) return hipErrorInvalidValue;
     if (nbk <= 16) launch(std::integral_constant<int, 16>());
     else if (nbk <= 32) launch(std::integral_constant<int, 32>());
     else if (nbk <= TB) launch(std::integral_constant<int, 64>());
@@ -337,55 +367,123 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
     // every element still takes the blocks' terms in MFLA's block order, and
     // each block's terms in ascending q.
     const bool ahead = a.side != nullptr && n > 3 * TB;
-    if (ahead && a.forward && a.far_flag && a.stream_next) {
-        // Forward look-ahead with the next block's rows streamed
-        // (k_trsm_diag_stream): block b's kernel applies its terms to block
-        // b + 1's rows once the side stream's update from block b - 1 has
-        // reached them; the side stream then updates the rows beyond block
-        // b + 1 and writes *far_flag = b + 1. No diagonal kernel waits for
-        // the side stream, and none has previous-block terms of its own.
+    if (ahead && a.far_flag) {
+        // With device counters (far_flag, start_sig): the side stream writes
+        // *far_flag = b + 1 after block b's far update, and block b's kernel
+        // waits on it (for the update that last reaches its rows) instead of
+        // the stream waiting on an event (~11 us each even when satisfied).
+        // Block b's far update waits on the side stream for the start mark
+        // of block b + 1's kernel (*start_sig >= b + 1: block b is solved)
+        // instead of an event recorded between the two diagonal kernels
+        // (~6.7 us each on the chain); the host enqueues it after block
+        // b + 1's kernel. Then every counter a kernel waits for is advanced by
+        // work enqueued before it, except in the streamed order below, where
+        // block b + 1's kernel waits for block b's far update: that order
+        // defers the far updates only between streams with hardware queues
+        // of their own (dedicated), and otherwise enqueues each far update
+        // after an event recorded behind its block's kernel.
+        if (!a.start_sig) return hipErrorInvalidValue;
         hipError_t r;
-        hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, s, a.far_flag, 0u);
-        if ((r = hipGetLastError()) != hipSuccess) return r;
-        bool pending = false;
+        if ((r = flag_launch(s, a.far_flag, 0u)) != hipSuccess) return r;
+        if ((r = flag_launch(s, a.start_sig, 0u)) != hipSuccess) return r;
+        // the side stream after the resets (it waited for the caller's work)
+        if ((r = hipEventRecord(a.ev_main, s)) != hipSuccess) return r;
+        if ((r = hipStreamWaitEvent(a.side, a.ev_main, 0)) != hipSuccess) return r;
+        struct Far { int64_t r0, rows, q0; int K; unsigned b; };
+        Far pend{0, 0, 0, 0, 0};
+        bool any_far = false;
+        auto far_update = [&](const Far &f) -> hipError_t {
+            hipError_t q;
+            hipLaunchKernelGGL(k_trsm_side_wait<N>, dim3(1), dim3(256), 0, a.side, a.start_sig, f.b + 1, a.far_flag + 1, X, f.r0, f.rows, nrhs);
+            if ((q = hipGetLastError()) != hipSuccess) return q;
+            if ((q = update(a.side, f.r0, f.rows, f.q0, f.K)) != hipSuccess) return q;
+            any_far = true;
+            return flag_launch(a.side, a.far_flag, f.b + 1);
+        };
         unsigned b = 0;
-        for (int64_t p0 = 0; p0 < n; p0 += TB, ++b) {
-            const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
-            const int64_t p1 = p0 + nbk;
-            const int nn = (int)((n - p1) < TB ? (n - p1) : TB);
-            if (nn > 0) {
-                hipLaunchKernelGGL(k_trsm_diag_stream<N>, dim3((unsigned)cdiv(nrhs, kStreamRhs)), dim3(512), 0, s, p0, nbk, p1, nn, nrhs, X, CA, D,
-                                   a.unit, a.swap_product, st, a.status_limit, a.far_flag, b, a.far_flag + 1);
-                if ((r = hipGetLastError()) != hipSuccess) return r;
-            } else if ((r = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit)) != hipSuccess) {
-                return r;
+        bool had_far[2] = {false, false};    // blocks b - 2, b - 1 (by parity)
+        if (a.forward && a.stream_next) {
+            // The next block's rows streamed (k_trsm_diag_stream): block b's
+            // kernel applies its terms to block b + 1's rows once the far
+            // update from block b - 1 has reached them (*far_flag >= b); no
+            // diagonal kernel has previous-block terms of its own.
+            for (int64_t p0 = 0; p0 < n; p0 += TB, ++b) {
+                const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
+                const int64_t p1 = p0 + nbk;
+                const int nn = (int)((n - p1) < TB ? (n - p1) : TB);
+                if (nn > 0) {
+                    hipLaunchKernelGGL(k_trsm_diag_stream<N>, dim3((unsigned)cdiv(nrhs, kStreamRhs)), dim3(512), 0, s, p0, nbk, p1, nn, nrhs, X,
+                                       CA, D, a.unit, a.swap_product, st, a.status_limit, a.far_flag, b, a.far_flag + 1, a.start_sig, b);
+                    if ((r = hipGetLastError()) != hipSuccess) return r;
+                } else if ((r = diag<N>(s, true, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit, 0, 0, nullptr, 0, nullptr,
+                                        a.start_sig, b)) != hipSuccess) {
+                    return r;
+                }
+                if (pend.rows > 0) {
+                    if ((r = far_update(pend)) != hipSuccess) return r;
+                    pend.rows = 0;
+                }
+                const int64_t far0 = p1 + nn;
+                if (far0 < n) {
+                    pend = Far{far0, n - far0, p0, nbk, b};
+                    if (!a.dedicated) {
+                        // block b solved: the start mark is not awaited
+                        if ((r = flag_launch(s, a.start_sig, b + 1)) != hipSuccess) return r;
+                        if ((r = far_update(pend)) != hipSuccess) return r;
+                        pend.rows = 0;
+                    }
+                }
             }
-            const int64_t far0 = p1 + nn;
-            if (far0 < n) {
-                if ((r = hipEventRecord(a.ev_main, s)) != hipSuccess) return r;
-                if ((r = hipStreamWaitEvent(a.side, a.ev_main, 0)) != hipSuccess) return r;
-                if ((r = update(a.side, far0, n - far0, p0, nbk)) != hipSuccess) return r;
-                hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, a.side, a.far_flag, b + 1);
-                if ((r = hipGetLastError()) != hipSuccess) return r;
-                pending = true;
+        } else {
+            // Block b's kernel takes block b - 1's terms for its rows in its
+            // prologue (K > 0) and waits for block b - 2's far update.
+            int64_t prev0 = 0;
+            int prevK = 0;
+            auto block_step = [&](int64_t p0, int nbk, int64_t far0, int64_t nfar) -> hipError_t {
+                hipError_t q;
+                const unsigned wait = (b >= 2 && had_far[b & 1]) ? b - 1 : 0;
+                if ((q = diag<N>(s, a.forward != 0, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit, prev0, prevK,
+                                 wait ? a.far_flag : nullptr, wait, a.far_flag + 1, a.start_sig, b)) != hipSuccess)
+                    return q;
+                if (pend.rows > 0) {
+                    if ((q = far_update(pend)) != hipSuccess) return q;
+                    pend.rows = 0;
+                }
+                had_far[b & 1] = nfar > 0;
+                if (nfar > 0) pend = Far{far0, nfar, p0, nbk, b};
+                prev0 = p0;
+                prevK = nbk;
+                ++b;
+                return hipSuccess;
+            };
+            if (a.forward) {
+                for (int64_t p0 = 0; p0 < n; p0 += TB) {
+                    const int nbk = (int)((n - p0) < TB ? (n - p0) : TB);
+                    const int64_t p1 = p0 + nbk, nn = (n - p1) < TB ? (n - p1) : TB;
+                    if ((r = block_step(p0, nbk, p1 + nn, n - p1 - nn)) != hipSuccess) return r;
+                }
+            } else {
+                for (int64_t p1 = n; p1 > 0; p1 -= TB) {
+                    const int64_t p0 = p1 > TB ? p1 - TB : 0;
+                    const int64_t nn = p0 < TB ? p0 : TB;
+                    if ((r = block_step(p0, (int)(p1 - p0), 0, p0 - nn)) != hipSuccess) return r;
+                }
             }
         }
-        if (pending) {
-            if ((e = hipEventRecord(a.ev_side, a.side)) != hipSuccess) return e;
-            if ((e = hipStreamWaitEvent(s, a.ev_side, 0)) != hipSuccess) return e;
+        if (pend.rows > 0) {
+            // no kernel after the last one with a far update: mark it here
+            if ((r = flag_launch(s, a.start_sig, b)) != hipSuccess) return r;
+            if ((r = far_update(pend)) != hipSuccess) return r;
+        }
+        if (any_far) {
+            if ((r = hipEventRecord(a.ev_side, a.side)) != hipSuccess) return r;
+            if ((r = hipStreamWaitEvent(s, a.ev_side, 0)) != hipSuccess) return r;
         }
         return hipSuccess;
     }
     if (ahead) {
-        // With a device counter (far_flag): the side stream writes
-        // *far_flag = b + 1 after block b's update, and block b's kernel waits
-        // for block b - 2's in its prologue instead of an event wait on the
-        // stream (each of which costs the stream ~11 us even when satisfied).
-        const bool dev_wait = a.far_flag != nullptr;
-        if (dev_wait) {
-            hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, s, a.far_flag, 0u);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-        }
+        // Event waits: block b's kernel after block b - 2's far update on the
+        // side stream, each far update after its block's kernel.
         hipEvent_t far_ev[2] = {a.ev_side, a.ev_side2};
         bool far_pending[2] = {false, false};
         int64_t prev0 = 0;
@@ -394,23 +492,17 @@ static hipError_t solve_t(hipStream_t s, const TrsmArgs &a, const DV &X, const D
         auto block_step = [&](int64_t p0, int nbk, int64_t far0, int64_t nfar) -> hipError_t {
             hipError_t r;
             const int slot = (int)(b & 1);
-            unsigned wait = 0;
             if (far_pending[slot]) {   // the update from block b - 2
-                if (dev_wait) wait = (unsigned)(b - 1);
-                else if ((r = hipStreamWaitEvent(s, far_ev[slot], 0)) != hipSuccess) return r;
+                if ((r = hipStreamWaitEvent(s, far_ev[slot], 0)) != hipSuccess) return r;
                 far_pending[slot] = false;
             }
-            if ((r = diag<N>(s, a.forward != 0, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit, prev0, prevK,
-                             dev_wait ? a.far_flag : nullptr, wait, a.far_flag ? a.far_flag + 1 : nullptr)) != hipSuccess)
+            if ((r = diag<N>(s, a.forward != 0, p0, nbk, nrhs, X, CA, D, a.unit, a.swap_product, st, a.status_limit, prev0, prevK)) !=
+                hipSuccess)
                 return r;
             if (nfar > 0) {
                 if ((r = hipEventRecord(a.ev_main, s)) != hipSuccess) return r;
                 if ((r = hipStreamWaitEvent(a.side, a.ev_main, 0)) != hipSuccess) return r;
                 if ((r = update(a.side, far0, nfar, p0, nbk)) != hipSuccess) return r;
-                if (dev_wait) {
-                    hipLaunchKernelGGL(k_trsm_flag, dim3(1), dim3(64), 0, a.side, a.far_flag, (unsigned)(b + 1));
-                    if ((r = hipGetLastError()) != hipSuccess) return r;
-                }
                 if ((r = hipEventRecord(far_ev[slot], a.side)) != hipSuccess) return r;
                 far_pending[slot] = true;
             }

@@ -28,6 +28,12 @@ struct TrsmArgs {
     // terms to the next block's rows as they are solved (the same operations
     // and order as applying them after).
     int stream_next;
+    // With far_flag: a device word each diagonal kernel marks at its start
+    // (block b's far update waits for block b + 1's mark on the side stream).
+    unsigned *start_sig;
+    // The caller's stream and `side` have hardware queues of their own (so
+    // a kernel on one may wait for work enqueued after it on the other).
+    int dedicated;
 };
 
 // *flag = value once the work enqueued before it on stream s is done (a
